@@ -1,0 +1,209 @@
+"""Benchmark: plain fusion at scale on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain_scale]
+
+A step is one depth frame through the fusion path (bilateral filter ->
+block allocation -> directional TSDF integration) of the 1280x720 plain-
+at-scale workload: 8x8x3 m room with 150 seeded primitives, 500 frames on a
+random walk (0.02 m / +-1 deg per frame), voxel 5 mm, truncation 2 cm, six
+directional bins x 3 fp32 slots per voxel, 5x5 bilateral (sigma_s 4.5 px,
+sigma_r 30 mm).  Frames are rendered on the GPU and resident in HBM before
+the timed region; W warm-up frames are integrated untimed, then K frames are
+timed (W + K <= 500, in sequence).  After the timed region one full
+marching-cubes extraction is timed separately.
+
+value = voxel updates / s over the timed frames (an update is a voxel that
+reaches the layer update with weight > 0 -- voxel_hash_tsdf.h:1031-1055).
+roofline: integrate kernel, algorithmic bytes = 96 B per update (3 slots x
+16 B read + written), HIP events on the library's stream.
+cpu_baseline: the reference's CPU path (oracle/_ref, compiled from its
+sources) or, if absent, the restated oracle, on frame 0 of the same workload,
+one core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = 96
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, frame0_u16, gpu_updates_frame0: int) -> dict | None:
+    """Times the CPU path on frame 0 (filter + allocation + integration)."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    import voxelhashmt_amd as vhm
+    depth = frame0_u16.astype(np.float32) * np.float32(0.001)
+    pose = vhm.pose_array(w.poses[0])
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    if O.ref_available() and w.filter_sigma_spatial == 1.5 and w.filter_sigma_range == 0.0012:
+        kind = "reference"
+    else:
+        kind = "port"
+    vp = vhm.default_params(w.voxel_size, w.truncation, filter_sigma_spatial=w.filter_sigma_spatial,
+                            filter_sigma_range=w.filter_sigma_range)
+    ov = O.OracleVolume(O.oracle_params(vp))
+    t0 = time.perf_counter()
+    if kind == "reference":
+        rv = O.RefVolume(w.voxel_size, w.truncation)
+        rv.integrate(depth, w.K, pose)
+        updates = gpu_updates_frame0
+    else:
+        ov.integrate(depth, w.K, pose)
+        updates = ov.updates()
+    dt = time.perf_counter() - t0
+    return {"value": updates / dt, "unit": "voxel updates/s", "cores": 1, "kind": kind,
+            "sample": f"frame 0 of {w.name} ({w.K.width}x{w.K.height}): filter + allocation + integration, "
+                      f"{updates} updates in {dt:.1f} s on 1 host core (of {os.cpu_count()})",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=480)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="plain_scale", choices=["plain_scale", "plain_small"])
+    ap.add_argument("--max-blocks", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; the sharded design is described in "
+                                   "DESIGN.md (--gpus must be 1)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import voxelhashmt_amd as vhm
+    from voxelhashmt_amd import synth
+
+    w = synth.plain_scale() if args.workload == "plain_scale" else synth.plain_small()
+    n_total = len(w.poses)
+    W = min(args.warmup, n_total - 1)
+    K = min(args.steps, n_total - W)
+    ok, msg = vhm.device_available()
+    if not ok:
+        raise SystemExit(f"no GPU: {msg}")
+    log(f"[bench] {msg}; workload {w.name}, warmup {W}, steps {K}")
+
+    dev = torch.device("cuda:0")
+    frames = torch.empty((W + K, w.K.height, w.K.width), dtype=torch.int16, device=dev)
+    synth.render(w.scene, w.K, w.poses[: W + K], w.noise, 0, frames)
+    torch.cuda.synchronize()
+    log(f"[bench] rendered {W + K} frames into HBM")
+
+    max_blocks = args.max_blocks or w.max_blocks
+    vp = vhm.default_params(w.voxel_size, w.truncation, filter_sigma_spatial=w.filter_sigma_spatial,
+                            filter_sigma_range=w.filter_sigma_range, hash_log2_buckets=w.hash_log2_buckets,
+                            max_blocks=max_blocks, max_width=w.K.width, max_height=w.K.height)
+    vol = vhm.Volume(vp)
+    updates_frame0 = None
+    for i in range(W):
+        vol.integrate(frames[i], w.K, w.poses[i])
+        if i == 0:
+            updates_frame0 = vol.stats()["updates"]
+    vol.synchronize()
+    s0 = vol.stats()
+    log(f"[bench] warmup done: blocks={vol.block_count()}")
+
+    vol.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(W, W + K):
+        vol.integrate(frames[i], w.K, w.poses[i])
+    vol.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s1 = vol.stats()
+    kt = vol.kernel_times()
+    vol.set_timing(False)
+    updates = s1["updates"] - s0["updates"]
+    visited = s1["visited"] - s0["visited"]
+    blocks = vol.block_count()
+    log(f"[bench] timed {K} frames in {elapsed * 1e3:.1f} ms; updates={updates} blocks={blocks}")
+
+    mesh = {}
+    if not args.no_mesh:
+        vol.set_timing(True)
+        t = time.perf_counter()
+        tris = vol.extract_mesh()
+        mesh_s = time.perf_counter() - t
+        mk = vol.kernel_times()["mesh"]
+        vol.set_timing(False)
+        mesh = {"mesh_ms": mesh_s * 1e3, "mesh_gpu_ms": mk["ms"], "triangles": int(len(tris))}
+        log(f"[bench] mesh: {len(tris)} triangles in {mesh_s * 1e3:.1f} ms")
+
+    integ = kt["integrate"]
+    avg_ms = integ["ms"] / max(integ["launches"], 1)
+    achieved = BYTES_PER_UPDATE * updates / max(integ["ms"] * 1e-3, 1e-12) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "integrate_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "integrateKernel", "avg_launch_ms": round(avg_ms, 4),
+                "updates_per_launch": updates / max(integ["launches"], 1)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        f0 = synth.render(w.scene, w.K, w.poses[:1], w.noise)[0]
+        if updates_frame0 is None:
+            with vhm.Volume(vp) as v2:
+                v2.integrate(f0, w.K, w.poses[0])
+                updates_frame0 = v2.stats()["updates"]
+        log("[bench] timing the CPU baseline on frame 0 ...")
+        cpu = cpu_baseline(w, f0, updates_frame0)
+        log(f"[bench] cpu baseline: {cpu['value']:.0f} updates/s ({cpu['seconds']:.1f} s)")
+
+    out = {
+        "metric": "voxel updates/s (% HBM peak)",
+        "value": updates / elapsed,
+        "unit": "voxel updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded analytic scene, GPU-rendered uint16 depth resident in HBM)",
+        "config": {"workload": w.name, "frames": K, "resolution": f"{w.K.width}x{w.K.height}",
+                   "voxel_size_m": w.voxel_size, "truncation_m": w.truncation, "directional_bins": 6,
+                   "slots_per_bin": 3, "bilateral": "5x5 sigma_s=4.5px sigma_r=30mm",
+                   "blocks_allocated": blocks, "voxels_visited": visited},
+        "frames_per_s": K / elapsed,
+        "kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **mesh,
+    }
+    if cpu:
+        out["cpu_baseline"] = {k: v for k, v in cpu.items() if k != "seconds"}
+    print(json.dumps(out))
+    vol.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+/*
+ * vhm -- MI355X (gfx950) voxel-hash TSDF fusion: C-ABI of libvhm.so.
+ *
+ * Drop-in GPU backend for the fusion path of HM-ZC/voxelhashmt: depth
+ * bilateral filter -> hash-block allocation -> directional multi-layer TSDF
+ * integration -> multi-layer marching cubes.  Results equal the reference's
+ * CPU path (TsdfIntegrationBackend with ComputeBackendPreference::kCpu) on
+ * identical inputs.
+ *
+ * Conventions
+ *  - Plain C types only: pointers, sizes, POD structs.  No C++ or HIP types.
+ *  - Every entry point returns an int status (VHM_OK == 0).  On failure
+ *    vhm_last_error(ctx) holds "<action> failed: <detail>", the message shape
+ *    of the reference's throwCudaError (cuda_support.h:11-19).
+ *  - One context == one volume (the reference keys its GPU cache by the
+ *    VoxelHashTSDF address, cuda_tsdf_integrator.cu:114-130).  A context owns
+ *    one HIP stream; work on it is ordered and asynchronous.  Device-side
+ *    capacity errors are sticky and are returned by the next vhm_synchronize
+ *    or query call.  A context must not be used from two threads at once.
+ *  - memkind: VHM_HOST for pageable/pinned host memory, VHM_DEVICE for
+ *    device memory on the context's GPU.
+ *  - Block keys are int32 triples (x, y, z) in block units, as BlockKey
+ *    (voxel_hash_tsdf.h:33-41).  Triangles are 9 floats (v0, v1, v2), as
+ *    Triangle (marching_cubes.h:11-15).
+ */
+#ifndef VHM_H_
+#define VHM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum vhm_status {
+  VHM_OK = 0,
+  VHM_ERR_INVALID_ARGUMENT = 1,
+  VHM_ERR_HIP = 2,
+  /* Persistent hash table or block pool full.  The reference's analogue is
+   * CudaActiveBlockAllocationFallback (cuda_block_allocator.h:11-15). */
+  VHM_ERR_CAPACITY = 3,
+  /* A block coordinate outside +-2^20 (the reference's packed-key range,
+   * cuda_block_allocator.cu:161-174). */
+  VHM_ERR_KEY_RANGE = 4,
+  VHM_ERR_NO_DEVICE = 5,
+  VHM_ERR_BUFFER_TOO_SMALL = 6
+};
+
+enum vhm_memkind { VHM_HOST = 0, VHM_DEVICE = 1 };
+
+/* Constant presets: the reference CPU constants (voxel_hash_tsdf.h:194-204,
+ * reconstruction_backend.cpp:26-32) or its CUDA constants
+ * (cuda_tsdf_integrator.cu:18-35). */
+enum vhm_preset { VHM_PRESET_REFERENCE_CPU = 0, VHM_PRESET_REFERENCE_CUDA = 1 };
+
+/* Pinhole intrinsics, as Intrinsics (math_types.h:200-207). */
+typedef struct vhm_intrinsics {
+  int32_t width, height;
+  float fx, fy, cx, cy;
+} vhm_intrinsics;
+
+/* Camera-to-world pose T_wc, as Pose (math_types.h:180-198): R row-major. */
+typedef struct vhm_pose {
+  float R[9];
+  float t[3];
+} vhm_pose;
+
+typedef struct vhm_params {
+  float voxel_size;            /* VoxelHashTSDF(voxel_size, ...)              */
+  float truncation_distance;   /* VoxelHashTSDF(..., truncation_distance)     */
+  /* capacity (no reference counterpart: the reference grows host maps)     */
+  uint32_t hash_log2_buckets;  /* persistent hash table: 2^k buckets        */
+  uint32_t max_blocks;         /* pre-reserved block pool (147,456 B each)  */
+  int32_t max_width, max_height;
+  /* depth bilateral filter, reconstruction_backend.cpp:26-32 (radius 2)    */
+  float filter_sigma_spatial, filter_sigma_range, filter_sigma_angular;
+  float filter_disc_base, filter_disc_rel, filter_min_cos;
+  /* integrator, voxel_hash_tsdf.h:194-204                                  */
+  float max_weight, min_view_angle_weight;
+  float normal_sigma_spatial, normal_sigma_range, normal_sigma_angular;
+  float disc_base, disc_rel;
+  float layer_merge_scale, layer_extract_scale, layer_normal_cos;
+} vhm_params;
+
+typedef struct vhm_ctx vhm_ctx;
+
+/* Fills *p with a preset and the given volume geometry / capacity. */
+void vhm_default_params(vhm_params* p, int preset, float voxel_size, float truncation_distance);
+
+/* queryCudaBackendStatus (cuda_tsdf_integrator.cu:934-944): 1 if a GPU is
+ * usable, 0 otherwise; msg receives the reason either way. */
+int vhm_query_device(char* msg, size_t msg_capacity);
+
+/* Creates a volume on the current device: reserves the block pool and hash
+ * table (replaces GpuVolumeCache, cuda_tsdf_integrator.cu:114-167). */
+int vhm_create(const vhm_params* p, vhm_ctx** out);
+int vhm_destroy(vhm_ctx* ctx);
+const char* vhm_last_error(const vhm_ctx* ctx);
+/* Clears the volume (all blocks, counters) keeping the reservations. */
+int vhm_reset(vhm_ctx* ctx);
+
+/* One frame: filter -> allocate -> integrate.  The GPU path of
+ * TsdfIntegrationBackend::integrate (reconstruction_backend.cpp:259-277),
+ * i.e. integrateDepthFrameCuda (cuda_tsdf_integrator.cu:946-1010), with the
+ * CPU path's constants.  Depth in metres (f32) or raw uint16 scaled by
+ * depth_scale (dataset_io.h:342-356).  Asynchronous.
+ * flags: VHM_FLAG_PREFILTERED skips the filter (stage-isolated testing). */
+#define VHM_FLAG_PREFILTERED 1u
+int vhm_integrate_f32(vhm_ctx* ctx, const float* depth, int memkind, const vhm_intrinsics* K,
+                      const vhm_pose* T_wc, int allocation_stride, uint32_t flags);
+int vhm_integrate_u16(vhm_ctx* ctx, const uint16_t* depth, float depth_scale, int memkind,
+                      const vhm_intrinsics* K, const vhm_pose* T_wc, int allocation_stride, uint32_t flags);
+
+/* The depth bilateral filter alone (applyDepthBilateralFilter,
+ * reconstruction_backend.cpp:96-189).  Synchronous. */
+int vhm_filter_depth_f32(vhm_ctx* ctx, const float* depth, int in_memkind, const vhm_intrinsics* K,
+                         float* out, int out_memkind);
+int vhm_filter_depth_u16(vhm_ctx* ctx, const uint16_t* depth, float depth_scale, int in_memkind,
+                         const vhm_intrinsics* K, float* out, int out_memkind);
+
+/* allocateActiveBlocksCuda (cuda_block_allocator.h:17-22): the frame's
+ * active block keys for an (already filtered) depth frame.  Unlike the
+ * reference's throw-away per-frame table the keys are also inserted into the
+ * persistent table (as prepareFrameIntegration, voxel_hash_tsdf.h:133-139).
+ * Writes min(capacity, n) keys; *n_out = n.  Synchronous. */
+int vhm_allocate_active_blocks(vhm_ctx* ctx, const float* depth, int memkind, const vhm_intrinsics* K,
+                               const vhm_pose* T_wc, int allocation_stride, int32_t* keys_out,
+                               size_t capacity, size_t* n_out);
+
+/* Waits for queued work; returns the first sticky device error. */
+int vhm_synchronize(vhm_ctx* ctx);
+
+/* blockCount / observedVoxelCount (reconstruction_backend.cpp:293-315;
+ * gpuVolumeCacheBlockCount / gpuVolumeCacheObservedVoxelCount,
+ * cuda_tsdf_integrator.cu:1017-1052).  Synchronous, O(1). */
+int vhm_block_count(vhm_ctx* ctx, size_t* out);
+int vhm_observed_voxel_count(vhm_ctx* ctx, size_t* out);
+
+/* Counters since creation/reset: voxel updates (voxels reaching the layer
+ * update with weight > 0), voxels visited (512 per active block), frames. */
+int vhm_stats(vhm_ctx* ctx, uint64_t* updates, uint64_t* visited, uint64_t* frames);
+
+/* gpuVolumeCacheBlockKeys (cuda_tsdf_integrator.cu:1054-1057): all
+ * allocated keys in pool order. */
+int vhm_block_keys(vhm_ctx* ctx, int32_t* keys_out, size_t capacity, size_t* n_out);
+
+/* Voxel layers of the given blocks in the reference's AoS Voxel layout
+ * (voxel_hash_tsdf.h:18-31; 288 B per voxel, 147,456 B per block, local
+ * index x + 8(y + 8z)) -- what copyBlocksToFlat (voxel_hash_tsdf.h:147-164)
+ * returns per record.  Unallocated keys yield zeros.  Synchronous. */
+int vhm_download_blocks(vhm_ctx* ctx, const int32_t* keys, size_t n, void* out_host);
+
+/* extractMesh / extractMeshCuda (reconstruction_backend.cpp:279-291,
+ * cuda_mesh_extractor.cu:636-720).  Two passes on the GPU (count, scan,
+ * emit) in a deterministic order: direction, block pool order, cube, cluster.
+ * Writes min(capacity, n) triangles; *n_out = n.  tris_out == NULL with
+ * capacity 0 only counts.  Synchronous. */
+int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capacity, size_t* n_out);
+
+/* Per-kernel GPU time (ms, HIP events on the context stream) accumulated
+ * while timing is enabled: [0] filter, [1] allocate, [2] integrate, [3] mesh;
+ * launches[i] counts launches.  Enabling resets the accumulators. */
+int vhm_set_timing(vhm_ctx* ctx, int enabled);
+int vhm_kernel_times(vhm_ctx* ctx, double ms_out[4], uint64_t launches_out[4]);
+
+/* --- synthetic depth (no reference counterpart; benchmark/test input) ---- */
+
+/* Analytic scene: the interior of an axis-aligned room plus spheres
+ * (kind 0: centre, radius) and axis-aligned boxes (kind 1: centre,
+ * half-extents). */
+typedef struct vhm_primitive {
+  int32_t kind;
+  float c[3];
+  float r;
+  float h[3];
+} vhm_primitive;
+
+typedef struct vhm_scene {
+  float room_min[3], room_max[3];
+  int32_t num_primitives;
+  const vhm_primitive* primitives; /* host pointer */
+} vhm_scene;
+
+/* Depth noise: z + N(0, sigma(z)), sigma(z) = a + b (z - c)^2 metres,
+ * dropout probability p, output rounded to millimetres (uint16, 0 = none).
+ * Gaussian and dropout draws come from Philox4x32-10 keyed by (seed, frame)
+ * with the pixel index as counter, so every pixel is independent of launch
+ * shape. */
+typedef struct vhm_noise {
+  float a, b, c, dropout;
+  float max_depth; /* 0 = unlimited */
+  uint64_t seed;
+  int32_t enabled;
+} vhm_noise;
+
+/* Renders n_frames uint16 depth frames (frame i uses poses[i] and frame id
+ * first_frame + i) into out (n_frames x H x W).  memkind of out:
+ * VHM_DEVICE renders on the GPU; VHM_HOST renders on the host. */
+int vhm_synth_render(const vhm_scene* scene, const vhm_intrinsics* K, const vhm_pose* poses, int n_frames,
+                     uint32_t first_frame, const vhm_noise* noise, uint16_t* out, int out_memkind);
+
+/* Philox4x32-10 block (for known-answer tests of the generator). */
+void vhm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VHM_H_ */

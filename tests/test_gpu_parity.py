@@ -1,0 +1,200 @@
+"""GPU parity: the HIP path (libvhm.so, through its C-ABI) against the CPU
+oracle -- the restated reference CPU pipeline -- and, where it was built,
+the reference's own CPU code, on identical seeded inputs.
+
+Bar (from the fusion path's contract): bit-exact filtered depth and block
+key sets; voxel layers within 1e-6 relative (they are in fact compared
+bit-for-bit and the count of differing words is reported); meshes equal as
+multisets with vertices within 1e-5 voxel units.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import voxelhashmt_amd as vhm
+from voxelhashmt_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-6  # per-voxel layer tolerance (relative)
+
+
+def _params(w, **kw):
+    return vhm.default_params(w.voxel_size, w.truncation, filter_sigma_spatial=w.filter_sigma_spatial,
+                              filter_sigma_range=w.filter_sigma_range, hash_log2_buckets=w.hash_log2_buckets,
+                              max_blocks=w.max_blocks, max_width=w.K.width, max_height=w.K.height, **kw)
+
+
+def _metres(frames_u16):
+    # dataset_io.h:349: static_cast<float>(raw) * depth_scale, raw 0 stays 0
+    return frames_u16.astype(np.float32) * np.float32(0.001)
+
+
+def _layers_close(a, b):
+    """Element-wise |a-b| <= 1e-6 |b| on the float fields, equality on normals."""
+    ok = True
+    for f in ("weight", "surface_coord", "plane_offset"):
+        x, y = a[f].astype(np.float64), b[f].astype(np.float64)
+        ok &= bool(np.all(np.abs(x - y) <= REL_TOL * np.abs(y) + 1e-30))
+    return ok and bool(np.array_equal(a["packed_normal"], b["packed_normal"]))
+
+
+def _diff_words(a, b):
+    return int((a.view(np.uint32) != b.view(np.uint32)).sum())
+
+
+@pytest.mark.parametrize("name", ["tiny", "vga"])
+def test_filter_bitexact(gpu, name):
+    w = synth.tiny(n_frames=2, seed=11) if name == "tiny" else synth.plain_small(n_frames=2)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    print(f"[filter] {w.name} sigma_s={w.filter_sigma_spatial} sigma_r={w.filter_sigma_range}", flush=True)
+    vp = _params(w)
+    op = O.oracle_params(vp)
+    with vhm.Volume(vp) as vol:
+        for i in range(len(frames)):
+            g16 = vol.filter_depth(frames[i], w.K)            # uint16 input path
+            g32 = vol.filter_depth(_metres(frames[i]), w.K)   # float input path
+            o = O.oracle_filter(op, _metres(frames[i]), w.K)
+            assert np.array_equal(g16.view(np.uint32), o.view(np.uint32)), "u16 filter differs from oracle"
+            assert np.array_equal(g32.view(np.uint32), o.view(np.uint32)), "f32 filter differs from oracle"
+            if O.ref_available() and w.filter_sigma_spatial == 1.5:
+                r = O.ref_filter(_metres(frames[i]), w.K)
+                assert np.array_equal(g32.view(np.uint32), r.view(np.uint32))
+
+
+@pytest.mark.parametrize("name,stride", [("tiny", 1), ("tiny", 2), ("vga", 1)])
+def test_allocation_key_set(gpu, name, stride):
+    w = synth.tiny(n_frames=3, seed=5) if name == "tiny" else synth.plain_small(n_frames=2)
+    frames = _metres(synth.render(w.scene, w.K, w.poses, w.noise))
+    vp = _params(w)
+    ov = O.OracleVolume(O.oracle_params(vp))
+    print(f"[alloc] {w.name} stride={stride}", flush=True)
+    with vhm.Volume(vp) as vol:
+        for i in range(len(frames)):
+            pose = vhm.pose_array(w.poses[i])
+            g = O.sorted_keys(vol.allocate_active_blocks(frames[i], w.K, w.poses[i], stride))
+            o = O.sorted_keys(ov.prepare(frames[i], w.K, pose, stride))
+            assert len(g) == len(o) and np.array_equal(g, o), f"frame {i}: {len(g)} vs {len(o)} keys"
+        # the persistent set is the union of the frames' sets
+        assert np.array_equal(O.sorted_keys(vol.block_keys()), O.sorted_keys(ov.block_keys()))
+
+
+def _run_pipeline(w, n_frames, check_ref):
+    frames = synth.render(w.scene, w.K, w.poses[:n_frames], w.noise)
+    vp = _params(w)
+    ov = O.OracleVolume(O.oracle_params(vp))
+    rv = O.RefVolume(w.voxel_size, w.truncation) if check_ref else None
+    vol = vhm.Volume(vp)
+    for i in range(n_frames):
+        vol.integrate(frames[i], w.K, w.poses[i])  # uint16 path, filter on the GPU
+        m = _metres(frames[i])
+        ov.integrate(m, w.K, vhm.pose_array(w.poses[i]))
+        if rv is not None:
+            rv.integrate(m, w.K, vhm.pose_array(w.poses[i]))
+    vol.synchronize()
+    return vol, ov, rv
+
+
+@pytest.mark.parametrize("seed", [3, 9])
+def test_pipeline_tiny(gpu, seed):
+    w = synth.tiny(n_frames=4, seed=seed)
+    print(f"[pipeline] {w.name} seed={seed}", flush=True)
+    vol, ov, rv = _run_pipeline(w, 4, check_ref=O.ref_available())
+    keys = O.sorted_keys(ov.block_keys())
+    assert np.array_equal(O.sorted_keys(vol.block_keys()), keys)
+    assert vol.block_count() == ov.block_count()
+    assert vol.observed_voxel_count() == ov.observed_voxel_count()
+    assert vol.stats()["updates"] == ov.updates()
+    g, o = vol.download_blocks(keys), ov.dump_blocks(keys)
+    print(f"[pipeline] blocks={len(keys)} differing words={_diff_words(g, o)}", flush=True)
+    assert _layers_close(g, o)
+    assert _diff_words(g, o) == 0
+    if rv is not None:
+        assert _diff_words(g, rv.dump_blocks(keys)) == 0
+    gm, om = O.canonical_mesh(vol.extract_mesh()), O.canonical_mesh(ov.extract_mesh())
+    print(f"[pipeline] triangles gpu={len(gm)} oracle={len(om)}", flush=True)
+    assert len(gm) == len(om)
+    assert np.array_equal(gm, om)
+    vol.close()
+
+
+def test_pipeline_qvga_config_sigmas(gpu):
+    """160x120 frames with the benchmark filter constants (sigma_s 4.5 px,
+    sigma_r 30 mm), 1 cm voxels."""
+    w = synth.tiny(n_frames=3, seed=21, width=160, height=120)
+    w.filter_sigma_spatial, w.filter_sigma_range, w.voxel_size, w.truncation = 4.5, 0.03, 0.01, 0.04
+    w.max_blocks, w.hash_log2_buckets = 60000, 18
+    vol, ov, _ = _run_pipeline(w, 3, check_ref=False)
+    keys = O.sorted_keys(ov.block_keys())
+    assert np.array_equal(O.sorted_keys(vol.block_keys()), keys)
+    g, o = vol.download_blocks(keys), ov.dump_blocks(keys)
+    print(f"[qvga] blocks={len(keys)} differing words={_diff_words(g, o)}", flush=True)
+    assert _layers_close(g, o)
+    gm, om = O.canonical_mesh(vol.extract_mesh()), O.canonical_mesh(ov.extract_mesh())
+    assert len(gm) == len(om)
+    if not np.array_equal(gm, om):  # vertices within 1e-5 voxel units
+        a = gm.view(np.float32).reshape(-1, 9)
+        b = om.view(np.float32).reshape(-1, 9)
+        assert np.abs(a - b).max() <= 1e-5 * w.voxel_size
+    vol.close()
+
+
+def test_fp64_oracle_report(gpu):
+    """Reports how far the float pipeline (GPU == float oracle) sits from the
+    same algorithm evaluated in double precision."""
+    w = synth.tiny(n_frames=3, seed=3)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    vp = _params(w)
+    d64 = O.OracleVolume(O.oracle_params(vp), fp64=True)
+    with vhm.Volume(vp) as vol:
+        for i in range(3):
+            vol.integrate(frames[i], w.K, w.poses[i])
+            d64.integrate(_metres(frames[i]), w.K, vhm.pose_array(w.poses[i]))
+        keys = O.sorted_keys(d64.block_keys())
+        g = vol.download_blocks(keys)
+    w64, sc64, po64, _ = d64.dump_blocks_soa(keys)
+    both = (g["weight"] > 0) & (w64 > 0)
+    rel_w = np.abs(g["weight"][both] - w64[both]) / np.abs(w64[both])
+    rel_po = np.abs(g["plane_offset"][both] - po64[both]) / np.maximum(np.abs(po64[both]), 1e-3)
+    print(f"[fp64] slots compared={both.sum()} weight rel err median={np.median(rel_w):.2e} "
+          f"p99={np.quantile(rel_w, 0.99):.2e}; plane_offset rel err median={np.median(rel_po):.2e}", flush=True)
+    assert np.median(rel_w) < 1e-5
+
+
+def test_empty_and_invalid_frames(gpu):
+    w = synth.tiny(n_frames=1)
+    with vhm.Volume(_params(w)) as vol:
+        vol.integrate(np.zeros((w.K.height, w.K.width), np.uint16), w.K, w.poses[0])
+        assert vol.block_count() == 0 and vol.observed_voxel_count() == 0
+        assert len(vol.extract_mesh()) == 0
+        big = vhm.make_intrinsics(w.K.width * 4, w.K.height, w.K.fx, w.K.fy, w.K.cx, w.K.cy)
+        with pytest.raises(vhm.VhmError) as e:
+            vol.integrate(np.zeros((big.height, big.width), np.uint16), big, w.poses[0])
+        assert e.value.code == 1
+
+
+def test_capacity_overflow_is_reported(gpu):
+    """A pool too small for the frame reports VHM_ERR_CAPACITY (the analogue of
+    CudaActiveBlockAllocationFallback)."""
+    w = synth.tiny(n_frames=1)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    with vhm.Volume(_params(w, max_blocks=64)) as vol:
+        vol.integrate(frames[0], w.K, w.poses[0])
+        with pytest.raises(vhm.VhmError) as e:
+            vol.synchronize()
+        assert e.value.code == 3
+
+
+def test_determinism(gpu):
+    w = synth.tiny(n_frames=3, seed=4)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    outs = []
+    for _ in range(2):
+        with vhm.Volume(_params(w)) as vol:
+            for i in range(3):
+                vol.integrate(frames[i], w.K, w.poses[i])
+            keys = O.sorted_keys(vol.block_keys())
+            outs.append((keys, vol.download_blocks(keys), vol.extract_mesh()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1].view(np.uint8), outs[1][1].view(np.uint8))
+    assert np.array_equal(O.canonical_mesh(outs[0][2]), O.canonical_mesh(outs[1][2]))

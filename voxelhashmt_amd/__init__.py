@@ -1,0 +1,291 @@
+"""voxelhashmt_amd -- MI355X-native voxel-hash TSDF fusion (libvhm.so).
+
+Thin ctypes view of the C-ABI in include/vhm.h, used by the tests, the
+benchmark and Python callers.  The product is the HIP library; this module
+only marshals arguments.  Loading fails loudly when libvhm.so is missing --
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libvhm.so"
+
+VHM_HOST, VHM_DEVICE = 0, 1
+PRESET_REFERENCE_CPU, PRESET_REFERENCE_CUDA = 0, 1
+FLAG_PREFILTERED = 1
+STATUS = {0: "VHM_OK", 1: "VHM_ERR_INVALID_ARGUMENT", 2: "VHM_ERR_HIP", 3: "VHM_ERR_CAPACITY",
+          4: "VHM_ERR_KEY_RANGE", 5: "VHM_ERR_NO_DEVICE", 6: "VHM_ERR_BUFFER_TOO_SMALL"}
+BLOCK_BYTES = 147456  # 512 voxels x 6 directions x 3 slots x 16 B
+VOXEL_DTYPE = np.dtype([("weight", "<f4"), ("surface_coord", "<f4"), ("plane_offset", "<f4"),
+                        ("packed_normal", "<u4")])
+
+
+class Intrinsics(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("fx", C.c_float), ("fy", C.c_float),
+                ("cx", C.c_float), ("cy", C.c_float)]
+
+
+class Pose(C.Structure):
+    _fields_ = [("R", C.c_float * 9), ("t", C.c_float * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, t) for n, t in [
+        ("voxel_size", C.c_float), ("truncation_distance", C.c_float),
+        ("hash_log2_buckets", C.c_uint32), ("max_blocks", C.c_uint32),
+        ("max_width", C.c_int32), ("max_height", C.c_int32),
+        ("filter_sigma_spatial", C.c_float), ("filter_sigma_range", C.c_float),
+        ("filter_sigma_angular", C.c_float), ("filter_disc_base", C.c_float),
+        ("filter_disc_rel", C.c_float), ("filter_min_cos", C.c_float),
+        ("max_weight", C.c_float), ("min_view_angle_weight", C.c_float),
+        ("normal_sigma_spatial", C.c_float), ("normal_sigma_range", C.c_float),
+        ("normal_sigma_angular", C.c_float), ("disc_base", C.c_float), ("disc_rel", C.c_float),
+        ("layer_merge_scale", C.c_float), ("layer_extract_scale", C.c_float),
+        ("layer_normal_cos", C.c_float)]]
+
+
+class Primitive(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("c", C.c_float * 3), ("r", C.c_float), ("h", C.c_float * 3)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("room_min", C.c_float * 3), ("room_max", C.c_float * 3), ("num_primitives", C.c_int32),
+                ("primitives", C.POINTER(Primitive))]
+
+
+class Noise(C.Structure):
+    _fields_ = [("a", C.c_float), ("b", C.c_float), ("c", C.c_float), ("dropout", C.c_float),
+                ("max_depth", C.c_float), ("seed", C.c_uint64), ("enabled", C.c_int32)]
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "vhm_default_params": (None, [C.POINTER(Params), C.c_int, C.c_float, C.c_float]),
+    "vhm_query_device": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "vhm_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "vhm_destroy": (C.c_int, [C.c_void_p]),
+    "vhm_last_error": (C.c_char_p, [C.c_void_p]),
+    "vhm_reset": (C.c_int, [C.c_void_p]),
+    "vhm_integrate_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Intrinsics), C.POINTER(Pose),
+                                    C.c_int, C.c_uint32]),
+    "vhm_integrate_u16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.POINTER(Intrinsics),
+                                    C.POINTER(Pose), C.c_int, C.c_uint32]),
+    "vhm_filter_depth_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Intrinsics), C.c_void_p,
+                                       C.c_int]),
+    "vhm_filter_depth_u16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int, C.POINTER(Intrinsics),
+                                       C.c_void_p, C.c_int]),
+    "vhm_allocate_active_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Intrinsics),
+                                             C.POINTER(Pose), C.c_int, C.c_void_p, C.c_size_t,
+                                             C.POINTER(C.c_size_t)]),
+    "vhm_synchronize": (C.c_int, [C.c_void_p]),
+    "vhm_block_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "vhm_observed_voxel_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "vhm_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "vhm_block_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vhm_download_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "vhm_extract_mesh": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vhm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "vhm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "vhm_synth_render": (C.c_int, [C.POINTER(Scene), C.POINTER(Intrinsics), C.POINTER(Pose), C.c_int, C.c_uint32,
+                                   C.POINTER(Noise), C.c_void_p, C.c_int]),
+    "vhm_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+}
+
+
+def exported_symbols():
+    """Names the C-ABI must export (every function declared in include/vhm.h)."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Loads libvhm.so (build it with `python -m voxelhashmt_amd.build`)."""
+    global _lib
+    if _lib is None:
+        path = os.environ.get("VHM_LIBRARY", str(LIB_PATH))
+        if not Path(path).exists():
+            raise RuntimeError(f"libvhm.so not built ({path}); run `python -m voxelhashmt_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class VhmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS.get(code, code)}: {message}")
+        self.code = code
+
+
+def _ptr(a) -> int:
+    """Address of a numpy array (host) or a torch tensor (its device pointer)."""
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _kind(a) -> int:
+    if isinstance(a, np.ndarray):
+        return VHM_HOST
+    return VHM_DEVICE if a.is_cuda else VHM_HOST
+
+
+def make_intrinsics(width, height, fx, fy, cx, cy) -> Intrinsics:
+    return Intrinsics(int(width), int(height), float(fx), float(fy), float(cx), float(cy))
+
+
+def make_pose(R, t) -> Pose:
+    p = Pose()
+    R = np.asarray(R, dtype=np.float32).reshape(9)
+    t = np.asarray(t, dtype=np.float32).reshape(3)
+    for i in range(9):
+        p.R[i] = float(R[i])
+    for i in range(3):
+        p.t[i] = float(t[i])
+    return p
+
+
+def pose_array(pose: Pose) -> np.ndarray:
+    """12 floats: R row-major then t."""
+    return np.array(list(pose.R) + list(pose.t), dtype=np.float32)
+
+
+def default_params(voxel_size: float, truncation: float, preset: int = PRESET_REFERENCE_CPU, **overrides) -> Params:
+    p = Params()
+    lib().vhm_default_params(C.byref(p), preset, voxel_size, truncation)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def device_available() -> tuple[bool, str]:
+    buf = C.create_string_buffer(256)
+    ok = lib().vhm_query_device(buf, 256)
+    return bool(ok), buf.value.decode()
+
+
+class Volume:
+    """One GPU volume (a vhm_ctx): the drop-in for a VoxelHashTSDF driven by
+    TsdfIntegrationBackend's GPU path."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        self._h = C.c_void_p()
+        rc = lib().vhm_create(C.byref(params), C.byref(self._h))
+        if rc != 0:
+            raise VhmError(rc, "vhm_create failed")
+
+    def close(self):
+        if self._h:
+            lib().vhm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VhmError(rc, lib().vhm_last_error(self._h).decode())
+
+    def integrate(self, depth, K: Intrinsics, T_wc: Pose, stride: int = 1, depth_scale: float = 0.001,
+                  prefiltered: bool = False):
+        flags = FLAG_PREFILTERED if prefiltered else 0
+        if str(getattr(depth, "dtype", "")) in ("uint16", "torch.uint16", "torch.int16"):
+            self._check(lib().vhm_integrate_u16(self._h, _ptr(depth), depth_scale, _kind(depth), C.byref(K),
+                                                C.byref(T_wc), stride, flags))
+        else:
+            self._check(lib().vhm_integrate_f32(self._h, _ptr(depth), _kind(depth), C.byref(K), C.byref(T_wc),
+                                                stride, flags))
+
+    def filter_depth(self, depth, K: Intrinsics, depth_scale: float = 0.001) -> np.ndarray:
+        out = np.zeros((K.height, K.width), dtype=np.float32)
+        if str(getattr(depth, "dtype", "")) in ("uint16", "torch.uint16", "torch.int16"):
+            self._check(lib().vhm_filter_depth_u16(self._h, _ptr(depth), depth_scale, _kind(depth), C.byref(K),
+                                                   out.ctypes.data, VHM_HOST))
+        else:
+            self._check(lib().vhm_filter_depth_f32(self._h, _ptr(depth), _kind(depth), C.byref(K),
+                                                   out.ctypes.data, VHM_HOST))
+        return out
+
+    def allocate_active_blocks(self, depth, K: Intrinsics, T_wc: Pose, stride: int = 1) -> np.ndarray:
+        n = C.c_size_t()
+        cap = 1 << int(self.params.hash_log2_buckets)  # a frame activates at most one key per bucket
+        out = np.zeros((cap, 3), dtype=np.int32)
+        self._check(lib().vhm_allocate_active_blocks(self._h, _ptr(depth), _kind(depth), C.byref(K),
+                                                     C.byref(T_wc), stride, out.ctypes.data, cap, C.byref(n)))
+        return out[: n.value].copy()
+
+    def synchronize(self):
+        self._check(lib().vhm_synchronize(self._h))
+
+    def block_count(self) -> int:
+        n = C.c_size_t()
+        self._check(lib().vhm_block_count(self._h, C.byref(n)))
+        return n.value
+
+    def observed_voxel_count(self) -> int:
+        n = C.c_size_t()
+        self._check(lib().vhm_observed_voxel_count(self._h, C.byref(n)))
+        return n.value
+
+    def stats(self) -> dict:
+        u, v, f = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._check(lib().vhm_stats(self._h, C.byref(u), C.byref(v), C.byref(f)))
+        return {"updates": u.value, "visited": v.value, "frames": f.value}
+
+    def block_keys(self) -> np.ndarray:
+        n = self.block_count()
+        out = np.zeros((max(n, 1), 3), dtype=np.int32)
+        m = C.c_size_t()
+        self._check(lib().vhm_block_keys(self._h, out.ctypes.data, n, C.byref(m)))
+        return out[:n]
+
+    def download_blocks(self, keys: np.ndarray) -> np.ndarray:
+        """Blocks in the reference's AoS layout: (n, 512, 6, 3) of VOXEL_DTYPE."""
+        keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros((len(keys), 512, 6, 3), dtype=VOXEL_DTYPE)
+        if len(keys):
+            self._check(lib().vhm_download_blocks(self._h, keys.ctypes.data, len(keys), out.ctypes.data))
+        return out
+
+    def extract_mesh(self, iso_level: float = 0.0) -> np.ndarray:
+        """Triangles (n, 3, 3) float32."""
+        n = C.c_size_t()
+        self._check(lib().vhm_extract_mesh(self._h, iso_level, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 3, 3), dtype=np.float32)
+        m = C.c_size_t()
+        if n.value:
+            self._check(lib().vhm_extract_mesh(self._h, iso_level, out.ctypes.data, n.value, C.byref(m)))
+        return out[: n.value]
+
+    def set_timing(self, enabled: bool):
+        self._check(lib().vhm_set_timing(self._h, int(enabled)))
+
+    def kernel_times(self) -> dict:
+        ms = (C.c_double * 4)()
+        cnt = (C.c_uint64 * 4)()
+        self._check(lib().vhm_kernel_times(self._h, ms, cnt))
+        names = ["filter", "allocate", "integrate", "mesh"]
+        return {n: {"ms": ms[i], "launches": cnt[i]} for i, n in enumerate(names)}

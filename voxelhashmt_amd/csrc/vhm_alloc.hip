@@ -1,0 +1,206 @@
+// Hash-block allocation for one frame.
+//
+// Reference rule (collectBlocksForFrame, voxel_hash_tsdf.h:638-671): for every
+// valid pixel (stride s) back-project, take the surface point and the points
+// one truncation distance in front of / behind it along the camera ray, and
+// activate the 3x3x3 block neighbourhood of each point's block
+// (floorDiv(floor(x / voxel), 8), voxel_hash_tsdf.h:595-617).  The CUDA twin
+// (cuda_block_allocator.cu:231-300) issues all 81 candidate inserts per pixel
+// as global CAS into a table rebuilt (and memset) every frame.
+//
+// Here one workgroup covers a 16x16 sample tile and deduplicates twice in LDS
+// before touching HBM: the tile's (<= 768) centre keys, then their 27-block
+// neighbourhoods.  Each surviving key is inserted into the persistent
+// open-addressing table with a 64-bit CAS (linear probing).  A per-key frame
+// stamp (atomic exchange) elects one finder per key per frame, which appends
+// the key's slot to the frame's active list; inserting lanes take a pool index
+// with one atomic add.  LDS overflow falls back to direct global inserts, so
+// the resulting key set equals the reference's for every input.
+#include "vhm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace vhm {
+namespace {
+
+constexpr int kTile = 16;
+constexpr int kCentreSlots = 1024;  // >= 3 * 256 centre keys
+constexpr int kNbrSlots = 4096;
+constexpr int kNbrProbe = 32;
+
+__device__ __forceinline__ uint32_t ldsSlot(uint64_t key, int bits) {
+  return static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+// Inserts key into the persistent table; returns its slot (kInvalid if the
+// probe bound is exhausted).  `created` is set for the inserting lane.
+__device__ uint32_t findOrInsert(const VolumeView& V, uint64_t key, bool& created) {
+  uint32_t slot = hashSlot(key, V.log2cap);
+  const int bound = (V.mask + 1u) < static_cast<uint32_t>(kMaxProbe) ? static_cast<int>(V.mask + 1u) : kMaxProbe;
+  for (int probe = 0; probe < bound; ++probe) {
+    const uint64_t cur = __hip_atomic_load(&V.keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == key) return slot;
+    if (cur == kEmptyKey) {
+      const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&V.keys[slot]),
+                                      static_cast<unsigned long long>(kEmptyKey),
+                                      static_cast<unsigned long long>(key));
+      if (prev == kEmptyKey) {
+        created = true;
+        return slot;
+      }
+      if (prev == key) return slot;
+    }
+    slot = (slot + 1u) & V.mask;
+  }
+  return kInvalid;
+}
+
+__device__ void activateKey(const VolumeView& V, uint64_t key, uint32_t frame) {
+  bool created = false;
+  const uint32_t slot = findOrInsert(V, key, created);
+  if (slot == kInvalid) {
+    atomicOr(&V.hdr->error, kErrHashFull);
+    return;
+  }
+  if (created) {
+    const uint32_t p = atomicAdd(&V.hdr->num_blocks, 1u);
+    if (p < V.max_blocks) {
+      V.vals[slot] = p;
+      V.pool_keys[p] = key;
+    } else {
+      atomicOr(&V.hdr->error, kErrPoolFull);
+    }
+  }
+  if (atomicExch(&V.stamp[slot], frame) != frame) {
+    const uint32_t i = atomicAdd(&V.hdr->active_count, 1u);
+    V.active[i] = slot;  // i < 2^k: at most one entry per table slot per frame
+  }
+}
+
+__device__ __forceinline__ int worldToVoxel1(float x, float voxel) {
+  return static_cast<int>(floorf(x / voxel));
+}
+
+__global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ depth, VolumeView V, AllocArgs a) {
+  __shared__ uint64_t centres[kCentreSlots];
+  __shared__ uint64_t centre_list[3 * 256];
+  __shared__ uint64_t nbrs[kNbrSlots];
+  __shared__ uint32_t n_centres;
+
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kCentreSlots; i += 256) centres[i] = kEmptyKey;
+  for (int i = tid; i < kNbrSlots; i += 256) nbrs[i] = kEmptyKey;
+  if (tid == 0) n_centres = 0;
+  __syncthreads();
+
+  // 1. centre keys of this lane's pixel (voxel_hash_tsdf.h:645-661).
+  const int sx = blockIdx.x * kTile + tid % kTile, sy = blockIdx.y * kTile + tid / kTile;
+  const int u = sx * a.stride, v = sy * a.stride;
+  if (u < a.K.width && v < a.K.height) {
+    const float d = depth[static_cast<size_t>(v) * a.K.width + u];
+    if (d > 0.0f) {
+      const F3 pw = transformPoint(a.T_wc, backProjectPixel(u, v, d, a.K));
+      const F3 origin{a.T_wc.t[0], a.T_wc.t[1], a.T_wc.t[2]};
+      const F3 ray = normalized3(sub3(pw, origin));
+      const F3 pts[3] = {pw, sub3(pw, mul3(ray, a.trunc)), add3(pw, mul3(ray, a.trunc))};
+      for (int j = 0; j < 3; ++j) {
+        const int bx = floorDiv(worldToVoxel1(pts[j].x, a.voxel), kBlockSide);
+        const int by = floorDiv(worldToVoxel1(pts[j].y, a.voxel), kBlockSide);
+        const int bz = floorDiv(worldToVoxel1(pts[j].z, a.voxel), kBlockSide);
+        if (!keyInRange(bx - 1, by - 1, bz - 1) || !keyInRange(bx + 1, by + 1, bz + 1)) {
+          atomicOr(&V.hdr->error, kErrKeyRange);
+          continue;
+        }
+        const uint64_t key = packKey(bx, by, bz);
+        uint32_t s = ldsSlot(key, 10);
+        for (;;) {  // never full: at most 768 keys in 1024 slots
+          const unsigned long long prev =
+              atomicCAS(reinterpret_cast<unsigned long long*>(&centres[s]), static_cast<unsigned long long>(kEmptyKey),
+                        static_cast<unsigned long long>(key));
+          if (prev == kEmptyKey) {
+            centre_list[atomicAdd(&n_centres, 1u)] = key;
+            break;
+          }
+          if (prev == key) break;
+          s = (s + 1u) & (kCentreSlots - 1);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // 2. 27-neighbourhoods of the unique centres (voxel_hash_tsdf.h:626-636).
+  const uint32_t work = n_centres * 27u;
+  for (uint32_t w = tid; w < work; w += 256) {
+    const uint64_t c = centre_list[w / 27u];
+    const int o = static_cast<int>(w % 27u);
+    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+    const uint64_t key = packKey(keyX(c) + dx, keyY(c) + dy, keyZ(c) + dz);
+    uint32_t s = ldsSlot(key, 12);
+    bool placed = false;
+    for (int p = 0; p < kNbrProbe; ++p) {
+      const unsigned long long prev =
+          atomicCAS(reinterpret_cast<unsigned long long*>(&nbrs[s]), static_cast<unsigned long long>(kEmptyKey),
+                    static_cast<unsigned long long>(key));
+      if (prev == kEmptyKey || prev == key) {
+        placed = true;
+        break;
+      }
+      s = (s + 1u) & (kNbrSlots - 1);
+    }
+    if (!placed) activateKey(V, key, a.frame);  // LDS set crowded: go global directly
+  }
+  __syncthreads();
+
+  // 3. persistent insert + frame activation of the unique neighbourhood keys.
+  for (int i = tid; i < kNbrSlots; i += 256) {
+    const uint64_t key = nbrs[i];
+    if (key != kEmptyKey) activateKey(V, key, a.frame);
+  }
+}
+
+__global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) {
+  hdr->active_count = 0u;
+  hdr->frame = frame;
+}
+
+}  // namespace
+
+hipError_t launchFrameBegin(DevHeader* hdr, uint32_t frame, hipStream_t s) {
+  frameBeginKernel<<<1, 1, 0, s>>>(hdr, frame);
+  return hipGetLastError();
+}
+
+hipError_t launchAllocate(const float* depth, const VolumeView& V, const AllocArgs& a, hipStream_t s) {
+  const int sw = (a.K.width + a.stride - 1) / a.stride, sh = (a.K.height + a.stride - 1) / a.stride;
+  const dim3 grid((sw + kTile - 1) / kTile, (sh + kTile - 1) / kTile);
+  allocateKernel<<<grid, 256, 0, s>>>(depth, V, a);
+  return hipGetLastError();
+}
+
+}  // namespace vhm
+
+namespace vhm {
+namespace {
+
+__global__ __launch_bounds__(256) void activeKeysKernel(VolumeView V, int32_t* __restrict__ out, uint32_t max_n) {
+  const uint32_t n = V.hdr->active_count < max_n ? V.hdr->active_count : max_n;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint64_t key = V.keys[V.active[i]];
+    out[3 * i] = keyX(key);
+    out[3 * i + 1] = keyY(key);
+    out[3 * i + 2] = keyZ(key);
+  }
+}
+
+}  // namespace
+
+// Keys of the current frame's active list (at most max_n of them).
+hipError_t launchActiveKeys(const VolumeView& V, int32_t* out, hipStream_t s, uint32_t max_n) {
+  const uint32_t grid = (max_n + 255u) / 256u < 1024u ? (max_n + 255u) / 256u : 1024u;
+  if (grid == 0) return hipSuccess;
+  activeKeysKernel<<<grid, 256, 0, s>>>(V, out, max_n);
+  return hipGetLastError();
+}
+
+}  // namespace vhm

@@ -1,0 +1,697 @@
+// C-ABI of libvhm.so (declared in include/vhm.h): volume context, frame
+// integration, stage entry points, queries, mesh extraction and synthetic
+// depth.  Host code only; kernels live in vhm_{filter,alloc,integrate,mesh,
+// synth}.hip.
+//
+// Replaces the reference's GPU backend functions (cuda_tsdf_integrator.h,
+// cuda_block_allocator.h, cuda_mesh_extractor.h) and its process-global
+// singletons (cuda_tsdf_integrator.cu:114-124) with an explicit context that
+// owns a pre-reserved block pool, a persistent hash table and one stream.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vhm.h"
+#include "vhm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace vhm {
+void philox4x32_10(const uint32_t in_ctr[4], const uint32_t in_key[2], uint32_t out[4]);
+void synthRenderHost(const SynthDeviceScene& sc, const Intrinsics& K, const Pose34* poses, int n_frames,
+                     uint32_t first_frame, const SynthNoise& nz, uint16_t* out);
+hipError_t launchActiveKeys(const VolumeView& V, int32_t* out, hipStream_t s, uint32_t max_n);
+}  // namespace vhm
+
+using namespace vhm;
+
+struct vhm_ctx {
+  vhm_params P{};
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  uint32_t frame = 0;
+  uint64_t frames_integrated = 0;
+  // persistent volume
+  uint64_t* keys = nullptr;
+  uint32_t* vals = nullptr;
+  uint32_t* stamp = nullptr;
+  uint64_t* pool_keys = nullptr;
+  uint32_t* pool = nullptr;
+  uint32_t* obs = nullptr;
+  uint32_t* dirty = nullptr;
+  uint32_t* active = nullptr;
+  DevHeader* hdr = nullptr;
+  // per-frame buffers
+  void* depth_in = nullptr;  // max W*H*4 bytes (f32 or u16 staging)
+  float* filtered = nullptr;
+  // grown on demand
+  uint32_t* mesh_counts = nullptr;
+  uint32_t* mesh_offsets = nullptr;
+  uint32_t* mesh_partials = nullptr;
+  size_t mesh_counts_cap = 0, mesh_offsets_cap = 0, mesh_partials_cap = 0;
+  float* tris = nullptr;
+  size_t tris_cap = 0;
+  int32_t* key_buf = nullptr;
+  size_t key_buf_cap = 0;
+  uint32_t* gather_buf = nullptr;
+  size_t gather_cap = 0;
+  // timing
+  bool timing = false;
+  struct Ev {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<Ev> events;
+  double ms[4] = {0, 0, 0, 0};
+  uint64_t launches[4] = {0, 0, 0, 0};
+  std::string err;
+};
+
+namespace {
+
+int fail(vhm_ctx* c, int code, const std::string& what) {
+  if (c) c->err = what;
+  return code;
+}
+
+int hipFail(vhm_ctx* c, hipError_t e, const char* action) {
+  return fail(c, VHM_ERR_HIP, std::string(action) + " failed: " + hipGetErrorString(e));
+}
+
+#define VHM_TRY(ctx, expr, action)                        \
+  do {                                                    \
+    const hipError_t _e = (expr);                         \
+    if (_e != hipSuccess) return hipFail(ctx, _e, action); \
+  } while (0)
+
+VolumeView view(vhm_ctx* c) {
+  VolumeView V;
+  V.keys = c->keys;
+  V.vals = c->vals;
+  V.stamp = c->stamp;
+  V.log2cap = c->P.hash_log2_buckets;
+  V.mask = (1u << c->P.hash_log2_buckets) - 1u;
+  V.pool_keys = c->pool_keys;
+  V.pool = c->pool;
+  V.obs = c->obs;
+  V.dirty = c->dirty;
+  V.max_blocks = c->P.max_blocks;
+  V.active = c->active;
+  V.hdr = c->hdr;
+  return V;
+}
+
+Intrinsics toK(const vhm_intrinsics* K) { return {K->width, K->height, K->fx, K->fy, K->cx, K->cy}; }
+
+Pose34 toPose(const vhm_pose* p) {
+  Pose34 o;
+  for (int i = 0; i < 9; ++i) o.R[i] = p->R[i];
+  for (int i = 0; i < 3; ++i) o.t[i] = p->t[i];
+  return o;
+}
+
+// 1 / (2 sigma^2) formed as the reference's constexprs (float).
+float invTwoSigmaSq(float s) { return 1.0f / (2.0f * s * s); }
+
+FilterArgs filterArgs(const vhm_ctx* c, const vhm_intrinsics* K, float scale) {
+  FilterArgs a{};
+  a.K = toK(K);
+  a.scale = scale;
+  const float inv_s = invTwoSigmaSq(c->P.filter_sigma_spatial);
+  for (int dv = -2; dv <= 2; ++dv)
+    for (int du = -2; du <= 2; ++du) {
+      const float sq = static_cast<float>(du * du + dv * dv);
+      a.spatial_w[(dv + 2) * 5 + (du + 2)] = std::exp(-sq * inv_s);
+    }
+  a.inv_r = invTwoSigmaSq(c->P.filter_sigma_range);
+  a.inv_a = invTwoSigmaSq(c->P.filter_sigma_angular);
+  a.disc_base = c->P.filter_disc_base;
+  a.disc_rel = c->P.filter_disc_rel;
+  a.min_cos = c->P.filter_min_cos;
+  return a;
+}
+
+int checkFrame(vhm_ctx* c, const void* depth, const vhm_intrinsics* K) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  if (!depth || !K) return fail(c, VHM_ERR_INVALID_ARGUMENT, "integrate failed: null argument");
+  if (K->width <= 0 || K->height <= 0 || K->width > c->P.max_width || K->height > c->P.max_height)
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "integrate failed: frame size outside the context's reservation");
+  if (!(K->fx != 0.0f) || !(K->fy != 0.0f))
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "integrate failed: zero focal length");
+  return VHM_OK;
+}
+
+void beginTimed(vhm_ctx* c, int kind, hipEvent_t* b) {
+  if (!c->timing) return;
+  hipEvent_t a;
+  hipEventCreate(&a);
+  hipEventCreate(b);
+  hipEventRecord(a, c->stream);
+  c->events.push_back({a, *b, kind});
+}
+
+void endTimed(vhm_ctx* c, hipEvent_t b) {
+  if (c->timing) hipEventRecord(b, c->stream);
+}
+
+void drainEvents(vhm_ctx* c) {
+  for (auto& e : c->events) {
+    hipEventSynchronize(e.b);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e.a, e.b);
+    c->ms[e.kind] += ms;
+    c->launches[e.kind] += 1;
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  c->events.clear();
+}
+
+// Stage a frame's depth into device memory; returns the device pointer.
+template <typename T>
+int stageDepth(vhm_ctx* c, const T* depth, int memkind, const vhm_intrinsics* K, const T** dev) {
+  if (memkind == VHM_DEVICE) {
+    *dev = depth;
+    return VHM_OK;
+  }
+  const size_t bytes = sizeof(T) * static_cast<size_t>(K->width) * K->height;
+  VHM_TRY(c, hipMemcpyAsync(c->depth_in, depth, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync H2D depth");
+  *dev = static_cast<const T*>(c->depth_in);
+  return VHM_OK;
+}
+
+int filterStage(vhm_ctx* c, const void* dev_in, bool u16, float scale, const vhm_intrinsics* K, float* out) {
+  const FilterArgs fa = filterArgs(c, K, scale);
+  hipEvent_t ev{};
+  beginTimed(c, 0, &ev);
+  const hipError_t e = u16 ? launchFilterU16(static_cast<const uint16_t*>(dev_in), out, fa, c->stream)
+                           : launchFilterF32(static_cast<const float*>(dev_in), out, fa, c->stream);
+  endTimed(c, ev);
+  VHM_TRY(c, e, "depth filter kernel launch");
+  return VHM_OK;
+}
+
+int allocStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, const vhm_pose* T_wc, int stride) {
+  ++c->frame;
+  VHM_TRY(c, launchFrameBegin(c->hdr, c->frame, c->stream), "frame begin kernel launch");
+  AllocArgs aa{};
+  aa.K = toK(K);
+  aa.T_wc = toPose(T_wc);
+  aa.voxel = c->P.voxel_size;
+  aa.trunc = c->P.truncation_distance;
+  aa.stride = stride < 1 ? 1 : stride;
+  aa.frame = c->frame;
+  hipEvent_t ev{};
+  beginTimed(c, 1, &ev);
+  const hipError_t e = launchAllocate(dev_depth, view(c), aa, c->stream);
+  endTimed(c, ev);
+  VHM_TRY(c, e, "block allocation kernel launch");
+  return VHM_OK;
+}
+
+int integrateStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, const vhm_pose* T_wc) {
+  IntegrateArgs ia{};
+  ia.K = toK(K);
+  ia.T_wc = toPose(T_wc);
+  ia.T_cw = inversePose(ia.T_wc);
+  ia.voxel = c->P.voxel_size;
+  ia.trunc = c->P.truncation_distance;
+  ia.merge_tol = c->P.voxel_size * c->P.layer_merge_scale;
+  ia.merge_tol = ia.merge_tol < 1e-6f ? 1e-6f : ia.merge_tol;  // std::max(1e-6f, .)
+  ia.min_cos = c->P.layer_normal_cos;
+  ia.max_weight = c->P.max_weight;
+  ia.min_view = c->P.min_view_angle_weight;
+  ia.disc_base = c->P.disc_base;
+  ia.disc_rel = c->P.disc_rel;
+  const float inv_s = invTwoSigmaSq(c->P.normal_sigma_spatial);
+  for (int s = 0; s < 3; ++s) ia.spatial_w[s] = std::exp(-static_cast<float>(s) * inv_s);
+  ia.inv_r = invTwoSigmaSq(c->P.normal_sigma_range);
+  ia.inv_a = invTwoSigmaSq(c->P.normal_sigma_angular);
+  hipEvent_t ev{};
+  beginTimed(c, 2, &ev);
+  const hipError_t e = launchIntegrate(dev_depth, view(c), ia, c->num_cus * 8, c->stream);
+  endTimed(c, ev);
+  VHM_TRY(c, e, "integrate kernel launch");
+  ++c->frames_integrated;
+  return VHM_OK;
+}
+
+template <typename T>
+int integrateFrame(vhm_ctx* c, const T* depth, bool u16, float scale, int memkind, const vhm_intrinsics* K,
+                   const vhm_pose* T_wc, int stride, uint32_t flags) {
+  int rc = checkFrame(c, depth, K);
+  if (rc) return rc;
+  if (!T_wc) return fail(c, VHM_ERR_INVALID_ARGUMENT, "integrate failed: null pose");
+  if (u16 && (flags & VHM_FLAG_PREFILTERED))
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "integrate failed: a prefiltered frame must be float metres");
+  const T* dev = nullptr;
+  if ((rc = stageDepth(c, depth, memkind, K, &dev))) return rc;
+  const float* fdepth = nullptr;
+  if (flags & VHM_FLAG_PREFILTERED) {
+    fdepth = reinterpret_cast<const float*>(dev);
+  } else {
+    if ((rc = filterStage(c, dev, u16, scale, K, c->filtered))) return rc;
+    fdepth = c->filtered;
+  }
+  if ((rc = allocStage(c, fdepth, K, T_wc, stride))) return rc;
+  return integrateStage(c, fdepth, K, T_wc);
+}
+
+int readHeader(vhm_ctx* c, DevHeader* h) {
+  VHM_TRY(c, hipMemcpyAsync(h, c->hdr, sizeof(DevHeader), hipMemcpyDeviceToHost, c->stream), "hipMemcpy D2H header");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (c->timing) drainEvents(c);
+  if (h->error & kErrKeyRange)
+    return fail(c, VHM_ERR_KEY_RANGE, "block allocation failed: block coordinate outside +-2^20");
+  if (h->error & (kErrHashFull | kErrPoolFull))
+    return fail(c, VHM_ERR_CAPACITY,
+                std::string("block allocation failed: ") +
+                    ((h->error & kErrPoolFull) ? "block pool full (max_blocks)" : "hash table full") +
+                    "; create the volume with a larger reservation");
+  return VHM_OK;
+}
+
+template <typename T>
+int ensure(vhm_ctx* c, T** buf, size_t* cap, size_t n, const char* what) {
+  if (n <= *cap) return VHM_OK;
+  if (*buf) hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  VHM_TRY(c, hipMalloc(reinterpret_cast<void**>(buf), sizeof(T) * n), what);
+  *cap = n;
+  return VHM_OK;
+}
+
+int initVolume(vhm_ctx* c) {
+  const size_t cap = size_t(1) << c->P.hash_log2_buckets;
+  const hipStream_t s = c->stream;
+  VHM_TRY(c, hipMemsetAsync(c->keys, 0xFF, cap * sizeof(uint64_t), s), "hipMemset keys");
+  VHM_TRY(c, hipMemsetAsync(c->vals, 0xFF, cap * sizeof(uint32_t), s), "hipMemset vals");
+  VHM_TRY(c, hipMemsetAsync(c->stamp, 0, cap * sizeof(uint32_t), s), "hipMemset stamps");
+  VHM_TRY(c, hipMemsetAsync(c->pool, 0, size_t(c->P.max_blocks) * kBlockWords * 4u, s), "hipMemset pool");
+  VHM_TRY(c, hipMemsetAsync(c->obs, 0, size_t(c->P.max_blocks) * 16u * 4u, s), "hipMemset observed");
+  VHM_TRY(c, hipMemsetAsync(c->dirty, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset dirty");
+  VHM_TRY(c, hipMemsetAsync(c->hdr, 0, sizeof(DevHeader), s), "hipMemset header");
+  VHM_TRY(c, hipStreamSynchronize(s), "hipStreamSynchronize");
+  c->frame = 0;
+  c->frames_integrated = 0;
+  return VHM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void vhm_default_params(vhm_params* p, int preset, float voxel_size, float truncation_distance) {
+  std::memset(p, 0, sizeof(*p));
+  p->voxel_size = voxel_size;
+  p->truncation_distance = truncation_distance;
+  p->hash_log2_buckets = 20;
+  p->max_blocks = 65536;
+  p->max_width = 1280;
+  p->max_height = 720;
+  // reconstruction_backend.cpp:26-32
+  p->filter_sigma_spatial = 1.5f;
+  p->filter_sigma_range = 0.0012f;
+  p->filter_sigma_angular = 0.35f;
+  p->filter_disc_base = 0.0012f;
+  p->filter_disc_rel = 0.0025f;
+  p->filter_min_cos = 0.6f;
+  // voxel_hash_tsdf.h:194-204
+  p->max_weight = 100.0f;
+  p->min_view_angle_weight = 0.15f;
+  p->normal_sigma_spatial = 1.0f;
+  p->normal_sigma_range = 0.0012f;
+  p->normal_sigma_angular = 0.35f;
+  p->disc_base = 0.0012f;
+  p->disc_rel = 0.0025f;
+  p->layer_merge_scale = 0.25f;
+  p->layer_extract_scale = 0.55f;
+  p->layer_normal_cos = 0.75f;
+  if (preset == VHM_PRESET_REFERENCE_CUDA) {  // cuda_tsdf_integrator.cu:27-35
+    p->disc_base = 0.012f;
+    p->disc_rel = 0.025f;
+    p->filter_disc_base = 0.012f;
+    p->filter_disc_rel = 0.025f;
+    p->layer_normal_cos = 0.9f;
+    p->filter_min_cos = 0.9f;
+  }
+}
+
+int vhm_query_device(char* msg, size_t cap) {
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  std::string m;
+  int ok = 0;
+  if (e != hipSuccess)
+    m = hipGetErrorString(e);
+  else if (n <= 0)
+    m = "no HIP device detected";
+  else {
+    hipDeviceProp_t prop{};
+    hipGetDeviceProperties(&prop, 0);
+    m = std::to_string(n) + " HIP device(s) detected (" + prop.gcnArchName + ")";
+    ok = 1;
+  }
+  if (msg && cap) std::snprintf(msg, cap, "%s", m.c_str());
+  return ok;
+}
+
+int vhm_create(const vhm_params* p, vhm_ctx** out) {
+  if (!p || !out) return VHM_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto* c = new vhm_ctx();
+  c->P = *p;
+  auto bad = [&](const char*) {
+    delete c;
+    return VHM_ERR_INVALID_ARGUMENT;
+  };
+  if (!(p->voxel_size > 0.0f) || !(p->truncation_distance > 0.0f)) return bad("voxel_size/truncation must be > 0");
+  if (p->hash_log2_buckets < 4 || p->hash_log2_buckets > 30) return bad("hash_log2_buckets outside [4, 30]");
+  if (p->max_blocks == 0 || p->max_blocks > (1u << 31)) return bad("max_blocks outside [1, 2^31]");
+  if (p->max_width <= 0 || p->max_height <= 0) return bad("max_width/max_height must be > 0");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    delete c;
+    return VHM_ERR_NO_DEVICE;
+  }
+  hipGetDevice(&c->device);
+  hipDeviceProp_t prop{};
+  hipGetDeviceProperties(&prop, c->device);
+  c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  const size_t cap = size_t(1) << p->hash_log2_buckets;
+  const size_t px = size_t(p->max_width) * p->max_height;
+  if (e == hipSuccess) e = hipMalloc(&c->keys, cap * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&c->vals, cap * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&c->stamp, cap * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&c->active, cap * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&c->pool_keys, size_t(p->max_blocks) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc(&c->pool, size_t(p->max_blocks) * kBlockWords * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->obs, size_t(p->max_blocks) * 16u * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->dirty, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->hdr, sizeof(DevHeader));
+  if (e == hipSuccess) e = hipMalloc(&c->depth_in, px * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->filtered, px * 4u);
+  if (e != hipSuccess) {
+    vhm_destroy(c);
+    return VHM_ERR_HIP;
+  }
+  if (initVolume(c) != VHM_OK) {
+    vhm_destroy(c);
+    return VHM_ERR_HIP;
+  }
+  *out = c;
+  return VHM_OK;
+}
+
+int vhm_destroy(vhm_ctx* c) {
+  if (!c) return VHM_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& ev : c->events) {
+    hipEventDestroy(ev.a);
+    hipEventDestroy(ev.b);
+  }
+  void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
+                  c->obs,          c->dirty,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
+                  c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return VHM_OK;
+}
+
+const char* vhm_last_error(const vhm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int vhm_reset(vhm_ctx* c) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  return initVolume(c);
+}
+
+int vhm_integrate_f32(vhm_ctx* c, const float* depth, int memkind, const vhm_intrinsics* K, const vhm_pose* T_wc,
+                      int stride, uint32_t flags) {
+  return integrateFrame<float>(c, depth, false, 1.0f, memkind, K, T_wc, stride, flags);
+}
+
+int vhm_integrate_u16(vhm_ctx* c, const uint16_t* depth, float scale, int memkind, const vhm_intrinsics* K,
+                      const vhm_pose* T_wc, int stride, uint32_t flags) {
+  return integrateFrame<uint16_t>(c, depth, true, scale, memkind, K, T_wc, stride, flags);
+}
+
+static int filterEntry(vhm_ctx* c, const void* depth, bool u16, float scale, int in_kind, const vhm_intrinsics* K,
+                       float* out, int out_kind) {
+  int rc = checkFrame(c, depth, K);
+  if (rc) return rc;
+  if (!out) return fail(c, VHM_ERR_INVALID_ARGUMENT, "filter failed: null output");
+  const void* dev = nullptr;
+  if (u16)
+    rc = stageDepth(c, static_cast<const uint16_t*>(depth), in_kind, K, reinterpret_cast<const uint16_t**>(&dev));
+  else
+    rc = stageDepth(c, static_cast<const float*>(depth), in_kind, K, reinterpret_cast<const float**>(&dev));
+  if (rc) return rc;
+  float* dst = out_kind == VHM_DEVICE ? out : c->filtered;
+  if ((rc = filterStage(c, dev, u16, scale, K, dst))) return rc;
+  if (out_kind != VHM_DEVICE)
+    VHM_TRY(c, hipMemcpyAsync(out, c->filtered, sizeof(float) * size_t(K->width) * K->height, hipMemcpyDeviceToHost,
+                              c->stream),
+            "hipMemcpy D2H filtered depth");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return VHM_OK;
+}
+
+int vhm_filter_depth_f32(vhm_ctx* c, const float* depth, int in_kind, const vhm_intrinsics* K, float* out,
+                         int out_kind) {
+  return filterEntry(c, depth, false, 1.0f, in_kind, K, out, out_kind);
+}
+
+int vhm_filter_depth_u16(vhm_ctx* c, const uint16_t* depth, float scale, int in_kind, const vhm_intrinsics* K,
+                         float* out, int out_kind) {
+  return filterEntry(c, depth, true, scale, in_kind, K, out, out_kind);
+}
+
+int vhm_allocate_active_blocks(vhm_ctx* c, const float* depth, int memkind, const vhm_intrinsics* K,
+                               const vhm_pose* T_wc, int stride, int32_t* keys_out, size_t capacity, size_t* n_out) {
+  int rc = checkFrame(c, depth, K);
+  if (rc) return rc;
+  if (!T_wc || !n_out) return fail(c, VHM_ERR_INVALID_ARGUMENT, "allocate failed: null argument");
+  const float* dev = nullptr;
+  if ((rc = stageDepth(c, depth, memkind, K, &dev))) return rc;
+  if ((rc = allocStage(c, dev, K, T_wc, stride))) return rc;
+  DevHeader h{};
+  if ((rc = readHeader(c, &h))) return rc;
+  *n_out = h.active_count;
+  const size_t m = capacity < h.active_count ? capacity : h.active_count;
+  if (m > 0 && keys_out) {
+    if ((rc = ensure(c, &c->key_buf, &c->key_buf_cap, 3 * size_t(h.active_count), "hipMalloc key buffer")))
+      return rc;
+    VHM_TRY(c, launchActiveKeys(view(c), c->key_buf, c->stream, h.active_count), "active key gather launch");
+    VHM_TRY(c, hipMemcpyAsync(keys_out, c->key_buf, 3 * m * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H keys");
+    VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  }
+  return VHM_OK;
+}
+
+int vhm_synchronize(vhm_ctx* c) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  return readHeader(c, &h);
+}
+
+int vhm_block_count(vhm_ctx* c, size_t* out) {
+  if (!c || !out) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  const int rc = readHeader(c, &h);
+  *out = h.num_blocks < c->P.max_blocks ? h.num_blocks : c->P.max_blocks;
+  return rc;
+}
+
+int vhm_observed_voxel_count(vhm_ctx* c, size_t* out) {
+  if (!c || !out) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  const int rc = readHeader(c, &h);
+  *out = static_cast<size_t>(h.observed);
+  return rc;
+}
+
+int vhm_stats(vhm_ctx* c, uint64_t* updates, uint64_t* visited, uint64_t* frames) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  const int rc = readHeader(c, &h);
+  if (updates) *updates = h.updates;
+  if (visited) *visited = h.visited;
+  if (frames) *frames = c->frames_integrated;
+  return rc;
+}
+
+int vhm_block_keys(vhm_ctx* c, int32_t* keys_out, size_t capacity, size_t* n_out) {
+  size_t n = 0;
+  int rc = vhm_block_count(c, &n);
+  if (rc) return rc;
+  if (n_out) *n_out = n;
+  const size_t m = capacity < n ? capacity : n;
+  if (m == 0 || !keys_out) return VHM_OK;
+  std::vector<uint64_t> packed(m);
+  VHM_TRY(c, hipMemcpy(packed.data(), c->pool_keys, m * sizeof(uint64_t), hipMemcpyDeviceToHost),
+          "hipMemcpy D2H pool keys");
+  for (size_t i = 0; i < m; ++i) {
+    keys_out[3 * i] = keyX(packed[i]);
+    keys_out[3 * i + 1] = keyY(packed[i]);
+    keys_out[3 * i + 2] = keyZ(packed[i]);
+  }
+  return VHM_OK;
+}
+
+int vhm_download_blocks(vhm_ctx* c, const int32_t* keys, size_t n, void* out_host) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  if (n == 0) return VHM_OK;
+  if (!keys || !out_host) return fail(c, VHM_ERR_INVALID_ARGUMENT, "download failed: null argument");
+  DevHeader h{};
+  int rc = readHeader(c, &h);
+  if (rc) return rc;
+  const size_t chunk = 512;
+  if ((rc = ensure(c, &c->key_buf, &c->key_buf_cap, 3 * chunk, "hipMalloc key buffer"))) return rc;
+  if ((rc = ensure(c, &c->gather_buf, &c->gather_cap, chunk * kBlockWords, "hipMalloc gather buffer"))) return rc;
+  for (size_t off = 0; off < n; off += chunk) {
+    const size_t m = (n - off) < chunk ? (n - off) : chunk;
+    VHM_TRY(c, hipMemcpyAsync(c->key_buf, keys + 3 * off, 3 * m * sizeof(int32_t), hipMemcpyHostToDevice, c->stream),
+            "hipMemcpy H2D keys");
+    VHM_TRY(c, launchGatherBlocksAoS(c->key_buf, static_cast<uint32_t>(m), view(c), c->gather_buf, c->stream),
+            "block gather launch");
+    VHM_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(out_host) + off * kBlockWords * 4u, c->gather_buf,
+                              m * kBlockWords * 4u, hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H blocks");
+    VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  }
+  return VHM_OK;
+}
+
+int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, size_t* n_out) {
+  if (!c || !n_out) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  int rc = readHeader(c, &h);
+  if (rc) return rc;
+  const uint32_t nb = h.num_blocks < c->P.max_blocks ? h.num_blocks : c->P.max_blocks;
+  *n_out = 0;
+  if (nb == 0) return VHM_OK;
+  const size_t entries = size_t(nb) * kDirs;
+  if (entries > 0xffffffffull) return fail(c, VHM_ERR_CAPACITY, "mesh extraction failed: too many blocks");
+  if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
+  if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
+  if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
+                   "hipMalloc mesh partials")))
+    return rc;
+  MeshArgs ma{};
+  ma.voxel = c->P.voxel_size;
+  ma.iso = iso;
+  ma.extract_tol = c->P.voxel_size * c->P.layer_extract_scale;
+  ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
+  hipEvent_t ev{};
+  beginTimed(c, 3, &ev);
+  VHM_TRY(c, launchMeshCount(view(c), nb, ma, c->mesh_counts, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
+          "mesh scan kernel launch");
+  uint32_t total = 0;
+  const size_t tiles = scanPartialsSize(static_cast<uint32_t>(entries)) - 1;
+  VHM_TRY(c, hipMemcpyAsync(&total, c->mesh_partials + tiles, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H triangle count");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *n_out = total;
+  if (tris_out && capacity > 0 && total > 0) {
+    if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
+    VHM_TRY(c, launchMeshEmit(view(c), nb, ma, c->mesh_offsets, c->tris, total, c->stream),
+            "mesh emit kernel launch");
+    const size_t m = capacity < total ? capacity : total;
+    VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, m * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H triangles");
+  }
+  endTimed(c, ev);
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (c->timing) drainEvents(c);
+  return VHM_OK;
+}
+
+int vhm_set_timing(vhm_ctx* c, int enabled) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  hipStreamSynchronize(c->stream);
+  drainEvents(c);
+  c->timing = enabled != 0;
+  for (int i = 0; i < 4; ++i) {
+    c->ms[i] = 0.0;
+    c->launches[i] = 0;
+  }
+  return VHM_OK;
+}
+
+int vhm_kernel_times(vhm_ctx* c, double ms_out[4], uint64_t launches_out[4]) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  drainEvents(c);
+  for (int i = 0; i < 4; ++i) {
+    if (ms_out) ms_out[i] = c->ms[i];
+    if (launches_out) launches_out[i] = c->launches[i];
+  }
+  return VHM_OK;
+}
+
+int vhm_synth_render(const vhm_scene* scene, const vhm_intrinsics* K, const vhm_pose* poses, int n_frames,
+                     uint32_t first_frame, const vhm_noise* noise, uint16_t* out, int out_kind) {
+  if (!scene || !K || !poses || !out || n_frames < 0 || K->width <= 0 || K->height <= 0)
+    return VHM_ERR_INVALID_ARGUMENT;
+  if (scene->num_primitives < 0 || scene->num_primitives > 256 || (scene->num_primitives > 0 && !scene->primitives))
+    return VHM_ERR_INVALID_ARGUMENT;
+  if (n_frames == 0) return VHM_OK;
+  std::vector<float> prims(8 * size_t(scene->num_primitives));
+  for (int i = 0; i < scene->num_primitives; ++i) {
+    const vhm_primitive& p = scene->primitives[i];
+    const float v[8] = {static_cast<float>(p.kind), p.c[0], p.c[1], p.c[2], p.r, p.h[0], p.h[1], p.h[2]};
+    std::memcpy(&prims[8 * size_t(i)], v, sizeof(v));
+  }
+  SynthDeviceScene sc{};
+  for (int i = 0; i < 3; ++i) {
+    sc.room_min[i] = scene->room_min[i];
+    sc.room_max[i] = scene->room_max[i];
+  }
+  sc.num_primitives = scene->num_primitives;
+  SynthNoise nz{};
+  if (noise) {
+    nz.a = noise->a;
+    nz.b = noise->b;
+    nz.c = noise->c;
+    nz.dropout = noise->dropout;
+    nz.max_depth = noise->max_depth;
+    nz.key0 = static_cast<uint32_t>(noise->seed);
+    nz.key1 = static_cast<uint32_t>(noise->seed >> 32);
+    nz.enabled = noise->enabled;
+  }
+  std::vector<Pose34> P(static_cast<size_t>(n_frames));
+  for (int i = 0; i < n_frames; ++i) P[i] = toPose(&poses[i]);
+  const Intrinsics k = toK(K);
+  if (out_kind != VHM_DEVICE) {
+    sc.prims = prims.data();
+    synthRenderHost(sc, k, P.data(), n_frames, first_frame, nz, out);
+    return VHM_OK;
+  }
+  float* dprims = nullptr;
+  Pose34* dposes = nullptr;
+  hipError_t e = hipMalloc(&dprims, sizeof(float) * (prims.size() + 8));
+  if (e == hipSuccess) e = hipMalloc(&dposes, sizeof(Pose34) * P.size());
+  if (e == hipSuccess) e = hipMemcpy(dprims, prims.data(), sizeof(float) * prims.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dposes, P.data(), sizeof(Pose34) * P.size(), hipMemcpyHostToDevice);
+  sc.prims = dprims;
+  if (e == hipSuccess) e = launchSynthRender(sc, k, dposes, n_frames, first_frame, nz, out, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (dprims) hipFree(dprims);
+  if (dposes) hipFree(dposes);
+  return e == hipSuccess ? VHM_OK : VHM_ERR_HIP;
+}
+
+void vhm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  vhm::philox4x32_10(ctr, key, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// Depth bilateral filter: per-pixel normals and the 5x5 normal-aware
+// bilateral filter fused in one LDS-tiled kernel.
+//
+// Reference: estimateSurfaceNormalAtPixel (reconstruction_backend.cpp:57-94)
+// and applyDepthBilateralFilter (reconstruction_backend.cpp:96-189); the CUDA
+// twin is two kernels with a global round trip of the normals
+// (cuda_tsdf_integrator.cu:629-725).
+//
+// Tiling: a workgroup of 256 lanes (4 wavefronts) filters a 32x8 pixel tile.
+// LDS holds the tile's depth with a 3-pixel apron (38x14, converted from
+// uint16 once) and the normals of the tile with a 2-pixel apron (36x12), so
+// every depth sample is read from HBM once per tile (plus the apron) and the
+// normals never leave the CU.  Spatial Gaussian weights are precomputed on the
+// host; the range and angular Gaussians use expf_glibc.
+#include "vhm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace vhm {
+namespace {
+
+constexpr int kTX = 32, kTY = 8, kR = 2;
+constexpr int kDW = kTX + 2 * (kR + 1), kDH = kTY + 2 * (kR + 1);  // depth tile 38x14
+constexpr int kNW = kTX + 2 * kR, kNH = kTY + 2 * kR;              // normal tile 36x12
+
+template <typename In>
+__device__ __forceinline__ float loadDepth(const In* in, size_t i, float scale);
+template <>
+__device__ __forceinline__ float loadDepth<float>(const float* in, size_t i, float) {
+  return in[i];
+}
+template <>
+__device__ __forceinline__ float loadDepth<uint16_t>(const uint16_t* in, size_t i, float scale) {
+  // dataset_io.h:342-356: 0 stays 0, otherwise raw * depth_scale (kept if > 0).
+  const uint16_t raw = in[i];
+  if (raw == 0u) return 0.0f;
+  const float d = static_cast<float>(raw) * scale;
+  return d > 0.0f ? d : 0.0f;
+}
+
+template <typename In>
+__global__ __launch_bounds__(256) void filterKernel(const In* __restrict__ in, float* __restrict__ out, FilterArgs a) {
+  __shared__ float sd[kDH][kDW];
+  __shared__ float snx[kNH][kNW], sny[kNH][kNW], snz[kNH][kNW];
+  __shared__ uint8_t sok[kNH][kNW];
+
+  const int W = a.K.width, H = a.K.height;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
+  const int dx0 = x0 - (kR + 1), dy0 = y0 - (kR + 1);
+
+  for (int i = tid; i < kDW * kDH; i += 256) {
+    const int lx = i % kDW, ly = i / kDW;
+    const int gx = dx0 + lx, gy = dy0 + ly;
+    float d = 0.0f;
+    if (gx >= 0 && gx < W && gy >= 0 && gy < H) d = loadDepth<In>(in, static_cast<size_t>(gy) * W + gx, a.scale);
+    sd[ly][lx] = d;
+  }
+  __syncthreads();
+
+  // Normals for the tile plus a 2-pixel apron (reconstruction_backend.cpp:57-94).
+  for (int i = tid; i < kNW * kNH; i += 256) {
+    const int lx = i % kNW, ly = i / kNW;
+    const int u = x0 - kR + lx, v = y0 - kR + ly;
+    const int cx = lx + 1, cy = ly + 1;  // position in the depth tile
+    bool ok = false;
+    F3 n{0.0f, 0.0f, 0.0f};
+    if (!(u <= 0 || v <= 0 || u + 1 >= W || v + 1 >= H)) {
+      const float c = sd[cy][cx], l = sd[cy][cx - 1], r = sd[cy][cx + 1], up = sd[cy - 1][cx], dn = sd[cy + 1][cx];
+      if (!(c <= 0.0f || l <= 0.0f || r <= 0.0f || up <= 0.0f || dn <= 0.0f)) {
+        const float thr = discontinuityThreshold(c, a.disc_base, a.disc_rel);
+        if (!(fabsf(l - c) > thr || fabsf(r - c) > thr || fabsf(up - c) > thr || fabsf(dn - c) > thr)) {
+          const F3 pl = backProjectCont(static_cast<float>(u - 1), static_cast<float>(v), l, a.K);
+          const F3 pr = backProjectCont(static_cast<float>(u + 1), static_cast<float>(v), r, a.K);
+          const F3 pu = backProjectCont(static_cast<float>(u), static_cast<float>(v - 1), up, a.K);
+          const F3 pd = backProjectCont(static_cast<float>(u), static_cast<float>(v + 1), dn, a.K);
+          n = normalized3(cross3(sub3(pd, pu), sub3(pr, pl)));
+          if (!(norm3(n) <= 1e-6f)) {
+            const F3 cp = backProjectCont(static_cast<float>(u), static_cast<float>(v), c, a.K);
+            n = orientTowardCamera(n, cp);
+            ok = norm3(n) > 1e-6f;
+          }
+        }
+      }
+    }
+    snx[ly][lx] = n.x;
+    sny[ly][lx] = n.y;
+    snz[ly][lx] = n.z;
+    sok[ly][lx] = ok ? 1 : 0;
+  }
+  __syncthreads();
+
+  const int tx = tid % kTX, ty = tid / kTX;
+  const int u = x0 + tx, v = y0 + ty;
+  if (u >= W || v >= H) return;
+  const int cdx = tx + kR + 1, cdy = ty + kR + 1;  // centre in the depth tile
+  const int cnx = tx + kR, cny = ty + kR;          // centre in the normal tile
+  const float c = sd[cdy][cdx];
+  float result = 0.0f;
+  if (c > 0.0f) {
+    const bool has_n = sok[cny][cnx] != 0;
+    const F3 cn = has_n ? F3{snx[cny][cnx], sny[cny][cnx], snz[cny][cnx]} : F3{0.0f, 0.0f, 0.0f};
+    const float thr = discontinuityThreshold(c, a.disc_base, a.disc_rel);
+    float wd = 0.0f, ws = 0.0f;
+    for (int dv = -kR; dv <= kR; ++dv) {
+      const int sv = v + dv;
+      if (sv < 0 || sv >= H) continue;
+      for (int du = -kR; du <= kR; ++du) {
+        const int su = u + du;
+        if (su < 0 || su >= W) continue;
+        const float s = sd[cdy + dv][cdx + du];
+        if (s <= 0.0f) continue;
+        if (fabsf(s - c) > thr) continue;
+        const float dd = s - c;
+        float w = a.spatial_w[(dv + kR) * (2 * kR + 1) + (du + kR)] * gaussianWeight(dd * dd, a.inv_r);
+        if (has_n) {
+          if (sok[cny + dv][cnx + du] == 0) continue;
+          const F3 sn{snx[cny + dv][cnx + du], sny[cny + dv][cnx + du], snz[cny + dv][cnx + du]};
+          const float cosv = sclamp(dot3(cn, sn), -1.0f, 1.0f);
+          if (cosv < a.min_cos) continue;
+          const float ad = 1.0f - cosv;
+          w *= gaussianWeight(ad * ad, a.inv_a);
+        }
+        wd += w * s;
+        ws += w;
+      }
+    }
+    result = ws > 1e-6f ? (wd / ws) : c;
+  }
+  out[static_cast<size_t>(v) * W + u] = result;
+}
+
+}  // namespace
+
+hipError_t launchFilterF32(const float* in, float* out, const FilterArgs& a, hipStream_t s) {
+  const dim3 grid((a.K.width + kTX - 1) / kTX, (a.K.height + kTY - 1) / kTY);
+  filterKernel<float><<<grid, 256, 0, s>>>(in, out, a);
+  return hipGetLastError();
+}
+
+hipError_t launchFilterU16(const uint16_t* in, float* out, const FilterArgs& a, hipStream_t s) {
+  const dim3 grid((a.K.width + kTX - 1) / kTX, (a.K.height + kTY - 1) / kTY);
+  filterKernel<uint16_t><<<grid, 256, 0, s>>>(in, out, a);
+  return hipGetLastError();
+}
+
+}  // namespace vhm

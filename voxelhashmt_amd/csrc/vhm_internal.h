@@ -1,0 +1,129 @@
+// Device-side data layout and kernel launchers shared by the HIP translation
+// units of libvhm.so.  Not part of the public C-ABI (include/vhm.h).
+//
+// HBM layout of one volume (one context):
+//   hash table   keys[2^k] u64 packed block key (empty = ~0)
+//                vals[2^k] u32 pool index of the block (~0 = none)
+//                stamp[2^k] u32 last frame id that touched the key
+//   block pool   pool[max_blocks][6 dir][4 field][3 slot][512 voxel] (f32/u32)
+//                field 0 weight, 1 surface_coord, 2 plane_offset,
+//                3 packed_normal.  One (dir, field, slot) row is 2 KiB of
+//                contiguous voxels, so a wavefront's 64 lanes read 256 B.
+//                147,456 B per block, as the reference's Voxel array.
+//                pool_keys[max_blocks] u64, obs[max_blocks][16] u32 observed
+//                bitmask, dirty[max_blocks] u32.
+//   per frame    active[2^k] u32 hash slots touched this frame,
+//                depth staging, filtered depth (W x H f32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vhm_math.h"
+
+namespace vhm {
+
+constexpr int kBlockSide = 8;
+constexpr int kBlockVoxels = 512;
+constexpr int kDirs = 6;
+constexpr int kSlots = 3;
+constexpr int kFields = 4;
+constexpr size_t kBlockWords = static_cast<size_t>(kDirs) * kFields * kSlots * kBlockVoxels;  // 36864
+constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr int kMaxProbe = 1024;
+
+// Sticky device error bits.
+constexpr uint32_t kErrPoolFull = 1u;
+constexpr uint32_t kErrHashFull = 2u;
+constexpr uint32_t kErrKeyRange = 4u;
+
+struct DevHeader {
+  uint32_t num_blocks;    // pool slots handed out (can pass max_blocks on overflow)
+  uint32_t active_count;  // keys touched by the current frame
+  uint32_t error;         // sticky kErr* bits
+  uint32_t frame;
+  unsigned long long observed;  // voxels with any layer weight > 0
+  unsigned long long updates;   // voxel updates (weight > 0, valid direction)
+  unsigned long long visited;   // voxels visited (512 per integrated block)
+  unsigned long long mesh_total;
+};
+
+struct VolumeView {
+  uint64_t* keys;
+  uint32_t* vals;
+  uint32_t* stamp;
+  uint32_t log2cap;
+  uint32_t mask;
+  uint64_t* pool_keys;
+  uint32_t* pool;  // float fields are stored as their bit patterns
+  uint32_t* obs;
+  uint32_t* dirty;
+  uint32_t max_blocks;
+  uint32_t* active;
+  DevHeader* hdr;
+};
+
+VHM_HD size_t fieldOffset(int dir, int field, int slot) {
+  return ((static_cast<size_t>(dir) * kFields + field) * kSlots + slot) * kBlockVoxels;
+}
+
+struct FilterArgs {
+  Intrinsics K;
+  float scale;          // u16 -> metres
+  float spatial_w[25];  // exp(-r^2 * inv_s) for the 5x5 taps, row-major dv, du
+  float inv_r, inv_a;
+  float disc_base, disc_rel, min_cos;
+};
+
+struct AllocArgs {
+  Intrinsics K;
+  Pose34 T_wc;
+  float voxel, trunc;
+  int stride;
+  uint32_t frame;
+};
+
+struct IntegrateArgs {
+  Intrinsics K;
+  Pose34 T_wc, T_cw;
+  float voxel, trunc;
+  float merge_tol, min_cos, max_weight, min_view;
+  float disc_base, disc_rel;
+  float spatial_w[3];  // normal bilateral exp(-s * inv_s), s = 0, 1, 2
+  float inv_r, inv_a;
+};
+
+struct MeshArgs {
+  float voxel, iso, extract_tol;
+};
+
+// Launchers (each defined next to its kernel).  All are asynchronous on `s`.
+hipError_t launchFilterF32(const float* in, float* out, const FilterArgs& a, hipStream_t s);
+hipError_t launchFilterU16(const uint16_t* in, float* out, const FilterArgs& a, hipStream_t s);
+hipError_t launchFrameBegin(DevHeader* hdr, uint32_t frame, hipStream_t s);
+hipError_t launchAllocate(const float* depth, const VolumeView& V, const AllocArgs& a, hipStream_t s);
+hipError_t launchIntegrate(const float* depth, const VolumeView& V, const IntegrateArgs& a, int grid,
+                           hipStream_t s);
+hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeView& V, uint32_t* out,
+                                 hipStream_t s);
+hipError_t launchMeshCount(const VolumeView& V, uint32_t num_blocks, const MeshArgs& a, uint32_t* counts,
+                           hipStream_t s);
+hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s);
+size_t scanPartialsSize(uint32_t n);
+hipError_t launchMeshEmit(const VolumeView& V, uint32_t num_blocks, const MeshArgs& a, const uint32_t* offsets,
+                          float* tris, uint32_t capacity, hipStream_t s);
+
+struct SynthDeviceScene {
+  float room_min[3], room_max[3];
+  int num_primitives;
+  const float* prims;  // 8 floats each: kind, cx, cy, cz, r, hx, hy, hz
+};
+struct SynthNoise {
+  float a, b, c, dropout, max_depth;
+  uint32_t key0, key1;
+  int enabled;
+};
+hipError_t launchSynthRender(const SynthDeviceScene& sc, const Intrinsics& K, const Pose34* poses, int n_frames,
+                             uint32_t first_frame, const SynthNoise& nz, uint16_t* out, hipStream_t s);
+
+}  // namespace vhm

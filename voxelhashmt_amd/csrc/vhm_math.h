@@ -1,0 +1,233 @@
+// Scalar math shared by the HIP kernels (and compiled for the host by the
+// exhaustive expf check).  Every function performs the reference CPU path's
+// float operations in the reference's order, so that with FP contraction off
+// the GPU reproduces the reference bit for bit:
+//   * vector ops, normalized, dot, cross          math_types.h:44-101
+//   * std::max / std::min / clampf semantics      math_types.h:12-14
+//   * pinhole back-projection / projection        math_types.h:228-244,
+//                                                 voxel_hash_tsdf.h:586-593
+//   * octahedral normal packing                   voxel_hash_tsdf.h:268-309
+//   * exp of the Gaussian weights                 voxel_hash_tsdf.h:552-554,
+//                                                 reconstruction_backend.cpp:34-36
+//
+// exp: the reference calls std::exp(float), i.e. glibc's expf.  vhm_expf is
+// an implementation of the published table-driven method glibc uses (Szabolcs
+// Nagy's expf from ARM optimized-routines, in glibc since 2.28): x*32/ln2 =
+// k + r, 2^(k/32) from a 32-entry table of 2^(i/32), a degree-3 polynomial in
+// r, all in double with fused multiply-adds, rounded once to float.  The table
+// below is 2^(i/32) rounded to double with i<<47 subtracted from its bit
+// pattern (it is recomputed from first principles in the tests).  The check
+// against the host's glibc expf over every float in [-103, 88] (2.24e9
+// inputs) found no difference.
+
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#pragma clang fp contract(off)
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define VHM_HD __host__ __device__ __forceinline__
+#define VHM_TABLE __device__ __constant__
+#else
+#define VHM_HD inline
+#define VHM_TABLE
+#endif
+
+namespace vhm {
+
+VHM_TABLE static const uint64_t kExp2Tab32[32] = {
+    0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
+    0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
+    0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
+    0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
+    0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
+    0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
+    0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
+    0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull};
+
+VHM_HD double bitsToDouble(uint64_t u) { return __builtin_bit_cast(double, u); }
+VHM_HD uint64_t doubleToBits(double d) { return __builtin_bit_cast(uint64_t, d); }
+
+// exp(x) for float x, identical to glibc 2.35 expf on an FMA-capable x86-64.
+VHM_HD float expf_glibc(float x) {
+  if (!(x == x)) return x + x;
+  if (x < -0x1.9fe368p6f) return 0.0f;                 // below log(2^-150), and -inf
+  if (x > 0x1.62e42ep6f) return __builtin_inff();      // above log(2^128)
+  const double kInvLn2N = 0x1.71547652b82fep+0 * 32.0;
+  const double kShift = 0x1.8p+52;
+  const double kC0 = 0x1.c6af84b912394p-5 / 32.0 / 32.0 / 32.0;
+  const double kC1 = 0x1.ebfce50fac4f3p-3 / 32.0 / 32.0;
+  const double kC2 = 0x1.62e42ff0c52d6p-1 / 32.0;
+  const double xd = static_cast<double>(x);
+  double kd = __builtin_fma(kInvLn2N, xd, kShift);     // round(x*32/ln2) in the low bits
+  const uint64_t ki = doubleToBits(kd);
+  kd -= kShift;
+  const double r = __builtin_fma(kInvLn2N, xd, -kd);
+  const uint64_t t = kExp2Tab32[ki & 31u] + (ki << 47);
+  const double s = bitsToDouble(t);
+  const double z = __builtin_fma(kC0, r, kC1);
+  const double r2 = r * r;
+  double y = __builtin_fma(kC2, r, 1.0);
+  y = __builtin_fma(z, r2, y);
+  y = y * s;
+  return static_cast<float>(y);
+}
+
+// ----------------------------------------------------------------- vectors --
+
+struct F3 {
+  float x, y, z;
+};
+
+VHM_HD F3 add3(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+VHM_HD F3 sub3(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+VHM_HD F3 mul3(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+VHM_HD F3 div3(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+VHM_HD float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+VHM_HD F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+VHM_HD float norm3(F3 v) { return sqrtf(dot3(v, v)); }
+VHM_HD F3 normalized3(F3 v) {
+  const float len = norm3(v);
+  if (len <= 1e-8f) return {0.0f, 0.0f, 0.0f};
+  return div3(v, len);
+}
+
+// std::max(a, b) / std::min(a, b) / clampf exactly (no NaN/-0 reinterpretation).
+VHM_HD float smax(float a, float b) { return (a < b) ? b : a; }
+VHM_HD float smin(float a, float b) { return (b < a) ? b : a; }
+VHM_HD float sclamp(float v, float lo, float hi) { return smax(lo, smin(hi, v)); }
+VHM_HD int imin(int a, int b) { return (b < a) ? b : a; }
+
+// std::lround for the non-huge floats used here (round half away from zero).
+VHM_HD int lroundi(float x) { return static_cast<int>(roundf(x)); }
+
+// Row-major rotation + translation (Pose, math_types.h:180-198).
+struct Pose34 {
+  float R[9];
+  float t[3];
+};
+
+VHM_HD F3 rotate(const Pose34& P, F3 v) {
+  return {P.R[0] * v.x + P.R[1] * v.y + P.R[2] * v.z, P.R[3] * v.x + P.R[4] * v.y + P.R[5] * v.z,
+          P.R[6] * v.x + P.R[7] * v.y + P.R[8] * v.z};
+}
+VHM_HD F3 transformPoint(const Pose34& P, F3 p) { return add3(rotate(P, p), F3{P.t[0], P.t[1], P.t[2]}); }
+
+// Pose::inverse (math_types.h:192-197): R^T and R^T * (t * -1).
+VHM_HD Pose34 inversePose(const Pose34& P) {
+  Pose34 o;
+  o.R[0] = P.R[0]; o.R[1] = P.R[3]; o.R[2] = P.R[6];
+  o.R[3] = P.R[1]; o.R[4] = P.R[4]; o.R[5] = P.R[7];
+  o.R[6] = P.R[2]; o.R[7] = P.R[5]; o.R[8] = P.R[8];
+  const F3 mt{P.t[0] * -1.0f, P.t[1] * -1.0f, P.t[2] * -1.0f};
+  const F3 r = rotate(o, mt);
+  o.t[0] = r.x;
+  o.t[1] = r.y;
+  o.t[2] = r.z;
+  return o;
+}
+
+struct Intrinsics {
+  int width, height;
+  float fx, fy, cx, cy;
+};
+
+VHM_HD F3 backProjectPixel(int u, int v, float d, const Intrinsics& K) {
+  return {(static_cast<float>(u) - K.cx) * d / K.fx, (static_cast<float>(v) - K.cy) * d / K.fy, d};
+}
+VHM_HD F3 backProjectCont(float u, float v, float d, const Intrinsics& K) {
+  return {(u - K.cx) * d / K.fx, (v - K.cy) * d / K.fy, d};
+}
+VHM_HD bool projectPoint(F3 p, const Intrinsics& K, float& u, float& v) {
+  if (p.z <= 1e-6f) return false;
+  u = K.fx * p.x / p.z + K.cx;
+  v = K.fy * p.y / p.z + K.cy;
+  return u >= 0.0f && u <= static_cast<float>(K.width - 1) && v >= 0.0f && v <= static_cast<float>(K.height - 1);
+}
+
+VHM_HD float discontinuityThreshold(float d, float base, float rel) { return smax(base, rel * smax(d, 0.0f)); }
+
+VHM_HD F3 orientTowardCamera(F3 n, F3 p) {
+  if (norm3(n) <= 1e-6f || norm3(p) <= 1e-6f) return n;
+  const F3 ray = normalized3(p);
+  return dot3(n, ray) > 0.0f ? mul3(n, -1.0f) : n;
+}
+
+VHM_HD float gaussianWeight(float sq, float inv_two_sigma_sq) { return expf_glibc(-sq * inv_two_sigma_sq); }
+
+// ------------------------------------------------------- octahedral normals --
+
+VHM_HD float signNotZero(float v) { return v >= 0.0f ? 1.0f : -1.0f; }
+
+VHM_HD uint32_t packNormalOct(F3 raw) {
+  const F3 n = normalized3(raw);
+  const float s = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+  if (s <= 1e-6f) return 0u;
+  F3 p = div3(n, s);
+  if (p.z < 0.0f) {
+    const float px = (1.0f - fabsf(p.y)) * signNotZero(p.x);
+    const float py = (1.0f - fabsf(p.x)) * signNotZero(p.y);
+    p.x = px;
+    p.y = py;
+  }
+  const float qx = sclamp(p.x * 0.5f + 0.5f, 0.0f, 1.0f);
+  const float qy = sclamp(p.y * 0.5f + 0.5f, 0.0f, 1.0f);
+  return static_cast<uint32_t>(lroundi(qx * 65535.0f)) | (static_cast<uint32_t>(lroundi(qy * 65535.0f)) << 16u);
+}
+
+VHM_HD float dequantOct(uint32_t c) { return (static_cast<float>(c & 0xffffu) / 65535.0f) * 2.0f - 1.0f; }
+
+VHM_HD F3 unpackNormalOct(uint32_t packed) {
+  F3 n{dequantOct(packed), dequantOct(packed >> 16u), 0.0f};
+  n.z = 1.0f - fabsf(n.x) - fabsf(n.y);
+  if (n.z < 0.0f) {
+    const float px = (1.0f - fabsf(n.y)) * signNotZero(n.x);
+    const float py = (1.0f - fabsf(n.x)) * signNotZero(n.y);
+    n.x = px;
+    n.y = py;
+  }
+  return normalized3(n);
+}
+
+// ------------------------------------------------------------ block keys --
+
+// floorDiv / positiveMod (voxel_hash_tsdf.h:531-546).
+VHM_HD int floorDiv(int value, int divisor) {
+  int q = value / divisor;
+  const int r = value % divisor;
+  if (r != 0 && ((r < 0) != (divisor < 0))) --q;
+  return q;
+}
+VHM_HD int positiveMod(int value, int divisor) {
+  int r = value % divisor;
+  if (r < 0) r += divisor;
+  return r;
+}
+
+// Packed block key: 21 bits per coordinate with bias 2^20 (the reference's
+// device key packing, cuda_block_allocator.cu:19-22, 161-174); all ones = empty.
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr int kKeyBits = 21;
+constexpr int kKeyBias = 1 << (kKeyBits - 1);
+constexpr uint64_t kKeyMask = (1ull << kKeyBits) - 1ull;
+
+VHM_HD bool keyInRange(int x, int y, int z) {
+  return x >= -kKeyBias && x < kKeyBias && y >= -kKeyBias && y < kKeyBias && z >= -kKeyBias && z < kKeyBias;
+}
+VHM_HD uint64_t packKey(int x, int y, int z) {
+  return static_cast<uint64_t>(x + kKeyBias) | (static_cast<uint64_t>(y + kKeyBias) << kKeyBits) |
+         (static_cast<uint64_t>(z + kKeyBias) << (2 * kKeyBits));
+}
+VHM_HD int keyX(uint64_t k) { return static_cast<int>(k & kKeyMask) - kKeyBias; }
+VHM_HD int keyY(uint64_t k) { return static_cast<int>((k >> kKeyBits) & kKeyMask) - kKeyBias; }
+VHM_HD int keyZ(uint64_t k) { return static_cast<int>((k >> (2 * kKeyBits)) & kKeyMask) - kKeyBias; }
+
+// Fibonacci hashing of the packed key onto a power-of-two table.
+VHM_HD uint32_t hashSlot(uint64_t key, uint32_t log2cap) {
+  return static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> (64u - log2cap));
+}
+
+}  // namespace vhm

@@ -1,0 +1,402 @@
+// Directional multi-layer marching cubes in two passes (count, then emit
+// at scanned offsets), with a deterministic output order.
+//
+// Reference: VoxelHashTSDF::extractMesh / extractDirectionalMultiLayerMesh
+// (voxel_hash_tsdf.h:99-107, 842-939) with selectClosestLayerForSurface
+// (394-421), completeCubeValuesWithFallback (423-477), planeSignedDistance
+// (336-339) and polygoniseCube (marching_cubes.h:34-94).  CUDA twin:
+// countTrianglesKernel / extractTrianglesKernel launched once per direction
+// with a host-built lookup table and an atomic output cursor
+// (cuda_mesh_extractor.cu:553-720).
+//
+// One workgroup handles one (direction, block): it looks up the 7 positive
+// neighbour blocks in the persistent hash table, stages the 9x9x9 corner
+// region of that direction in LDS (unpacked slot normals, plane offsets,
+// observed bits -- each voxel's normals are unpacked once instead of once per
+// cube corner), and each lane runs two cubes.  Candidate plane distances are
+// kept sorted in a per-lane LDS column; clusters are consumed as they close,
+// so no cluster array is stored.  Triangles of a workgroup are written in
+// cube order at offset[(dir, block)] + the workgroup's exclusive scan.
+#include "vhm_internal.h"
+#include "vhm_mc_tables.h"
+
+#pragma clang fp contract(off)
+
+namespace vhm {
+namespace {
+
+__device__ __constant__ McTables c_mc = makeMcTables();
+
+constexpr int kR = 9;  // corner region edge (8 voxels + 1 from the + neighbours)
+constexpr int kRegion = kR * kR * kR;
+constexpr int kMaxCand = 24;
+
+struct MeshShared {
+  float nx[kSlots][kRegion], ny[kSlots][kRegion], nz[kSlots][kRegion], po[kSlots][kRegion];
+  uint8_t bits[kRegion];  // bit k: slot k observed; bit 3: voxel's block allocated
+  uint32_t src[8];        // pool index of self + 7 positive neighbours (kInvalid if absent)
+  float cand[kMaxCand][256];
+  uint32_t scan[256];
+  uint32_t total;
+};
+
+__device__ uint32_t lookupPool(const VolumeView& V, int x, int y, int z) {
+  if (!keyInRange(x, y, z)) return kInvalid;
+  const uint64_t key = packKey(x, y, z);
+  uint32_t slot = hashSlot(key, V.log2cap);
+  const int bound = (V.mask + 1u) < static_cast<uint32_t>(kMaxProbe) ? static_cast<int>(V.mask + 1u) : kMaxProbe;
+  for (int probe = 0; probe < bound; ++probe) {
+    const uint64_t cur = V.keys[slot];
+    if (cur == key) return V.vals[slot];
+    if (cur == kEmptyKey) return kInvalid;
+    slot = (slot + 1u) & V.mask;
+  }
+  return kInvalid;
+}
+
+__device__ void stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, uint32_t self, int dir) {
+  const int tid = threadIdx.x;
+  if (tid < 8) {
+    const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
+    uint32_t p = tid == 0 ? self : lookupPool(V, keyX(key) + ox, keyY(key) + oy, keyZ(key) + oz);
+    S.src[tid] = (p < V.max_blocks) ? p : kInvalid;
+  }
+  __syncthreads();
+  for (int r = tid; r < kRegion; r += 256) {
+    const int x = r % kR, y = (r / kR) % kR, z = r / (kR * kR);
+    const int sb = (x == 8 ? 1 : 0) | (y == 8 ? 2 : 0) | (z == 8 ? 4 : 0);
+    const uint32_t p = S.src[sb];
+    uint8_t b = 0;
+    if (p != kInvalid) {
+      b = 8;
+      const int vi = (x & 7) + kBlockSide * ((y & 7) + kBlockSide * (z & 7));
+      const uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) {
+        const float w = __uint_as_float(blk[fieldOffset(dir, 0, k) + vi]);
+        if (w > 0.0f) {
+          b |= static_cast<uint8_t>(1u << k);
+          const F3 n = unpackNormalOct(blk[fieldOffset(dir, 3, k) + vi]);
+          S.nx[k][r] = n.x;
+          S.ny[k][r] = n.y;
+          S.nz[k][r] = n.z;
+          S.po[k][r] = __uint_as_float(blk[fieldOffset(dir, 2, k) + vi]);
+        }
+      }
+    }
+    S.bits[r] = b;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float planeSD(const MeshShared& S, int k, int r, F3 p) {
+  return dot3(F3{S.nx[k][r], S.ny[k][r], S.nz[k][r]}, p) - S.po[k][r];
+}
+
+// completeCubeValuesWithFallback (voxel_hash_tsdf.h:423-477).
+__device__ bool completeCube(float val[8], uint32_t& valid) {
+  int n = 0;
+  float sum = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (valid & (1u << c)) {
+      ++n;
+      sum += val[c];
+    }
+  if (n == 8) return true;
+  if (n < 6) return false;
+  const float global = n > 0 ? (sum / static_cast<float>(n)) : 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (valid & (1u << c)) continue;
+    float ns = 0.0f;
+    int nc = 0;
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+      const int c0 = kMcEdgeCorners[e][0], c1 = kMcEdgeCorners[e][1];
+      int nb = -1;
+      if (c0 == c)
+        nb = c1;
+      else if (c1 == c)
+        nb = c0;
+      if (nb < 0 || !(valid & (1u << nb))) continue;
+      ns += val[nb];
+      ++nc;
+    }
+    val[c] = nc > 0 ? (ns / static_cast<float>(nc)) : global;
+    valid |= 1u << c;
+  }
+  return true;
+}
+
+__device__ __forceinline__ int cubeCase(const float val[8], float iso) {
+  int ci = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (val[c] < iso) ci |= 1 << c;
+  return ci;
+}
+
+// polygoniseCube (marching_cubes.h:48-94): writes the case's triangles.
+__device__ void emitCube(const F3 pos[8], const float val[8], float iso, float* out) {
+  const int ci = cubeCase(val, iso);
+  const int mask = c_mc.edge_mask[ci];
+  F3 ev[12];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    ev[e] = F3{0.0f, 0.0f, 0.0f};
+    if ((mask & (1 << e)) == 0) continue;
+    const int c0 = kMcEdgeCorners[e][0], c1 = kMcEdgeCorners[e][1];
+    const float delta = val[c1] - val[c0];
+    if (fabsf(delta) < 1e-6f) {
+      ev[e] = pos[c0];
+      continue;
+    }
+    const float t = sclamp((iso - val[c0]) / delta, 0.0f, 1.0f);
+    ev[e] = add3(pos[c0], mul3(sub3(pos[c1], pos[c0]), t));
+  }
+  const int nt = c_mc.num_tris[ci];
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const int e = c_mc.tri[ci][3 * i + j];
+      F3 q = ev[0];
+#pragma unroll
+      for (int m = 1; m < 12; ++m)
+        if (m == e) q = ev[m];
+      out[9 * i + 3 * j + 0] = q.x;
+      out[9 * i + 3 * j + 1] = q.y;
+      out[9 * i + 3 * j + 2] = q.z;
+    }
+}
+
+// One cube of extractDirectionalMultiLayerMesh (voxel_hash_tsdf.h:853-935).
+// Returns the triangle count; emits them at `out` when out != nullptr.
+__device__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a, float* out) {
+  const int tid = threadIdx.x;
+  const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
+  const int ox = keyX(key) * kBlockSide + lx, oy = keyY(key) * kBlockSide + ly, oz = keyZ(key) * kBlockSide + lz;
+  const float hv = 0.5f * a.voxel;
+  const F3 cc{(static_cast<float>(ox) + 0.5f) * a.voxel + hv, (static_cast<float>(oy) + 0.5f) * a.voxel + hv,
+              (static_cast<float>(oz) + 0.5f) * a.voxel + hv};
+  int rc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    rc[c] = (lx + kMcCornerOffset[c][0]) + kR * ((ly + kMcCornerOffset[c][1]) + kR * (lz + kMcCornerOffset[c][2]));
+
+  // candidates, kept sorted ascending (std::sort, voxel_hash_tsdf.h:887)
+  int nc = 0;
+  for (int c = 0; c < 8; ++c) {
+    const uint8_t b = S.bits[rc[c]];
+    if (!(b & 8)) continue;
+    for (int k = 0; k < kSlots; ++k) {
+      if (!(b & (1u << k))) continue;
+      const float s = planeSD(S, k, rc[c], cc);
+      int j = nc;
+      while (j > 0 && S.cand[j - 1][tid] > s) {
+        S.cand[j][tid] = S.cand[j - 1][tid];
+        --j;
+      }
+      S.cand[j][tid] = s;
+      ++nc;
+    }
+  }
+  if (nc == 0) return 0;
+
+  F3 pos[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    pos[c] = F3{(static_cast<float>(ox + kMcCornerOffset[c][0]) + 0.5f) * a.voxel,
+                (static_cast<float>(oy + kMcCornerOffset[c][1]) + 0.5f) * a.voxel,
+                (static_cast<float>(oz + kMcCornerOffset[c][2]) + 0.5f) * a.voxel};
+
+  uint32_t ntri = 0;
+  float centre = 0.0f;
+  int count = 0;
+  // Greedy clustering (voxel_hash_tsdf.h:888-907); a cluster is processed
+  // (909-934) when the next one opens or the candidates end.
+  for (int i = 0; i <= nc; ++i) {
+    const bool last = (i == nc);
+    const float sk = last ? 0.0f : S.cand[i][tid];
+    if (!last && count > 0 && !(fabsf(sk - centre) > a.extract_tol)) {
+      centre = (centre * static_cast<float>(count) + sk) / static_cast<float>(count + 1);
+      ++count;
+      continue;
+    }
+    if (count > 0) {
+      float val[8];
+      uint32_t valid = 0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        val[c] = 0.0f;
+        const uint8_t b = S.bits[rc[c]];
+        if (!(b & 8)) continue;
+        int best = -1;
+        float best_delta = 3.402823466e+38f;
+        for (int k = 0; k < kSlots; ++k) {
+          if (!(b & (1u << k))) continue;
+          const float delta = fabsf(planeSD(S, k, rc[c], cc) - centre);
+          if (delta <= a.extract_tol && delta < best_delta) {
+            best_delta = delta;
+            best = k;
+          }
+        }
+        if (best < 0) continue;
+        val[c] = planeSD(S, best, rc[c], pos[c]);
+        valid |= 1u << c;
+      }
+      if (completeCube(val, valid)) {
+        const uint32_t t = c_mc.num_tris[cubeCase(val, a.iso)];
+        if (out != nullptr && t > 0) emitCube(pos, val, a.iso, out + 9u * ntri);
+        ntri += t;
+      }
+    }
+    if (!last) {
+      centre = sk;
+      count = 1;
+    }
+  }
+  return ntri;
+}
+
+// Workgroup-wide exclusive scan of one value per lane (lane order).
+__device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
+  const int tid = threadIdx.x;
+  S.scan[tid] = x;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t y = tid >= off ? S.scan[tid - off] : 0u;
+    __syncthreads();
+    S.scan[tid] += y;
+    __syncthreads();
+  }
+  const uint32_t incl = S.scan[tid];
+  if (tid == 255) S.total = incl;
+  __syncthreads();
+  return incl - x;
+}
+
+__global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, uint32_t num_blocks, MeshArgs a,
+                                                       uint32_t* __restrict__ counts) {
+  __shared__ MeshShared S;
+  const uint32_t b = blockIdx.x, dir = blockIdx.y;
+  const uint64_t key = V.pool_keys[b];
+  stageRegion(S, V, key, b, static_cast<int>(dir));
+  uint32_t n = processCube(S, key, threadIdx.x, a, nullptr) + processCube(S, key, threadIdx.x + 256, a, nullptr);
+  blockExclusiveScan(S, n);
+  if (threadIdx.x == 0) counts[static_cast<size_t>(dir) * num_blocks + b] = S.total;
+}
+
+__global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, uint32_t num_blocks, MeshArgs a,
+                                                      const uint32_t* __restrict__ offsets, float* __restrict__ tris,
+                                                      uint32_t capacity) {
+  __shared__ MeshShared S;
+  const uint32_t b = blockIdx.x, dir = blockIdx.y;
+  const uint64_t key = V.pool_keys[b];
+  uint32_t base = offsets[static_cast<size_t>(dir) * num_blocks + b];
+  stageRegion(S, V, key, b, static_cast<int>(dir));
+  // cubes 0..255 precede 256..511 in the reference's lz, ly, lx loop order
+  for (int half = 0; half < 2; ++half) {
+    const int cube = threadIdx.x + 256 * half;
+    const uint32_t n = processCube(S, key, cube, a, nullptr);
+    const uint32_t start = base + blockExclusiveScan(S, n);
+    if (n > 0 && start + n <= capacity) processCube(S, key, cube, a, tris + 9ull * start);
+    base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
+  }
+}
+
+// -- exclusive scan over u32 (three phases: tiles, partials, add) ----------
+
+constexpr int kScanTile = 1024;
+
+__global__ __launch_bounds__(256) void scanTilesKernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                       uint32_t n, uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s[kScanTile];
+  const uint32_t base = blockIdx.x * kScanTile;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kScanTile; i += 256) s[i] = (base + i < n) ? in[base + i] : 0u;
+  __syncthreads();
+  // each lane scans 4 consecutive elements, then lanes are scanned
+  uint32_t v[4], acc = 0;
+  for (int j = 0; j < 4; ++j) {
+    v[j] = acc;
+    acc += s[tid * 4 + j];
+  }
+  __shared__ uint32_t lanes[256];
+  lanes[tid] = acc;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t y = tid >= off ? lanes[tid - off] : 0u;
+    __syncthreads();
+    lanes[tid] += y;
+    __syncthreads();
+  }
+  const uint32_t lane_excl = lanes[tid] - acc;
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t i = base + tid * 4 + j;
+    if (i < n) out[i] = v[j] + lane_excl;
+  }
+  if (tid == 255) partials[blockIdx.x] = lanes[255];
+}
+
+__global__ __launch_bounds__(256) void scanPartialsKernel(uint32_t* __restrict__ partials, uint32_t m) {
+  // single workgroup: sequential chunks of 256 with a running carry;
+  // partials[m] receives the grand total.
+  __shared__ uint32_t s[256];
+  __shared__ uint32_t carry;
+  const int tid = threadIdx.x;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < m; base += 256) {
+    const uint32_t x = (base + tid < m) ? partials[base + tid] : 0u;
+    s[tid] = x;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const uint32_t y = tid >= off ? s[tid - off] : 0u;
+      __syncthreads();
+      s[tid] += y;
+      __syncthreads();
+    }
+    if (base + tid < m) partials[base + tid] = carry + s[tid] - x;
+    __syncthreads();
+    if (tid == 0) carry += s[255];
+    __syncthreads();
+  }
+  if (tid == 0) partials[m] = carry;
+}
+
+__global__ __launch_bounds__(256) void scanAddKernel(uint32_t* __restrict__ out, uint32_t n,
+                                                     const uint32_t* __restrict__ partials) {
+  const uint32_t add = partials[blockIdx.x];
+  const uint32_t base = blockIdx.x * kScanTile;
+  for (int i = threadIdx.x; i < kScanTile; i += 256)
+    if (base + i < n) out[base + i] += add;
+}
+
+}  // namespace
+
+hipError_t launchMeshCount(const VolumeView& V, uint32_t num_blocks, const MeshArgs& a, uint32_t* counts,
+                           hipStream_t s) {
+  if (num_blocks == 0) return hipSuccess;
+  meshCountKernel<<<dim3(num_blocks, kDirs), 256, 0, s>>>(V, num_blocks, a, counts);
+  return hipGetLastError();
+}
+
+size_t scanPartialsSize(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+// out[i] = sum(in[0..i)); partials[(n + 1023) / 1024] receives the total.
+hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s) {
+  const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles > 0) scanTilesKernel<<<tiles, 256, 0, s>>>(in, out, n, partials);
+  scanPartialsKernel<<<1, 256, 0, s>>>(partials, tiles);
+  if (tiles > 0) scanAddKernel<<<tiles, 256, 0, s>>>(out, n, partials);
+  return hipGetLastError();
+}
+
+hipError_t launchMeshEmit(const VolumeView& V, uint32_t num_blocks, const MeshArgs& a, const uint32_t* offsets,
+                          float* tris, uint32_t capacity, hipStream_t s) {
+  if (num_blocks == 0) return hipSuccess;
+  meshEmitKernel<<<dim3(num_blocks, kDirs), 256, 0, s>>>(V, num_blocks, a, offsets, tris, capacity);
+  return hipGetLastError();
+}
+
+}  // namespace vhm
