@@ -1,0 +1,118 @@
+"""The product's per-voxel integration code (vhm_integrate_core.h), built for
+the host, against the oracle: the deduplicated fast normal path equals the
+reference-order path wherever it applies, and a whole frame integrated with
+the product's per-voxel code equals the oracle bit for bit."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import voxelhashmt_amd as vhm
+from voxelhashmt_amd import synth
+
+LIB = Path(__file__).resolve().parents[1] / "oracle" / "libintegrate_check.so"
+
+
+@pytest.fixture(scope="module")
+def chk():
+    if not LIB.exists():
+        pytest.fail(f"{LIB} missing: run `make -C oracle`")
+    L = C.CDLL(str(LIB))
+    L.vcheck_normals.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                 C.c_int, C.c_void_p]
+    L.vcheck_integrate_blocks.restype = C.c_uint64
+    L.vcheck_integrate_blocks.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_float] * 6
+    return L
+
+
+_LIBM = C.CDLL("libm.so.6")
+_LIBM.expf.restype, _LIBM.expf.argtypes = C.c_float, [C.c_float]
+
+
+def _p7(vp):
+    """Normal-estimation constants as the library derives them: glibc expf of
+    -s / (2 sigma_s^2) (numpy's float32 exp is a different implementation)."""
+    inv = lambda s: np.float32(1.0) / (np.float32(2.0) * np.float32(s) * np.float32(s))
+    inv_s = inv(vp.normal_sigma_spatial)
+    sw = [np.float32(_LIBM.expf(float(np.float32(-s) * inv_s))) for s in (0.0, 1.0, 2.0)]
+    return np.array([vp.disc_base, vp.disc_rel, *sw, inv(vp.normal_sigma_range), inv(vp.normal_sigma_angular)],
+                    np.float32)
+
+
+def _frame(width, height, seed=2):
+    w = synth.tiny(n_frames=2, seed=seed, width=width, height=height)
+    f = synth.render(w.scene, w.K, w.poses, w.noise).astype(np.float32) * np.float32(0.001)
+    vp = vhm.default_params(w.voxel_size, w.truncation)
+    op = O.oracle_params(vp)
+    return w, vp, [O.oracle_filter(op, f[i], w.K) for i in range(2)]
+
+
+def test_fast_normal_path_equals_reference_path(chk):
+    w, vp, fr = _frame(1100, 80)
+    rng = np.random.default_rng(0)
+    n = 60000
+    u = rng.uniform(0, w.K.width - 1, n).astype(np.float32)
+    # crowd the binade edges where (u + 1) - 1 != u can happen, and integer/half points
+    edges = np.array([2, 4, 8, 16, 32, 64, 128, 256, 512, 1024], np.float32)
+    u[: n // 3] = (rng.choice(edges, n // 3) - rng.uniform(0, 2.5, n // 3)).astype(np.float32)
+    u[n // 3: n // 2] = np.round(u[n // 3: n // 2] * 2) / 2
+    v = rng.uniform(0, w.K.height - 1, n).astype(np.float32)
+    uv = np.ascontiguousarray(np.stack([np.clip(u, 0, w.K.width - 1), v], 1))
+    K4, p7 = O.k4(w.K), _p7(vp)
+    ref = np.zeros((n, 6), np.float32)
+    fast = np.zeros((n, 6), np.float32)
+    d = np.ascontiguousarray(fr[0])
+    chk.vcheck_normals(d.ctypes.data, w.K.width, w.K.height, K4.ctypes.data, p7.ctypes.data, uv.ctypes.data, n, 0,
+                       ref.ctypes.data)
+    chk.vcheck_normals(d.ctypes.data, w.K.width, w.K.height, K4.ctypes.data, p7.ctypes.data, uv.ctypes.data, n, 1,
+                       fast.ctypes.data)
+    app = (fast[:, 0] == 1) & (fast[:, 1] == 1)
+    assert app.sum() > n // 2 and (~app).sum() > 100
+    assert np.array_equal(ref[app].view(np.uint32), fast[app].view(np.uint32))
+    # the reference-order path equals the oracle's estimateSurfaceNormalCamera
+    op = O.oracle_params(vp)
+    for i in np.flatnonzero(ref[:, 1] == 1)[:3000]:
+        o = np.array([-1, -2, -3], np.float32)
+        ok = O.oracle().vor_surface_normal(C.byref(op), d.ctypes.data, w.K.width, w.K.height, K4.ctypes.data,
+                                           float(uv[i, 0]), float(uv[i, 1]), o.ctypes.data)
+        assert ok == int(ref[i, 2])
+        if ok:
+            assert np.array_equal(o.view(np.uint32), ref[i, 3:].view(np.uint32))
+
+
+@pytest.mark.parametrize("size", [(96, 72), (300, 200)])
+def test_product_voxel_update_equals_oracle(chk, size):
+    w, vp, fr = _frame(*size, seed=8)
+    op = O.oracle_params(vp)
+    ov = O.OracleVolume(op)
+    blocks, keys = None, None
+    for i in range(2):
+        pose = vhm.pose_array(w.poses[i])
+        fkeys = ov.prepare(fr[i], w.K, pose)
+        all_keys = O.sorted_keys(ov.block_keys())
+        if blocks is None:
+            blocks = np.zeros((len(all_keys), 512, 6, 3), vhm.VOXEL_DTYPE)
+        else:  # carry earlier blocks into the new key order
+            old = {tuple(k): blocks[j] for j, k in enumerate(keys)}
+            blocks = np.stack([old.get(tuple(k), np.zeros((512, 6, 3), vhm.VOXEL_DTYPE)) for k in all_keys])
+        keys = all_keys
+        # integrate this frame's active blocks with the product's code
+        idx = {tuple(k): j for j, k in enumerate(keys)}
+        sel = np.array([idx[tuple(k)] for k in fkeys])
+        sub = np.ascontiguousarray(blocks[sel])
+        fk = np.ascontiguousarray(fkeys, np.int32)
+        d = np.ascontiguousarray(fr[i])
+        n_upd = chk.vcheck_integrate_blocks(sub.ctypes.data, fk.ctypes.data, len(fk), d.ctypes.data, w.K.width,
+                                            w.K.height, O.k4(w.K).ctypes.data, _p7(vp).ctypes.data, pose.ctypes.data,
+                                            vp.voxel_size, vp.truncation_distance,
+                                            max(np.float32(1e-6), np.float32(vp.voxel_size) * np.float32(0.25)),
+                                            vp.layer_normal_cos, vp.max_weight, vp.min_view_angle_weight)
+        blocks[sel] = sub
+        before = ov.updates()
+        O.oracle().vor_integrate_blocks(ov.h, fk.ctypes.data, len(fk), d.ctypes.data, w.K.width, w.K.height,
+                                        O.k4(w.K).ctypes.data, pose.ctypes.data)
+        assert n_upd == ov.updates() - before
+    assert np.array_equal(blocks.view(np.uint32), ov.dump_blocks(keys).view(np.uint32))
