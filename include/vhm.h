@@ -158,6 +158,19 @@ int vhm_download_blocks(vhm_ctx* ctx, const int32_t* keys, size_t n, void* out_h
  * capacity 0 only counts.  Synchronous. */
 int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capacity, size_t* n_out);
 
+/* Incremental extraction over the blocks changed since the previous
+ * incremental call: every block the integrator wrote (its dirty flag) plus
+ * its -x/-y/-z neighbours, whose cubes read it.  Re-meshes that subset only
+ * and clears the dirty flags.  Outputs the re-meshed block keys (pool
+ * order) and their triangles; tri_block_out[i] is the index (into
+ * block_keys_out) of the block owning triangle i.  A caller keeping one mesh
+ * per block replaces exactly those blocks.  No reference counterpart (the
+ * reference always meshes the whole volume); the per-block output equals
+ * vhm_extract_mesh's for those blocks.  Synchronous. */
+int vhm_extract_mesh_incremental(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out, size_t block_capacity,
+                                 size_t* n_blocks_out, float* tris_out, uint32_t* tri_block_out,
+                                 size_t tri_capacity, size_t* n_tris_out);
+
 /* Per-kernel GPU time (ms, HIP events on the context stream) accumulated
  * while timing is enabled: [0] filter, [1] allocate, [2] integrate, [3] mesh;
  * launches[i] counts launches.  Enabling resets the accumulators. */

@@ -198,3 +198,24 @@ def test_determinism(gpu):
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1].view(np.uint8), outs[1][1].view(np.uint8))
     assert np.array_equal(O.canonical_mesh(outs[0][2]), O.canonical_mesh(outs[1][2]))
+
+
+def test_incremental_mesh_equals_full_extraction(gpu):
+    """Keeping one mesh per block and replacing the blocks each incremental
+    call re-meshes gives, after every call, the full extraction's mesh."""
+    w = synth.tiny(n_frames=6, seed=12)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    with vhm.Volume(_params(w)) as vol:
+        per_block = {}
+        for i in range(6):
+            vol.integrate(frames[i], w.K, w.poses[i])
+            if i % 2 == 1:
+                keys, tris, owner = vol.extract_mesh_incremental()
+                print(f"[incremental] frame {i}: {len(keys)} blocks re-meshed, {len(tris)} triangles", flush=True)
+                assert len(keys) > 0
+                for j, k in enumerate(keys):
+                    per_block[tuple(k)] = tris[owner == j]
+                union = np.concatenate([t for t in per_block.values() if len(t)]) if per_block else np.zeros((0, 3, 3))
+                assert np.array_equal(O.canonical_mesh(union), O.canonical_mesh(vol.extract_mesh()))
+        keys, tris, owner = vol.extract_mesh_incremental()
+        assert len(keys) == 0 and len(tris) == 0  # nothing written since the last call

@@ -92,6 +92,9 @@ _SIGS = {
     "vhm_block_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vhm_download_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "vhm_extract_mesh": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vhm_extract_mesh_incremental": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t,
+                                               C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t,
+                                               C.POINTER(C.c_size_t)]),
     "vhm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "vhm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "vhm_synth_render": (C.c_int, [C.POINTER(Scene), C.POINTER(Intrinsics), C.POINTER(Pose), C.c_int, C.c_uint32,
@@ -279,6 +282,28 @@ class Volume:
         if n.value:
             self._check(lib().vhm_extract_mesh(self._h, iso_level, out.ctypes.data, n.value, C.byref(m)))
         return out[: n.value]
+
+    def extract_mesh_incremental(self, iso_level: float = 0.0, fetch: bool = True):
+        """Re-meshes the blocks changed since the last incremental call.
+        Returns (block_keys (m,3), triangles (n,3,3), owner (n,) indices into
+        block_keys); with fetch=False only the counts (m, n)."""
+        nb, nt = C.c_size_t(), C.c_size_t()
+        if not fetch:
+            self._check(lib().vhm_extract_mesh_incremental(self._h, iso_level, None, 0, C.byref(nb), None, None, 0,
+                                                           C.byref(nt)))
+            return nb.value, nt.value
+        cap_b = max(self.block_count(), 1)
+        keys = np.zeros((cap_b, 3), np.int32)
+        # triangle capacity: at most 5 per cube and cluster; grow and retry is not
+        # possible (the call consumes the dirty flags), so size generously
+        cap_t = cap_b * 6 * 512 * 2
+        tris = np.zeros((cap_t, 3, 3), np.float32)
+        owner = np.zeros(cap_t, np.uint32)
+        self._check(lib().vhm_extract_mesh_incremental(self._h, iso_level, keys.ctypes.data, cap_b, C.byref(nb),
+                                                       tris.ctypes.data, owner.ctypes.data, cap_t, C.byref(nt)))
+        if nt.value > cap_t:
+            raise VhmError(6, "incremental mesh larger than the host buffer")
+        return keys[: nb.value].copy(), tris[: nt.value].copy(), owner[: nt.value].copy()
 
     def set_timing(self, enabled: bool):
         self._check(lib().vhm_set_timing(self._h, int(enabled)))

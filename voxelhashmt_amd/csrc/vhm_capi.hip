@@ -58,6 +58,13 @@ struct vhm_ctx {
   size_t key_buf_cap = 0;
   uint32_t* gather_buf = nullptr;
   size_t gather_cap = 0;
+  // incremental meshing
+  uint32_t* remesh_flags = nullptr;  // max_blocks, kept zero between calls
+  uint32_t* remesh_pos = nullptr;
+  uint32_t* remesh_list = nullptr;
+  uint32_t* remesh_partials = nullptr;
+  uint32_t* tri_owner = nullptr;
+  size_t tri_owner_cap = 0;
   // timing
   bool timing = false;
   struct Ev {
@@ -294,6 +301,7 @@ int initVolume(vhm_ctx* c) {
   VHM_TRY(c, hipMemsetAsync(c->pool, 0, size_t(c->P.max_blocks) * kBlockWords * 4u, s), "hipMemset pool");
   VHM_TRY(c, hipMemsetAsync(c->obs, 0, size_t(c->P.max_blocks) * 16u * 4u, s), "hipMemset observed");
   VHM_TRY(c, hipMemsetAsync(c->dirty, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset dirty");
+  VHM_TRY(c, hipMemsetAsync(c->remesh_flags, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset remesh flags");
   VHM_TRY(c, hipMemsetAsync(c->hdr, 0, sizeof(DevHeader), s), "hipMemset header");
   VHM_TRY(c, hipStreamSynchronize(s), "hipStreamSynchronize");
   c->frame = 0;
@@ -396,6 +404,10 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->hdr, sizeof(DevHeader));
   if (e == hipSuccess) e = hipMalloc(&c->depth_in, px * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->filtered, px * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->remesh_flags, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->remesh_pos, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->remesh_list, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->remesh_partials, scanPartialsSize(p->max_blocks) * 4u);
   if (e != hipSuccess) {
     vhm_destroy(c);
     return VHM_ERR_HIP;
@@ -417,7 +429,8 @@ int vhm_destroy(vhm_ctx* c) {
   }
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
                   c->obs,          c->dirty,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
-                  c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf};
+                  c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
+                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -592,7 +605,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
   hipEvent_t ev{};
   beginTimed(c, 3, &ev);
-  VHM_TRY(c, launchMeshCount(view(c), nb, ma, c->mesh_counts, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchMeshCount(view(c), nullptr, nb, ma, c->mesh_counts, c->stream), "mesh count kernel launch");
   VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
           "mesh scan kernel launch");
   uint32_t total = 0;
@@ -603,11 +616,85 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   *n_out = total;
   if (tris_out && capacity > 0 && total > 0) {
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
-    VHM_TRY(c, launchMeshEmit(view(c), nb, ma, c->mesh_offsets, c->tris, total, c->stream),
+    VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_offsets, c->tris, nullptr, total, c->stream),
             "mesh emit kernel launch");
     const size_t m = capacity < total ? capacity : total;
     VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, m * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
             "hipMemcpy D2H triangles");
+  }
+  endTimed(c, ev);
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (c->timing) drainEvents(c);
+  return VHM_OK;
+}
+
+int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out, size_t block_capacity,
+                                 size_t* n_blocks_out, float* tris_out, uint32_t* tri_block_out, size_t tri_capacity,
+                                 size_t* n_tris_out) {
+  if (!c || !n_blocks_out || !n_tris_out) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  int rc = readHeader(c, &h);
+  if (rc) return rc;
+  *n_blocks_out = 0;
+  *n_tris_out = 0;
+  const uint32_t nb = h.num_blocks < c->P.max_blocks ? h.num_blocks : c->P.max_blocks;
+  if (nb == 0) return VHM_OK;
+  hipEvent_t ev{};
+  beginTimed(c, 3, &ev);
+  // dirty blocks and their -x/-y/-z neighbours -> flags -> ordered list
+  VHM_TRY(c, launchMarkRemesh(view(c), nb, c->remesh_flags, c->stream), "remesh mark kernel launch");
+  VHM_TRY(c, launchScan(c->remesh_flags, c->remesh_pos, nb, c->remesh_partials, c->stream), "remesh scan launch");
+  VHM_TRY(c, launchCompactFlags(c->remesh_flags, c->remesh_pos, nb, c->remesh_list, c->remesh_flags, c->stream),
+          "remesh compact launch");
+  uint32_t m = 0;
+  VHM_TRY(c, hipMemcpyAsync(&m, c->remesh_partials + (scanPartialsSize(nb) - 1), sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H remesh count");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *n_blocks_out = m;
+  if (m == 0) {
+    endTimed(c, ev);
+    return VHM_OK;
+  }
+  const size_t entries = size_t(m) * kDirs;
+  if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
+  if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
+  if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
+                   "hipMalloc mesh partials")))
+    return rc;
+  MeshArgs ma{};
+  ma.voxel = c->P.voxel_size;
+  ma.iso = iso;
+  ma.extract_tol = c->P.voxel_size * c->P.layer_extract_scale;
+  ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
+  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
+          "mesh scan kernel launch");
+  uint32_t total = 0;
+  VHM_TRY(c, hipMemcpyAsync(&total, c->mesh_partials + (scanPartialsSize(static_cast<uint32_t>(entries)) - 1),
+                            sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H triangle count");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *n_tris_out = total;
+  if (block_keys_out && block_capacity > 0) {
+    if ((rc = ensure(c, &c->key_buf, &c->key_buf_cap, 3 * size_t(m), "hipMalloc key buffer"))) return rc;
+    VHM_TRY(c, launchListKeys(view(c), c->remesh_list, m, c->key_buf, c->stream), "remesh key launch");
+    const size_t kb = block_capacity < m ? block_capacity : m;
+    VHM_TRY(c, hipMemcpyAsync(block_keys_out, c->key_buf, 3 * kb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H remesh keys");
+  }
+  if (total > 0 && tris_out && tri_capacity > 0) {
+    if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
+    if ((rc = ensure(c, &c->tri_owner, &c->tri_owner_cap, size_t(total), "hipMalloc triangle owners"))) return rc;
+    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_offsets, c->tris, c->tri_owner, total,
+                              c->stream),
+            "mesh emit kernel launch");
+    const size_t t = tri_capacity < total ? tri_capacity : total;
+    VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, t * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H triangles");
+    if (tri_block_out)
+      VHM_TRY(c, hipMemcpyAsync(tri_block_out, c->tri_owner, t * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+              "hipMemcpy D2H triangle owners");
   }
   endTimed(c, ev);
   VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");

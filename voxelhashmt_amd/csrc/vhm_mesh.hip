@@ -275,32 +275,79 @@ __device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
   return incl - x;
 }
 
-__global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, uint32_t num_blocks, MeshArgs a,
-                                                       uint32_t* __restrict__ counts) {
+// Grid (n, 6): workgroup (i, dir) meshes block list[i] (or pool index i
+// when list is null) in direction dir.
+__global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
+                                                       MeshArgs a, uint32_t* __restrict__ counts) {
   __shared__ MeshShared S;
-  const uint32_t b = blockIdx.x, dir = blockIdx.y;
+  const uint32_t i = blockIdx.x, dir = blockIdx.y;
+  const uint32_t b = list ? list[i] : i;
   const uint64_t key = V.pool_keys[b];
   stageRegion(S, V, key, b, static_cast<int>(dir));
-  uint32_t n = processCube(S, key, threadIdx.x, a, nullptr) + processCube(S, key, threadIdx.x + 256, a, nullptr);
-  blockExclusiveScan(S, n);
-  if (threadIdx.x == 0) counts[static_cast<size_t>(dir) * num_blocks + b] = S.total;
+  uint32_t c = processCube(S, key, threadIdx.x, a, nullptr) + processCube(S, key, threadIdx.x + 256, a, nullptr);
+  blockExclusiveScan(S, c);
+  if (threadIdx.x == 0) counts[static_cast<size_t>(dir) * n + i] = S.total;
 }
 
-__global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, uint32_t num_blocks, MeshArgs a,
-                                                      const uint32_t* __restrict__ offsets, float* __restrict__ tris,
+// Writes the triangles of workgroup (i, dir) at offsets[dir * n + i] in cube
+// order; tri_owner (optional) receives i for each triangle.
+__global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
+                                                      MeshArgs a, const uint32_t* __restrict__ offsets,
+                                                      float* __restrict__ tris, uint32_t* __restrict__ tri_owner,
                                                       uint32_t capacity) {
   __shared__ MeshShared S;
-  const uint32_t b = blockIdx.x, dir = blockIdx.y;
+  const uint32_t i = blockIdx.x, dir = blockIdx.y;
+  const uint32_t b = list ? list[i] : i;
   const uint64_t key = V.pool_keys[b];
-  uint32_t base = offsets[static_cast<size_t>(dir) * num_blocks + b];
+  uint32_t base = offsets[static_cast<size_t>(dir) * n + i];
   stageRegion(S, V, key, b, static_cast<int>(dir));
   // cubes 0..255 precede 256..511 in the reference's lz, ly, lx loop order
   for (int half = 0; half < 2; ++half) {
     const int cube = threadIdx.x + 256 * half;
-    const uint32_t n = processCube(S, key, cube, a, nullptr);
-    const uint32_t start = base + blockExclusiveScan(S, n);
-    if (n > 0 && start + n <= capacity) processCube(S, key, cube, a, tris + 9ull * start);
+    const uint32_t c = processCube(S, key, cube, a, nullptr);
+    const uint32_t start = base + blockExclusiveScan(S, c);
+    if (c > 0 && start + c <= capacity) {
+      processCube(S, key, cube, a, tris + 9ull * start);
+      if (tri_owner)
+        for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = i;
+    }
     base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
+  }
+}
+
+// Blocks whose cubes read a dirty block: the dirty block itself and its
+// -x/-y/-z neighbours (a cube at voxel v reads v + {0,1}^3).  Sets flags[q]
+// for each of those present in the table and clears the dirty flag.
+__global__ __launch_bounds__(256) void markRemeshKernel(VolumeView V, uint32_t num_blocks, uint32_t* __restrict__ flags) {
+  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < num_blocks; p += gridDim.x * 256u) {
+    if (V.dirty[p] == 0u) continue;
+    V.dirty[p] = 0u;
+    const uint64_t key = V.pool_keys[p];
+    for (int o = 0; o < 8; ++o) {
+      const uint32_t q = o == 0 ? p : lookupPool(V, keyX(key) - (o & 1), keyY(key) - ((o >> 1) & 1),
+                                                 keyZ(key) - (o >> 2));
+      if (q < V.max_blocks) flags[q] = 1u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void compactFlagsKernel(const uint32_t* __restrict__ flags,
+                                                          const uint32_t* __restrict__ pos, uint32_t n,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ flags_out) {
+  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < n; p += gridDim.x * 256u)
+    if (flags[p]) {
+      list[pos[p]] = p;
+      flags_out[p] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void listKeysKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
+                                                      int32_t* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint64_t key = V.pool_keys[list[i]];
+    out[3 * i] = keyX(key);
+    out[3 * i + 1] = keyY(key);
+    out[3 * i + 2] = keyZ(key);
   }
 }
 
@@ -374,10 +421,10 @@ __global__ __launch_bounds__(256) void scanAddKernel(uint32_t* __restrict__ out,
 
 }  // namespace
 
-hipError_t launchMeshCount(const VolumeView& V, uint32_t num_blocks, const MeshArgs& a, uint32_t* counts,
-                           hipStream_t s) {
-  if (num_blocks == 0) return hipSuccess;
-  meshCountKernel<<<dim3(num_blocks, kDirs), 256, 0, s>>>(V, num_blocks, a, counts);
+hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
+                           uint32_t* counts, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  meshCountKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts);
   return hipGetLastError();
 }
 
@@ -392,10 +439,35 @@ hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* p
   return hipGetLastError();
 }
 
-hipError_t launchMeshEmit(const VolumeView& V, uint32_t num_blocks, const MeshArgs& a, const uint32_t* offsets,
-                          float* tris, uint32_t capacity, hipStream_t s) {
+hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
+                          const uint32_t* offsets, float* tris, uint32_t* tri_owner, uint32_t capacity,
+                          hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  meshEmitKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, offsets, tris, tri_owner, capacity);
+  return hipGetLastError();
+}
+
+static uint32_t gridFor(uint32_t n) {
+  const uint32_t g = (n + 255u) / 256u;
+  return g == 0 ? 1u : (g < 4096u ? g : 4096u);
+}
+
+hipError_t launchMarkRemesh(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s) {
   if (num_blocks == 0) return hipSuccess;
-  meshEmitKernel<<<dim3(num_blocks, kDirs), 256, 0, s>>>(V, num_blocks, a, offsets, tris, capacity);
+  markRemeshKernel<<<gridFor(num_blocks), 256, 0, s>>>(V, num_blocks, flags);
+  return hipGetLastError();
+}
+
+hipError_t launchCompactFlags(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* list,
+                              uint32_t* flags_out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  compactFlagsKernel<<<gridFor(n), 256, 0, s>>>(flags, pos, n, list, flags_out);
+  return hipGetLastError();
+}
+
+hipError_t launchListKeys(const VolumeView& V, const uint32_t* list, uint32_t n, int32_t* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  listKeysKernel<<<gridFor(n), 256, 0, s>>>(V, list, n, out);
   return hipGetLastError();
 }
 

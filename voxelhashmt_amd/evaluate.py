@@ -1,0 +1,95 @@
+"""Reconstruction quality against the analytic scene.
+
+* Chamfer distance (mm) = (accuracy + completeness) / 2.
+  * accuracy: mean over mesh vertices of |SDF(v)|. SDF is the scene's exact
+    signed distance, whose magnitude is the Euclidean distance to the
+    surface outside the primitives.
+  * completeness: mean over ground-truth surface samples of the distance to
+    the nearest mesh vertex. The samples are noise-free depth pixels
+    back-projected from the trajectory's frames, so only surfaces the camera
+    saw are counted.
+* TSDF L1 (mm). For every observed voxel, the plane signed distance of its
+  heaviest slot is evaluated at the voxel centre (planeSignedDistance,
+  voxel_hash_tsdf.h:336-339). It is compared with the scene SDF at the same
+  centre, both clamped to +-truncation.
+
+This is analysis code, not a hot path. It runs on the host with numpy and
+scipy's k-d tree.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import VOXEL_DTYPE
+
+
+def unpack_octahedral(packed: np.ndarray) -> np.ndarray:
+    """unpackNormalOctahedral (voxel_hash_tsdf.h:291-309), vectorised in float32."""
+    packed = np.asarray(packed, np.uint32)
+    f = np.float32
+    x = (((packed & 0xFFFF).astype(f) / f(65535.0)) * f(2.0) - f(1.0)).astype(f)
+    y = ((((packed >> 16) & 0xFFFF).astype(f) / f(65535.0)) * f(2.0) - f(1.0)).astype(f)
+    z = (f(1.0) - np.abs(x) - np.abs(y)).astype(f)
+    neg = z < 0
+    sx = np.where(x >= 0, f(1), f(-1))
+    sy = np.where(y >= 0, f(1), f(-1))
+    px = np.where(neg, (f(1) - np.abs(y)) * sx, x)
+    py = np.where(neg, (f(1) - np.abs(x)) * sy, y)
+    n = np.stack([px, py, z], -1).astype(f)
+    ln = np.sqrt((n * n).sum(-1, keepdims=True)).astype(f)
+    return np.where(ln > 1e-8, n / np.maximum(ln, f(1e-30)), f(0)).astype(f)
+
+
+def gt_surface_points(scene, K, poses, stride: int = 8, max_points: int = 400_000, seed: int = 0) -> np.ndarray:
+    """Noise-free depth of every `stride`-th pixel of the given frames, back-projected to the world."""
+    from . import synth
+    depth = synth.render(scene, K, poses, None).astype(np.float64) * 1e-3
+    pts = []
+    vv, uu = np.mgrid[0:K.height:stride, 0:K.width:stride]
+    for i, p in enumerate(poses):
+        d = depth[i][vv, uu]
+        ok = d > 0
+        x = (uu[ok] - K.cx) / K.fx * d[ok]
+        y = (vv[ok] - K.cy) / K.fy * d[ok]
+        pc = np.stack([x, y, d[ok]], -1)
+        R = np.array(p.R, np.float64).reshape(3, 3)
+        pts.append(pc @ R.T + np.array(p.t, np.float64))
+    pts = np.concatenate(pts) if pts else np.zeros((0, 3))
+    if len(pts) > max_points:
+        pts = pts[np.random.default_rng(seed).choice(len(pts), max_points, replace=False)]
+    return pts
+
+
+def chamfer_mm(triangles: np.ndarray, scene, gt_points: np.ndarray) -> dict:
+    from scipy.spatial import cKDTree
+    verts = np.asarray(triangles, np.float64).reshape(-1, 3)
+    if len(verts) == 0 or len(gt_points) == 0:
+        return {"chamfer_mm": float("inf"), "accuracy_mm": float("inf"), "completeness_mm": float("inf")}
+    verts = np.unique(verts, axis=0)
+    acc = np.abs(scene.sdf(verts)).mean() * 1e3
+    comp_d, _ = cKDTree(verts).query(gt_points, k=1)
+    comp = comp_d.mean() * 1e3
+    return {"chamfer_mm": float((acc + comp) / 2), "accuracy_mm": float(acc), "completeness_mm": float(comp),
+            "vertices": int(len(verts))}
+
+
+def tsdf_l1_mm(keys: np.ndarray, blocks: np.ndarray, scene, voxel_size: float, truncation: float) -> dict:
+    """blocks: (n, 512, 6, 3) VOXEL_DTYPE as Volume.download_blocks returns."""
+    assert blocks.dtype == VOXEL_DTYPE
+    w = blocks["weight"].reshape(len(blocks), 512, 18)
+    best = w.argmax(-1)
+    obs = np.take_along_axis(w, best[..., None], -1)[..., 0] > 0
+    if not obs.any():
+        return {"tsdf_l1_mm": float("inf"), "observed": 0}
+    pn = np.take_along_axis(blocks["packed_normal"].reshape(len(blocks), 512, 18), best[..., None], -1)[..., 0]
+    po = np.take_along_axis(blocks["plane_offset"].reshape(len(blocks), 512, 18), best[..., None], -1)[..., 0]
+    li = np.arange(512)
+    local = np.stack([li & 7, (li >> 3) & 7, li >> 6], -1)
+    vox = keys[:, None, :].astype(np.int64) * 8 + local[None]
+    centre = ((vox.astype(np.float32) + np.float32(0.5)) * np.float32(voxel_size)).astype(np.float32)
+    n = unpack_octahedral(pn)
+    sd = (n * centre).sum(-1) - po
+    gt = scene.sdf(centre[obs].astype(np.float64))
+    t = truncation
+    err = np.abs(np.clip(sd[obs], -t, t) - np.clip(gt, -t, t))
+    return {"tsdf_l1_mm": float(err.mean() * 1e3), "observed": int(obs.sum())}
