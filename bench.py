@@ -7,10 +7,12 @@ block allocation -> directional TSDF integration) of the 1280x720 plain-
 at-scale workload: 8x8x3 m room with 150 seeded primitives, 500 frames on a
 random walk (0.02 m / +-1 deg per frame), voxel 5 mm, truncation 2 cm, six
 directional bins x 3 fp32 slots per voxel, 5x5 bilateral (sigma_s 4.5 px,
-sigma_r 30 mm).  Frames are rendered on the GPU and resident in HBM before
-the timed region; W warm-up frames are integrated untimed, then K frames are
-timed (W + K <= 500, in sequence).  After the timed region one full
-marching-cubes extraction is timed separately.
+sigma_r 30 mm), incremental marching cubes over the dirty blocks every 10
+frames (triangles written into a device buffer).  Frames are rendered on the
+GPU and resident in HBM before the timed region; W warm-up frames are
+integrated untimed, then K frames are timed (W + K <= 500, in sequence).
+After the timed region one full marching-cubes extraction is timed
+separately.
 
 value = voxel updates / s over the timed frames (an update is a voxel that
 reaches the layer update with weight > 0 -- voxel_hash_tsdf.h:1031-1055).
@@ -83,6 +85,8 @@ def main():
     ap.add_argument("--max-blocks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--mesh-every", type=int, default=10, help="incremental MC period in frames (0 = off)")
+    ap.add_argument("--mesh-tri-capacity", type=int, default=8_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the sharded design is described in "
@@ -123,11 +127,21 @@ def main():
     s0 = vol.stats()
     log(f"[bench] warmup done: blocks={vol.block_count()}")
 
+    mesh_every = args.mesh_every
+    tri_buf = torch.empty(args.mesh_tri_capacity * 9, dtype=torch.float32, device=dev) if mesh_every else None
+    if mesh_every:
+        vol.extract_mesh_incremental(device_tris=tri_buf)  # consume the warm-up's dirty blocks
+    inc_blocks = inc_tris = inc_calls = 0
     vol.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(W, W + K):
+    for j, i in enumerate(range(W, W + K)):
         vol.integrate(frames[i], w.K, w.poses[i])
+        if mesh_every and (j + 1) % mesh_every == 0:
+            nb, nt = vol.extract_mesh_incremental(device_tris=tri_buf)
+            inc_blocks += nb
+            inc_tris += nt
+            inc_calls += 1
     vol.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -195,6 +209,8 @@ def main():
                    "blocks_allocated": blocks, "voxels_visited": visited},
         "frames_per_s": K / elapsed,
         "kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items()},
+        "incremental_mesh": {"every_frames": args.mesh_every, "calls": inc_calls, "blocks_remeshed": inc_blocks,
+                             "triangles": inc_tris, "ms": round(kt["mesh"]["ms"], 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **mesh,

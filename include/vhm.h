@@ -166,16 +166,58 @@ int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capa
  * block_keys_out) of the block owning triangle i.  A caller keeping one mesh
  * per block replaces exactly those blocks.  No reference counterpart (the
  * reference always meshes the whole volume); the per-block output equals
- * vhm_extract_mesh's for those blocks.  Synchronous. */
+ * vhm_extract_mesh's for those blocks.  block_keys_out is host memory;
+ * tris_out / tri_block_out are in out_memkind memory (VHM_DEVICE: written in
+ * place on the GPU, nothing copied to the host).  Synchronous. */
 int vhm_extract_mesh_incremental(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out, size_t block_capacity,
                                  size_t* n_blocks_out, float* tris_out, uint32_t* tri_block_out,
-                                 size_t tri_capacity, size_t* n_tris_out);
+                                 size_t tri_capacity, int out_memkind, size_t* n_tris_out);
 
 /* Per-kernel GPU time (ms, HIP events on the context stream) accumulated
  * while timing is enabled: [0] filter, [1] allocate, [2] integrate, [3] mesh;
  * launches[i] counts launches.  Enabling resets the accumulators. */
 int vhm_set_timing(vhm_ctx* ctx, int enabled);
 int vhm_kernel_times(vhm_ctx* ctx, double ms_out[4], uint64_t launches_out[4]);
+
+/* --- learned block denoiser (no reference counterpart) ------------------ */
+
+/* Per 8^3 block with a 1-voxel halo: 12 input channels (per direction d the
+ * heaviest slot's plane signed distance at the voxel centre / truncation,
+ * clamped to [-1, 1], and its weight / max_weight), three 3x3x3
+ * convolutions 12->32->32->6 with ReLU, predicting the 6 tsdf channels of
+ * the 8^3 interior as input + residual.  BF16 MFMA, FP32 accumulation.
+ * 43,270 FP32 master parameters in the order W1 (3,3,3,12,32), b1 (32),
+ * W2 (3,3,3,32,32), b2 (32), W3 (3,3,3,32,6), b3 (6). */
+typedef struct vhm_denoiser vhm_denoiser;
+int vhm_denoiser_num_params(void);
+/* capacity: samples held on the device; batch_capacity: blocks per training step. */
+int vhm_denoiser_create(int capacity, int batch_capacity, vhm_denoiser** out);
+int vhm_denoiser_destroy(vhm_denoiser* d);
+const char* vhm_denoiser_last_error(const vhm_denoiser* d);
+int vhm_denoiser_set_params(vhm_denoiser* d, const float* params); /* also resets Adam */
+int vhm_denoiser_get_params(vhm_denoiser* d, float* params);
+int vhm_denoiser_clear(vhm_denoiser* d);
+int vhm_denoiser_count(const vhm_denoiser* d);
+/* Appends one sample per key: inputs gathered from input_vol, targets (tsdf
+ * and observed mask of the same features) from target_vol when not null. */
+int vhm_denoiser_append(vhm_denoiser* d, vhm_ctx* input_vol, vhm_ctx* target_vol, const int32_t* keys, size_t n);
+/* Forward over samples [offset, offset+n); y_out (host, n x 512 x 6) optional. */
+int vhm_denoiser_forward(vhm_denoiser* d, size_t offset, size_t n, float* y_out);
+/* Masked-MSE training step (forward, backward, deterministic gradient
+ * reduction, Adam with lr; lr <= 0 leaves the parameters unchanged).
+ * loss_out and grad_out (43,270 floats, host) optional. */
+int vhm_denoiser_train_step(vhm_denoiser* d, size_t offset, size_t n, float lr, float* loss_out, float* grad_out);
+/* Writes the last predictions of samples [offset, offset+n) into vol.  A cell
+ * (voxel, direction) is replaced only if its input weight >= min_weight and
+ * |prediction - input| <= max_delta (tsdf units); a block whose mean
+ * |prediction - input| exceeds block_max_delta keeps plain fusion. */
+int vhm_denoiser_apply(vhm_denoiser* d, vhm_ctx* vol, size_t offset, size_t n, float min_weight, float max_delta,
+                       float block_max_delta, uint64_t* cells_replaced, uint64_t* blocks_kept_plain);
+/* Sample inputs (bf16 bits, n x 1000 x 16), targets and masks (n x 512 x 6). */
+int vhm_denoiser_samples(vhm_denoiser* d, size_t offset, size_t n, uint16_t* x_out, float* t_out, float* m_out);
+/* Device check of the v_mfma_f32_16x16x16_bf16 operand layout the kernels
+ * assume: out[4 * lane + r] = D element held by (lane, r). */
+int vhm_mfma_layout_probe(float* out256);
 
 /* --- synthetic depth (no reference counterpart; benchmark/test input) ---- */
 

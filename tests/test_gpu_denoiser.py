@@ -1,0 +1,147 @@
+"""The BF16 MFMA denoiser kernels against the FP64 oracle
+(oracle/denoiser_ref.py): operand layout, feature gather, forward,
+gradients, training progress and the write-back fallback."""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import voxelhashmt_amd as vhm
+from voxelhashmt_amd import synth
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import denoiser_ref as D  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_operand_layout(gpu):
+    out = vhm.mfma_layout_probe()
+    for lane in range(64):
+        for r in range(4):
+            i, n = 4 * (lane // 16) + r, lane % 16
+            assert out[lane, r] == i + n / 16.0, (lane, r, out[lane, r])
+
+
+def _scene_volume(seed=5, frames=3):
+    w = synth.tiny(n_frames=frames, seed=seed)
+    f = synth.render(w.scene, w.K, w.poses, w.noise)
+    vp = vhm.default_params(w.voxel_size, w.truncation, hash_log2_buckets=16, max_blocks=16384,
+                            max_width=w.K.width, max_height=w.K.height)
+    vol = vhm.Volume(vp)
+    for i in range(frames):
+        vol.integrate(f[i], w.K, w.poses[i])
+    vol.synchronize()
+    return w, vol
+
+
+def _features_x(x_bits):
+    return (x_bits.astype(np.uint32) << 16).view(np.float32)
+
+
+def test_gather_equals_numpy_features(gpu):
+    w, vol = _scene_volume()
+    keys = O.sorted_keys(vol.block_keys())
+    blocks = vol.download_blocks(keys)
+    lut = {tuple(k): blocks[i] for i, k in enumerate(keys)}
+    # blocks with observations
+    obs = (blocks["weight"] > 0).reshape(len(keys), -1).sum(1)
+    pick = keys[np.argsort(-obs)[:6]]
+    with vhm.Denoiser(64, 16) as dn:
+        dn.append(vol, pick, vol)
+        x, t, m = dn.samples(0, len(pick))
+    for i, k in enumerate(pick):
+        ref = D.bf16_round(D.block_features(lut, tuple(k), w.voxel_size, w.truncation))
+        assert np.array_equal(_features_x(x[i]).view(np.uint32), ref.view(np.uint32))
+        # targets: the unrounded features of the same volume at the interior
+        full = D.block_features(lut, tuple(k), w.voxel_size, w.truncation)[1:9, 1:9, 1:9]
+        assert np.array_equal(t[i].reshape(8, 8, 8, 6), full[..., :6])
+        assert np.array_equal(m[i].reshape(8, 8, 8, 6) > 0, full[..., 6:12] > 0)
+    vol.close()
+
+
+def _random_batch(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-1, 1, size=(n, 10, 10, 10, 12))
+    X[..., 6:] = np.abs(X[..., 6:])
+    return X
+
+
+def _load_synthetic_samples(dn, vol, keys):
+    dn.append(vol, keys, vol)
+
+
+def test_forward_and_gradients_match_fp64_oracle(gpu):
+    w, vol = _scene_volume(seed=9, frames=4)
+    keys = O.sorted_keys(vol.block_keys())
+    obs = (vol.download_blocks(keys)["weight"] > 0).reshape(len(keys), -1).sum(1)
+    pick = keys[np.argsort(-obs)[:8]]
+    p = D.init_params(7)
+    p["W3"] *= 10.0
+    rng = np.random.default_rng(2)
+    for k in ("b1", "b2", "b3"):
+        p[k] = rng.normal(scale=0.05, size=p[k].shape)
+    flat = D.params_to_flat(p).astype(np.float32)
+    with vhm.Denoiser(64, 16) as dn:
+        dn.set_params(flat)
+        dn.append(vol, pick, vol)
+        x, t, m = dn.samples(0, len(pick))
+        # the targets here are the inputs' own features; perturb them so the loss is not ~0
+        y = dn.forward(0, len(pick))
+        X = _features_x(x)[..., :12].astype(np.float64)
+        pr = D.flat_to_params(flat.astype(np.float64))
+        yr, _ = D.forward(pr, X)
+        err = np.sqrt(((y.reshape(yr.shape) - yr) ** 2).mean()) / np.sqrt((yr ** 2).mean())
+        print(f"[denoiser] forward rms rel err vs fp64 = {err:.2e}", flush=True)
+        assert err < 1e-2
+        loss, g = dn.train_step(0, len(pick), 0.0, want_grad=True)
+        T = t.reshape(yr.shape).astype(np.float64)
+        M = m.reshape(yr.shape).astype(np.float64)
+        Lr, gr = D.loss_and_grads(pr, X, T, M)
+        print(f"[denoiser] loss gpu={loss:.6e} fp64={Lr:.6e}", flush=True)
+        assert abs(loss - Lr) <= 2e-2 * abs(Lr) + 1e-9
+        gflat = D.params_to_flat(gr)
+        o = 0
+        for name in D.ORDER:
+            nparam = int(np.prod(D.SHAPES[name]))
+            a, b = g[o:o + nparam].astype(np.float64), gflat[o:o + nparam]
+            o += nparam
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+            print(f"[denoiser] grad {name}: rel L2 err {rel:.2e}, cosine {cos:.6f}", flush=True)
+            assert rel < 5e-2 and cos > 0.998, name
+    vol.close()
+
+
+def test_training_reduces_loss_and_apply_fallback(gpu):
+    """Inputs after 2 noisy frames, targets after 8 frames of the same scene:
+    a few Adam steps reduce the masked loss; apply() with a block threshold
+    below zero keeps every block plain and leaves the volume unchanged."""
+    w = synth.tiny(n_frames=8, seed=4)
+    f = synth.render(w.scene, w.K, w.poses, w.noise)
+    vp = vhm.default_params(w.voxel_size, w.truncation, hash_log2_buckets=16, max_blocks=16384,
+                            max_width=w.K.width, max_height=w.K.height)
+    few, many = vhm.Volume(vp), vhm.Volume(vp)
+    for i in range(8):
+        if i in (0, 4):
+            few.integrate(f[i], w.K, w.poses[i])
+        many.integrate(f[i], w.K, w.poses[i])
+    kf = {tuple(k) for k in few.block_keys()}
+    keys = np.array([k for k in O.sorted_keys(many.block_keys()) if tuple(k) in kf], np.int32)[:512]
+    with vhm.Denoiser(1024, 512) as dn:
+        dn.set_params(D.params_to_flat(D.init_params(1)).astype(np.float32))
+        dn.append(few, keys, many)
+        losses = [dn.train_step(0, len(keys), 1e-3) for _ in range(30)]
+        print(f"[denoiser] loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+        assert losses[-1] < 0.8 * losses[0]
+        dn.forward(0, len(keys), fetch=False)
+        before = few.download_blocks(keys)
+        r = dn.apply(few, block_max_delta=-1.0)
+        assert r["cells_replaced"] == 0 and r["blocks_kept_plain"] == len(keys)
+        assert np.array_equal(before.view(np.uint8), few.download_blocks(keys).view(np.uint8))
+        r = dn.apply(few, max_delta=10.0, block_max_delta=10.0)
+        assert r["cells_replaced"] > 0
+    few.close()
+    many.close()

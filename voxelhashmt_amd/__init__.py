@@ -93,13 +93,29 @@ _SIGS = {
     "vhm_download_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "vhm_extract_mesh": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vhm_extract_mesh_incremental": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t,
-                                               C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t,
+                                               C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
                                                C.POINTER(C.c_size_t)]),
     "vhm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "vhm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "vhm_synth_render": (C.c_int, [C.POINTER(Scene), C.POINTER(Intrinsics), C.POINTER(Pose), C.c_int, C.c_uint32,
                                    C.POINTER(Noise), C.c_void_p, C.c_int]),
     "vhm_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vhm_denoiser_num_params": (C.c_int, []),
+    "vhm_denoiser_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "vhm_denoiser_destroy": (C.c_int, [C.c_void_p]),
+    "vhm_denoiser_last_error": (C.c_char_p, [C.c_void_p]),
+    "vhm_denoiser_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vhm_denoiser_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vhm_denoiser_clear": (C.c_int, [C.c_void_p]),
+    "vhm_denoiser_count": (C.c_int, [C.c_void_p]),
+    "vhm_denoiser_append": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vhm_denoiser_forward": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p]),
+    "vhm_denoiser_train_step": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_float, C.POINTER(C.c_float),
+                                          C.c_void_p]),
+    "vhm_denoiser_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_float, C.c_float, C.c_float,
+                                     C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "vhm_denoiser_samples": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vhm_mfma_layout_probe": (C.c_int, [C.c_void_p]),
 }
 
 
@@ -283,14 +299,18 @@ class Volume:
             self._check(lib().vhm_extract_mesh(self._h, iso_level, out.ctypes.data, n.value, C.byref(m)))
         return out[: n.value]
 
-    def extract_mesh_incremental(self, iso_level: float = 0.0, fetch: bool = True):
+    def extract_mesh_incremental(self, iso_level: float = 0.0, device_tris=None, device_owner=None):
         """Re-meshes the blocks changed since the last incremental call.
         Returns (block_keys (m,3), triangles (n,3,3), owner (n,) indices into
-        block_keys); with fetch=False only the counts (m, n)."""
+        block_keys).  With device_tris (a torch CUDA float tensor of n_max*9
+        elements) and device_owner (int32, n_max) the triangles stay on the GPU
+        and (m, n) is returned."""
         nb, nt = C.c_size_t(), C.c_size_t()
-        if not fetch:
-            self._check(lib().vhm_extract_mesh_incremental(self._h, iso_level, None, 0, C.byref(nb), None, None, 0,
-                                                           C.byref(nt)))
+        if device_tris is not None:
+            cap = device_tris.numel() // 9
+            self._check(lib().vhm_extract_mesh_incremental(
+                self._h, iso_level, None, 0, C.byref(nb), device_tris.data_ptr(),
+                None if device_owner is None else device_owner.data_ptr(), cap, VHM_DEVICE, C.byref(nt)))
             return nb.value, nt.value
         cap_b = max(self.block_count(), 1)
         keys = np.zeros((cap_b, 3), np.int32)
@@ -300,7 +320,8 @@ class Volume:
         tris = np.zeros((cap_t, 3, 3), np.float32)
         owner = np.zeros(cap_t, np.uint32)
         self._check(lib().vhm_extract_mesh_incremental(self._h, iso_level, keys.ctypes.data, cap_b, C.byref(nb),
-                                                       tris.ctypes.data, owner.ctypes.data, cap_t, C.byref(nt)))
+                                                       tris.ctypes.data, owner.ctypes.data, cap_t, VHM_HOST,
+                                                       C.byref(nt)))
         if nt.value > cap_t:
             raise VhmError(6, "incremental mesh larger than the host buffer")
         return keys[: nb.value].copy(), tris[: nt.value].copy(), owner[: nt.value].copy()
@@ -314,3 +335,96 @@ class Volume:
         self._check(lib().vhm_kernel_times(self._h, ms, cnt))
         names = ["filter", "allocate", "integrate", "mesh"]
         return {n: {"ms": ms[i], "launches": cnt[i]} for i, n in enumerate(names)}
+
+
+class Denoiser:
+    """Learned per-block denoiser on the GPU (vhm_denoiser_*): a device sample
+    set, BF16 MFMA forward/backward, Adam, write-back with fallback."""
+
+    NUM_PARAMS = 43270
+
+    def __init__(self, capacity: int, batch_capacity: int = 4096):
+        self._h = C.c_void_p()
+        rc = lib().vhm_denoiser_create(int(capacity), int(batch_capacity), C.byref(self._h))
+        if rc != 0:
+            raise VhmError(rc, "vhm_denoiser_create failed")
+        self.capacity, self.batch_capacity = capacity, batch_capacity
+
+    def close(self):
+        if self._h:
+            lib().vhm_denoiser_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VhmError(rc, lib().vhm_denoiser_last_error(self._h).decode())
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        assert p.size == self.NUM_PARAMS
+        self._check(lib().vhm_denoiser_set_params(self._h, p.ctypes.data))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.NUM_PARAMS, np.float32)
+        self._check(lib().vhm_denoiser_get_params(self._h, p.ctypes.data))
+        return p
+
+    def clear(self):
+        self._check(lib().vhm_denoiser_clear(self._h))
+
+    @property
+    def count(self) -> int:
+        return lib().vhm_denoiser_count(self._h)
+
+    def append(self, input_vol: Volume, keys: np.ndarray, target_vol: Volume | None = None):
+        k = np.ascontiguousarray(keys, np.int32).reshape(-1, 3)
+        self._check(lib().vhm_denoiser_append(self._h, input_vol._h, target_vol._h if target_vol else None,
+                                              k.ctypes.data, len(k)))
+
+    def forward(self, offset: int = 0, n: int | None = None, fetch: bool = True):
+        n = self.count - offset if n is None else n
+        y = np.zeros((n, 512, 6), np.float32) if fetch else None
+        self._check(lib().vhm_denoiser_forward(self._h, offset, n, y.ctypes.data if fetch else None))
+        return y
+
+    def train_step(self, offset: int, n: int, lr: float, want_grad: bool = False):
+        loss = C.c_float()
+        g = np.zeros(self.NUM_PARAMS, np.float32) if want_grad else None
+        self._check(lib().vhm_denoiser_train_step(self._h, offset, n, lr, C.byref(loss),
+                                                  g.ctypes.data if want_grad else None))
+        return (loss.value, g) if want_grad else loss.value
+
+    def apply(self, vol: Volume, offset: int = 0, n: int | None = None, min_weight: float = 0.0,
+              max_delta: float = 0.5, block_max_delta: float = 0.25) -> dict:
+        n = self.count - offset if n is None else n
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(lib().vhm_denoiser_apply(self._h, vol._h, offset, n, min_weight, max_delta, block_max_delta,
+                                             C.byref(a), C.byref(b)))
+        return {"cells_replaced": a.value, "blocks_kept_plain": b.value}
+
+    def samples(self, offset: int, n: int):
+        x = np.zeros((n, 10, 10, 10, 16), np.uint16)
+        t = np.zeros((n, 512, 6), np.float32)
+        m = np.zeros((n, 512, 6), np.float32)
+        self._check(lib().vhm_denoiser_samples(self._h, offset, n, x.ctypes.data, t.ctypes.data, m.ctypes.data))
+        return x, t, m
+
+
+def mfma_layout_probe() -> np.ndarray:
+    out = np.zeros(256, np.float32)
+    rc = lib().vhm_mfma_layout_probe(out.ctypes.data)
+    if rc != 0:
+        raise VhmError(rc, "vhm_mfma_layout_probe failed")
+    return out.reshape(64, 4)

@@ -630,7 +630,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
 
 int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out, size_t block_capacity,
                                  size_t* n_blocks_out, float* tris_out, uint32_t* tri_block_out, size_t tri_capacity,
-                                 size_t* n_tris_out) {
+                                 int out_memkind, size_t* n_tris_out) {
   if (!c || !n_blocks_out || !n_tris_out) return VHM_ERR_INVALID_ARGUMENT;
   DevHeader h{};
   int rc = readHeader(c, &h);
@@ -683,7 +683,13 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
     VHM_TRY(c, hipMemcpyAsync(block_keys_out, c->key_buf, 3 * kb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
             "hipMemcpy D2H remesh keys");
   }
-  if (total > 0 && tris_out && tri_capacity > 0) {
+  if (total > 0 && tris_out && tri_capacity > 0 && out_memkind == VHM_DEVICE) {
+    // straight into the caller's device buffers; triangles past the capacity are dropped
+    const uint32_t cap = static_cast<uint32_t>(tri_capacity < total ? tri_capacity : total);
+    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_offsets, tris_out, tri_block_out, cap,
+                              c->stream),
+            "mesh emit kernel launch");
+  } else if (total > 0 && tris_out && tri_capacity > 0) {
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
     if ((rc = ensure(c, &c->tri_owner, &c->tri_owner_cap, size_t(total), "hipMalloc triangle owners"))) return rc;
     VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_offsets, c->tris, c->tri_owner, total,
@@ -782,3 +788,8 @@ void vhm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 }  // extern "C"
+
+// Context internals for the other host translation units (vhm_internal.h).
+vhm::VolumeView vhmCtxView(vhm_ctx* c) { return view(c); }
+hipStream_t vhmCtxStream(vhm_ctx* c) { return c->stream; }
+const vhm_params* vhmCtxParams(vhm_ctx* c) { return &c->P; }

@@ -1,0 +1,760 @@
+// Learned per-block denoiser: BF16 MFMA 3D convolutions with FP32
+// accumulation, forward and backward, Adam, and the write-back into the
+// volume with a confidence fallback to plain fusion.  No reference
+// counterpart (the reference has no learned component); the FP64 statement
+// of the network is oracle/denoiser_ref.py.
+//
+// Network per 8^3 block (channels-last, bf16 activations):
+//   X  [10][10][10][16]  tsdf_d (6), w_d (6), 4 zero channels; 1-voxel halo
+//   h1 = relu(conv3x3x3(X) valid,  12->32)     [8][8][8][32]
+//   h2 = relu(conv3x3x3(h1) same,  32->32)     [8][8][8][32]
+//   y  = X.tsdf[centre] + conv3x3x3(h2) same, 32->6
+// A convolution is an implicit GEMM: M = 512 output voxels, N = output
+// channels, K = 27 taps x input channels, on v_mfma_f32_16x16x16_bf16
+// (16x16 output tile per wavefront, K = 16 per instruction: 16 input
+// channels of one tap).  Operand layout of the instruction, per lane l:
+//   A[i][k]: i = l % 16, k = 4 (l / 16) + j   (4 bf16 per lane, j = 0..3)
+//   B[k][n]: n = l % 16, k = 4 (l / 16) + j
+//   D[i][n]: n = l % 16, i = 4 (l / 16) + r   (4 fp32 per lane, r = 0..3)
+// (vhm_mfma_layout_probe checks this on the device).
+//
+// Forward: one workgroup (4 wavefronts) per block.  LDS holds the padded
+// input grid and the padded h1 grid (ping-pong, 128 KiB); weights stream
+// from L2 ([tap][n][c] bf16, 8 bytes per lane per instruction).
+// Backward (training): per-layer kernels over blocks with the bf16
+// activations in HBM; weight gradients accumulate in each wavefront's
+// registers across the blocks it visits and are reduced in a fixed order
+// (deterministic); the data gradient is the same convolution with the
+// flipped, transposed weights.
+#include <cstring>
+
+#include "vhm_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace vhm {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kDnIn = 16;      // input channels (12 used)
+constexpr int kDnHid = 32;
+constexpr int kDnOut = 6;
+constexpr int kDnOutPad = 16;
+constexpr int kGrid = 10;      // padded grid edge
+constexpr int kGridVox = 1000;
+// master parameter offsets (oracle/denoiser_ref.py ORDER: W1 b1 W2 b2 W3 b3)
+constexpr int kOffW1 = 0, kOffB1 = 27 * 12 * 32, kOffW2 = kOffB1 + 32, kOffB2 = kOffW2 + 27 * 32 * 32,
+              kOffW3 = kOffB2 + 32, kOffB3 = kOffW3 + 27 * 32 * 6, kNumParams = kOffB3 + 6;  // 43270
+static_assert(kNumParams == 43270, "parameter count");
+
+VHM_HD uint16_t f2bf(float f) {  // round to nearest even (finite inputs)
+  const uint32_t u = __builtin_bit_cast(uint32_t, f);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+VHM_HD float bf2f(uint16_t h) { return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16); }
+
+
+
+namespace {
+
+__device__ __forceinline__ int padIndex(int m, int t) {
+  // output voxel m of the 8^3 grid, tap t = kx + 3 (ky + 3 kz): index in the
+  // 10^3 padded grid of the input it reads
+  const int x = m & 7, y = (m >> 3) & 7, z = m >> 6;
+  const int kx = t % 3, ky = (t / 3) % 3, kz = t / 9;
+  return ((z + kz) * kGrid + (y + ky)) * kGrid + (x + kx);
+}
+
+// acc[i] (i < MT) += conv over the padded LDS grid `in` ([1000][CIN] bf16)
+// for this wavefront's tiles: n-tile `nt` (16 output channels, weights
+// W [27][NTOT][CIN]) and m-tiles mt = mt0 + i * mstride.
+template <int CIN, int NTOT, int MT>
+__device__ __forceinline__ void convTiles(const uint16_t* in, const uint16_t* __restrict__ W, int nt, int mt0,
+                                          int mstride, v4f (&acc)[MT]) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  for (int t = 0; t < 27; ++t) {
+#pragma unroll
+    for (int cb = 0; cb < CIN / 16; ++cb) {
+      const v4s b = *reinterpret_cast<const v4s*>(W + (static_cast<size_t>(t) * NTOT + nt * 16 + li) * CIN +
+                                                  cb * 16 + lk);
+#pragma unroll
+      for (int i = 0; i < MT; ++i) {
+        const int m = (mt0 + i * mstride) * 16 + li;
+        const v4s a = *reinterpret_cast<const v4s*>(in + padIndex(m, t) * CIN + cb * 16 + lk);
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// Zero a padded grid's 488 halo voxels (the 8^3 interior is overwritten).
+template <int C>
+__device__ void zeroHalo(uint16_t* g) {
+  for (int r = threadIdx.x; r < kGridVox; r += 256) {
+    const int x = r % kGrid, y = (r / kGrid) % kGrid, z = r / 100;
+    if (x == 0 || y == 0 || z == 0 || x == 9 || y == 9 || z == 9)
+      for (int c = 0; c < C; ++c) g[r * C + c] = 0;
+  }
+}
+
+__device__ __forceinline__ int interiorIndex(int m) {
+  const int x = m & 7, y = (m >> 3) & 7, z = m >> 6;
+  return ((z + 1) * kGrid + (y + 1)) * kGrid + (x + 1);
+}
+
+// Writes relu(acc + bias) as bf16 into the padded grid (interior) and,
+// optionally, to global memory [512][32].
+template <int MT>
+__device__ void storeHidden(const v4f (&acc)[MT], const float* bias, int nt, int mt0, int mstride, uint16_t* grid,
+                            uint16_t* gout) {
+  const int lane = threadIdx.x & 63;
+  const int n = nt * 16 + (lane & 15);
+  const float bb = bias[n];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = (mt0 + i * mstride) * 16 + (lane >> 4) * 4 + r;
+      float v = acc[i][r] + bb;
+      v = v > 0.0f ? v : 0.0f;
+      const uint16_t h = f2bf(v);
+      grid[interiorIndex(m) * kDnHid + n] = h;
+      if (gout) gout[m * kDnHid + n] = h;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- gather --
+
+namespace {
+
+__device__ uint32_t dnLookup(const VolumeView& V, int x, int y, int z) {
+  if (!keyInRange(x, y, z)) return kInvalid;
+  const uint64_t key = packKey(x, y, z);
+  uint32_t slot = hashSlot(key, V.log2cap);
+  const int bound = (V.mask + 1u) < static_cast<uint32_t>(kMaxProbe) ? static_cast<int>(V.mask + 1u) : kMaxProbe;
+  for (int probe = 0; probe < bound; ++probe) {
+    const uint64_t cur = V.keys[slot];
+    if (cur == key) {
+      const uint32_t p = V.vals[slot];
+      return p < V.max_blocks ? p : kInvalid;
+    }
+    if (cur == kEmptyKey) return kInvalid;
+    slot = (slot + 1u) & V.mask;
+  }
+  return kInvalid;
+}
+
+// Heaviest observed slot of direction d (first on ties): tsdf at the voxel
+// centre = planeSignedDistance / trunc clamped to [-1, 1], weight / max.
+__device__ void voxelFeature(const uint32_t* blk, int vi, int d, F3 centre, float inv_trunc, float inv_maxw,
+                             float& tsdf, float& w, int& slot_out) {
+  float bw = 0.0f;
+  int bk = -1;
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const float wk = __uint_as_float(blk[fieldOffset(d, 0, k) + vi]);
+    if (wk > bw) {
+      bw = wk;
+      bk = k;
+    }
+  }
+  slot_out = bk;
+  if (bk < 0) {
+    tsdf = 0.0f;
+    w = 0.0f;
+    return;
+  }
+  const F3 n = unpackNormalOct(blk[fieldOffset(d, 3, bk) + vi]);
+  const float sd = dot3(n, centre) - __uint_as_float(blk[fieldOffset(d, 2, bk) + vi]);
+  tsdf = sclamp(sd * inv_trunc, -1.0f, 1.0f);
+  w = bw * inv_maxw;
+}
+
+struct DnGeom {
+  float voxel, inv_trunc, inv_maxw;
+};
+
+// X[b][1000][16] for block keys[b] (with the 26 neighbours' halo voxels).
+__global__ __launch_bounds__(256) void dnGatherKernel(VolumeView V, const int32_t* __restrict__ keys, DnGeom g,
+                                                      uint16_t* __restrict__ X) {
+  __shared__ uint32_t src[27];
+  const int b = blockIdx.x;
+  const int kx = keys[3 * b], ky = keys[3 * b + 1], kz = keys[3 * b + 2];
+  if (threadIdx.x < 27) {
+    const int o = threadIdx.x;
+    src[o] = dnLookup(V, kx + o % 3 - 1, ky + (o / 3) % 3 - 1, kz + o / 9 - 1);
+  }
+  __syncthreads();
+  uint16_t* out = X + static_cast<size_t>(b) * kGridVox * kDnIn;
+  for (int r = threadIdx.x; r < kGridVox; r += 256) {
+    const int x = r % kGrid - 1, y = (r / kGrid) % kGrid - 1, z = r / 100 - 1;  // -1..8 in block voxels
+    const int bxo = x < 0 ? 0 : (x > 7 ? 2 : 1), byo = y < 0 ? 0 : (y > 7 ? 2 : 1), bzo = z < 0 ? 0 : (z > 7 ? 2 : 1);
+    const uint32_t p = src[bxo + 3 * (byo + 3 * bzo)];
+    uint16_t ch[kDnIn];
+#pragma unroll
+    for (int c = 0; c < kDnIn; ++c) ch[c] = 0;
+    if (p != kInvalid) {
+      const int vi = (x & 7) + 8 * ((y & 7) + 8 * (z & 7));
+      const F3 centre{(static_cast<float>(kx * 8 + x) + 0.5f) * g.voxel, (static_cast<float>(ky * 8 + y) + 0.5f) * g.voxel,
+                      (static_cast<float>(kz * 8 + z) + 0.5f) * g.voxel};
+      const uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
+#pragma unroll
+      for (int d = 0; d < kDirs; ++d) {
+        float ts, w;
+        int s;
+        voxelFeature(blk, vi, d, centre, g.inv_trunc, g.inv_maxw, ts, w, s);
+        ch[d] = f2bf(ts);
+        ch[6 + d] = f2bf(w);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kDnIn; ++c) out[r * kDnIn + c] = ch[c];
+  }
+}
+
+// Targets T[b][512][6] (tsdf) and masks M (1 where the target slot is observed).
+__global__ __launch_bounds__(256) void dnTargetKernel(VolumeView V, const int32_t* __restrict__ keys, DnGeom g,
+                                                      float* __restrict__ T, float* __restrict__ M) {
+  const int b = blockIdx.x;
+  const int kx = keys[3 * b], ky = keys[3 * b + 1], kz = keys[3 * b + 2];
+  const uint32_t p = dnLookup(V, kx, ky, kz);
+  for (int vi = threadIdx.x; vi < 512; vi += 256) {
+    const int x = vi & 7, y = (vi >> 3) & 7, z = vi >> 6;
+    const F3 centre{(static_cast<float>(kx * 8 + x) + 0.5f) * g.voxel, (static_cast<float>(ky * 8 + y) + 0.5f) * g.voxel,
+                    (static_cast<float>(kz * 8 + z) + 0.5f) * g.voxel};
+    for (int d = 0; d < kDirs; ++d) {
+      float ts = 0.0f, w = 0.0f;
+      int s = -1;
+      if (p != kInvalid) voxelFeature(V.pool + static_cast<size_t>(p) * kBlockWords, vi, d, centre, g.inv_trunc,
+                                      g.inv_maxw, ts, w, s);
+      T[(static_cast<size_t>(b) * 512 + vi) * kDnOut + d] = ts;
+      M[(static_cast<size_t>(b) * 512 + vi) * kDnOut + d] = s >= 0 ? 1.0f : 0.0f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- weights --
+
+__global__ __launch_bounds__(256) void dnPackKernel(const float* __restrict__ P, uint16_t* W1t, uint16_t* W2t,
+                                                    uint16_t* W3t, uint16_t* W2b, uint16_t* W3b) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  // W1t [27][32][16] from W1 (kz,ky,kx,12,32)
+  if (i < 27 * 32 * 16) {
+    const int c = i % 16, n = (i / 16) % 32, t = i / 512;
+    W1t[i] = c < 12 ? f2bf(P[kOffW1 + (t * 12 + c) * 32 + n]) : 0;
+  }
+  if (i < 27 * 32 * 32) {
+    const int c = i % 32, n = (i / 32) % 32, t = i / 1024;
+    W2t[i] = f2bf(P[kOffW2 + (t * 32 + c) * 32 + n]);
+    // W2b[t'][n=c_in][k=o]: flipped tap 26 - t', input channel of the forward as output
+    const int o = c, cin = n;
+    W2b[i] = f2bf(P[kOffW2 + ((26 - t) * 32 + cin) * 32 + o]);
+  }
+  if (i < 27 * 16 * 32) {
+    const int c = i % 32, n = (i / 32) % 16, t = i / 512;
+    W3t[i] = n < 6 ? f2bf(P[kOffW3 + (t * 32 + c) * 6 + n]) : 0;
+    // W3b[t'][n=cin 32][k=o 16]
+    const int o = i % 16, cin = (i / 16) % 32, tt = i / 512;
+    W3b[i] = o < 6 ? f2bf(P[kOffW3 + ((26 - tt) * 32 + cin) * 6 + o]) : 0;
+  }
+}
+
+// ---------------------------------------------------------------- forward --
+
+struct DnFwdArgs {
+  const uint16_t* X;
+  DnWeights w;
+  float* Y;          // [n][512][6] prediction (tsdf units)
+  // training outputs (null for inference)
+  uint16_t* H1;      // [n][512][32]
+  uint16_t* H2;      // [n][512][32]
+  const float* T;    // [n][512][6]
+  const float* M;
+  float* dY;         // [n][512][6]  dL/dy
+  float* loss;       // [gridDim.x] partial sums of M (y - T)^2
+  float inv_count;   // 1 / max(sum M, 1)
+};
+
+__global__ __launch_bounds__(256) void dnForwardKernel(DnFwdArgs a) {
+  __shared__ __align__(16) uint16_t gridA[kGridVox * kDnHid];  // X (16 ch) then h2
+  __shared__ __align__(16) uint16_t gridB[kGridVox * kDnHid];  // h1
+  __shared__ float red[4];
+  const int b = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint16_t* Xb = a.X + static_cast<size_t>(b) * kGridVox * kDnIn;
+  {  // X -> gridA (16 channels, 8-byte copies)
+    const uint2* s = reinterpret_cast<const uint2*>(Xb);
+    uint2* d = reinterpret_cast<uint2*>(gridA);
+    for (int i = threadIdx.x; i < kGridVox * kDnIn / 4; i += 256) d[i] = s[i];
+  }
+  zeroHalo<kDnHid>(gridB);
+  __syncthreads();
+  // conv1 (valid on X): 2 n-tiles x 32 m-tiles; wave: n-tile wave % 2, m-tiles wave / 2 + 2 i
+  {
+    v4f acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles<kDnIn, 32, 16>(gridA, a.w.W1t, wave & 1, wave >> 1, 2, acc);
+    storeHidden<16>(acc, a.w.b + kOffB1, wave & 1, wave >> 1, 2, gridB,
+                    a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+  }
+  __syncthreads();
+  zeroHalo<kDnHid>(gridA);  // X is dead; gridA becomes h2
+  __syncthreads();
+  {
+    v4f acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles<kDnHid, 32, 16>(gridB, a.w.W2t, wave & 1, wave >> 1, 2, acc);
+    storeHidden<16>(acc, a.w.b + kOffB2, wave & 1, wave >> 1, 2, gridA,
+                    a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+  }
+  __syncthreads();
+  // conv3: 1 n-tile (6 of 16 used) x 32 m-tiles; wave: m-tiles wave + 4 i
+  float lsum = 0.0f;
+  {
+    v4f acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles<kDnHid, 16, 8>(gridA, a.w.W3t, 0, wave, 4, acc);
+    const int n = lane & 15;
+    if (n < kDnOut) {
+      const float bb = a.w.b[kOffB3 + n];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = (wave + 4 * i) * 16 + (lane >> 4) * 4 + r;
+          const float x0 = bf2f(Xb[interiorIndex(m) * kDnIn + n]);
+          const float y = x0 + (acc[i][r] + bb);
+          const size_t o = (static_cast<size_t>(b) * 512 + m) * kDnOut + n;
+          a.Y[o] = y;
+          if (a.dY) {
+            const float mk = a.M[o];
+            const float diff = (y - a.T[o]) * mk;
+            lsum += diff * (y - a.T[o]);
+            a.dY[o] = 2.0f * diff * a.inv_count;
+          }
+        }
+    }
+  }
+  if (a.loss) {  // workgroup partial of the loss numerator
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+    if (lane == 0) red[wave] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) a.loss[b] = (red[0] + red[1]) + (red[2] + red[3]);
+  }
+}
+
+// ------------------------------------------------------------- backward --
+
+// Weight-gradient tiles over one block: dW[t][c][o] += sum_v in[pad(v, t)][c]
+// * dout[v][o] for this wavefront's taps t = wave + 4 j.  A = input^T
+// (i = c), B = dout (n = o), K = voxels (16 per instruction).
+template <int CIN, int COUTP, int NTAPS>
+__device__ __forceinline__ void accumulateDW(const uint16_t* in /*padded [1000][CIN] or X*/,
+                                             const uint16_t* dout /*[512][COUTP] bf16 LDS*/, int wave,
+                                             v4f (&acc)[NTAPS][CIN / 16][COUTP / 16]) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = (lane >> 4) * 4;
+  for (int kb = 0; kb < 32; ++kb) {  // 16 voxels per step
+#pragma unroll
+    for (int j = 0; j < NTAPS; ++j) {
+      const int t = wave + 4 * j;
+      if (t >= 27) break;
+#pragma unroll
+      for (int ct = 0; ct < CIN / 16; ++ct) {
+        v4s av;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) av[e] = static_cast<short>(in[padIndex(kb * 16 + lk + e, t) * CIN + ct * 16 + li]);
+#pragma unroll
+        for (int ot = 0; ot < COUTP / 16; ++ot) {
+          v4s bv;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) bv[e] = static_cast<short>(dout[(kb * 16 + lk + e) * COUTP + ot * 16 + li]);
+          acc[j][ct][ot] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j][ct][ot], 0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
+struct DnBwdArgs {
+  int n_blocks;
+  const uint16_t* X;     // [n][1000][16]
+  const uint16_t* H1;    // [n][512][32]
+  const uint16_t* H2;
+  const float* dY;       // [n][512][6]
+  uint16_t* dA2;         // [n][512][32]   (b3 output, b2 input)
+  uint16_t* dA1;         // [n][512][32]   (b2 output, b1 input)
+  DnWeights w;
+  float* partial;        // [gridDim.x][kNumParams]
+};
+
+// Loads a [512][C] bf16 activation into the interior of a padded grid.
+template <int C>
+__device__ void loadInterior(const uint16_t* __restrict__ src, uint16_t* grid) {
+  for (int i = threadIdx.x; i < 512 * C / 4; i += 256) {
+    const int m = (i * 4) / C, c = (i * 4) % C;
+    *reinterpret_cast<uint2*>(grid + interiorIndex(m) * C + c) = *reinterpret_cast<const uint2*>(src + m * C + c);
+  }
+}
+
+// Layer 3: dW3, db3 and dA2 = (conv(dY_pad, W3 flipped^T)) * (h2 > 0).
+__global__ __launch_bounds__(256) void dnBackward3Kernel(DnBwdArgs a) {
+  __shared__ __align__(16) uint16_t h2p[kGridVox * kDnHid];
+  __shared__ __align__(16) uint16_t dyp[kGridVox * kDnOutPad];   // padded dY (bf16, 16 ch)
+  __shared__ __align__(16) uint16_t dyc[512 * kDnOutPad];        // dY [512][16]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  v4f acc[7][2][1];
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[j][c][0] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  float db = 0.0f;
+  zeroHalo<kDnHid>(h2p);
+  zeroHalo<kDnOutPad>(dyp);
+  for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    __syncthreads();
+    loadInterior<kDnHid>(a.H2 + static_cast<size_t>(b) * 512 * kDnHid, h2p);
+    for (int i = threadIdx.x; i < 512 * kDnOutPad; i += 256) {
+      const int m = i / kDnOutPad, o = i % kDnOutPad;
+      const float g = o < kDnOut ? a.dY[(static_cast<size_t>(b) * 512 + m) * kDnOut + o] : 0.0f;
+      const uint16_t h = f2bf(g);
+      dyc[i] = h;
+      dyp[interiorIndex(m) * kDnOutPad + o] = h;
+    }
+    __syncthreads();
+    // db3 (fp32 from dY directly): lanes 0..5 of wave 0
+    if (wave == 0 && lane < kDnOut)
+      for (int m = 0; m < 512; ++m) db += a.dY[(static_cast<size_t>(b) * 512 + m) * kDnOut + lane];
+    accumulateDW<kDnHid, kDnOutPad, 7>(h2p, dyc, wave, acc);
+    // dA2 = conv(dyp, W3b) masked by h2 > 0: 2 n-tiles x 32 m-tiles
+    v4f g[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) g[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles<kDnOutPad, 32, 16>(dyp, a.w.W3b, wave & 1, wave >> 1, 2, g);
+    const int n = (wave & 1) * 16 + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = ((wave >> 1) + 2 * i) * 16 + (lane >> 4) * 4 + r;
+        const bool on = h2p[interiorIndex(m) * kDnHid + n] != 0;
+        a.dA2[(static_cast<size_t>(b) * 512 + m) * kDnHid + n] = on ? f2bf(g[i][r]) : 0;
+      }
+  }
+  // per-workgroup partial gradient: dW3[t][c][o] at kOffW3 + (t*32 + c)*6 + o
+  float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const int t = wave + 4 * j;
+    if (t >= 27) break;
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c = ct * 16 + (lane >> 4) * 4 + r, o = lane & 15;
+        if (o < kDnOut) P[kOffW3 + (t * 32 + c) * 6 + o] = acc[j][ct][0][r];
+      }
+  }
+  if (wave == 0 && lane < kDnOut) P[kOffB3 + lane] = db;
+}
+
+// Layer 2: dW2, db2 and dA1 = conv(dA2_pad, W2 flipped^T) * (h1 > 0).
+__global__ __launch_bounds__(256) void dnBackward2Kernel(DnBwdArgs a) {
+  __shared__ __align__(16) uint16_t h1p[kGridVox * kDnHid];
+  __shared__ __align__(16) uint16_t dap[kGridVox * kDnHid];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  v4f acc[7][2][2];
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int o = 0; o < 2; ++o) acc[j][c][o] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  float db = 0.0f;
+  zeroHalo<kDnHid>(h1p);
+  zeroHalo<kDnHid>(dap);
+  for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    __syncthreads();
+    loadInterior<kDnHid>(a.H1 + static_cast<size_t>(b) * 512 * kDnHid, h1p);
+    loadInterior<kDnHid>(a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, dap);
+    __syncthreads();
+    if (wave == 0 && lane < kDnHid)
+      for (int m = 0; m < 512; ++m) db += bf2f(dap[interiorIndex(m) * kDnHid + lane]);
+    // dW2 over voxels: A = h1p patches (i = c), B = dA2 (n = o) read from the padded grid interior
+    {
+      const int li = lane & 15, lk = (lane >> 4) * 4;
+      for (int kb = 0; kb < 32; ++kb) {
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+          const int t = wave + 4 * j;
+          if (t >= 27) break;
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct) {
+            v4s av;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) av[e] = static_cast<short>(h1p[padIndex(kb * 16 + lk + e, t) * kDnHid + ct * 16 + li]);
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) {
+              v4s bv;
+#pragma unroll
+              for (int e = 0; e < 4; ++e)
+                bv[e] = static_cast<short>(dap[interiorIndex(kb * 16 + lk + e) * kDnHid + ot * 16 + li]);
+              acc[j][ct][ot] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j][ct][ot], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+    v4f g[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) g[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles<kDnHid, 32, 16>(dap, a.w.W2b, wave & 1, wave >> 1, 2, g);
+    const int n = (wave & 1) * 16 + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = ((wave >> 1) + 2 * i) * 16 + (lane >> 4) * 4 + r;
+        const bool on = h1p[interiorIndex(m) * kDnHid + n] != 0;
+        a.dA1[(static_cast<size_t>(b) * 512 + m) * kDnHid + n] = on ? f2bf(g[i][r]) : 0;
+      }
+  }
+  float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const int t = wave + 4 * j;
+    if (t >= 27) break;
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int ot = 0; ot < 2; ++ot)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int c = ct * 16 + (lane >> 4) * 4 + r, o = ot * 16 + (lane & 15);
+          P[kOffW2 + (t * 32 + c) * 32 + o] = acc[j][ct][ot][r];
+        }
+  }
+  if (wave == 0 && lane < kDnHid) P[kOffB2 + lane] = db;
+}
+
+// Layer 1: dW1, db1 from X and dA1.
+__global__ __launch_bounds__(256) void dnBackward1Kernel(DnBwdArgs a) {
+  __shared__ __align__(16) uint16_t xg[kGridVox * kDnIn];
+  __shared__ __align__(16) uint16_t da[512 * kDnHid];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  v4f acc[7][1][2];
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+#pragma unroll
+    for (int o = 0; o < 2; ++o) acc[j][0][o] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  float db = 0.0f;
+  for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
+    __syncthreads();
+    {
+      const uint2* s = reinterpret_cast<const uint2*>(a.X + static_cast<size_t>(b) * kGridVox * kDnIn);
+      uint2* d = reinterpret_cast<uint2*>(xg);
+      for (int i = threadIdx.x; i < kGridVox * kDnIn / 4; i += 256) d[i] = s[i];
+      const uint2* s2 = reinterpret_cast<const uint2*>(a.dA1 + static_cast<size_t>(b) * 512 * kDnHid);
+      uint2* d2 = reinterpret_cast<uint2*>(da);
+      for (int i = threadIdx.x; i < 512 * kDnHid / 4; i += 256) d2[i] = s2[i];
+    }
+    __syncthreads();
+    if (wave == 0 && lane < kDnHid)
+      for (int m = 0; m < 512; ++m) db += bf2f(da[m * kDnHid + lane]);
+    accumulateDW<kDnIn, kDnHid, 7>(xg, da, wave, acc);
+  }
+  float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const int t = wave + 4 * j;
+    if (t >= 27) break;
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c = (lane >> 4) * 4 + r, o = ot * 16 + (lane & 15);
+        if (c < 12) P[kOffW1 + (t * 12 + c) * 32 + o] = acc[j][0][ot][r];
+      }
+  }
+  if (wave == 0 && lane < kDnHid) P[kOffB1 + lane] = db;
+}
+
+// grad[i] = sum over workgroups g (in order) of partial[g][i].
+__global__ __launch_bounds__(256) void dnReduceKernel(const float* __restrict__ partial, int groups,
+                                                      float* __restrict__ grad) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kNumParams) return;
+  float s = 0.0f;
+  for (int g = 0; g < groups; ++g) s += partial[static_cast<size_t>(g) * kNumParams + i];
+  grad[i] = s;
+}
+
+// Adam (Kingma & Ba) with bias correction; fp32 master parameters.
+__global__ __launch_bounds__(256) void dnAdamKernel(float* __restrict__ P, const float* __restrict__ G,
+                                                    float* __restrict__ Mm, float* __restrict__ Vv, float lr, float b1,
+                                                    float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kNumParams) return;
+  const float g = G[i];
+  const float m = b1 * Mm[i] + (1.0f - b1) * g;
+  const float v = b2 * Vv[i] + (1.0f - b2) * g * g;
+  Mm[i] = m;
+  Vv[i] = v;
+  P[i] = P[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+// Write-back with confidence fallback.  For each voxel and direction whose
+// heaviest slot is observed and whose input weight is at least min_w, and
+// whose predicted change |y - x| is at most max_delta (in tsdf units), the
+// slot's plane_offset is set so that the plane's signed distance at the
+// voxel centre equals y * trunc.  Blocks whose mean |y - x| over such cells
+// exceeds block_max_delta keep plain fusion entirely.
+struct DnApplyArgs {
+  const int32_t* keys;
+  const float* Y;
+  const uint16_t* X;
+  DnGeom g;
+  float trunc, min_w, max_delta, block_max_delta;
+  unsigned long long* counters;  // [0] cells replaced, [1] blocks kept plain
+};
+
+__global__ __launch_bounds__(256) void dnApplyKernel(VolumeView V, DnApplyArgs a) {
+  __shared__ float s_sum[256];
+  __shared__ int s_cnt[256];
+  const int b = blockIdx.x;
+  const int kx = a.keys[3 * b], ky = a.keys[3 * b + 1], kz = a.keys[3 * b + 2];
+  const uint32_t p = dnLookup(V, kx, ky, kz);
+  float sum = 0.0f;
+  int cnt = 0;
+  for (int vi = threadIdx.x; vi < 512; vi += 256)
+    for (int d = 0; d < kDirs; ++d) {
+      const uint16_t* xr = a.X + (static_cast<size_t>(b) * kGridVox + interiorIndex(vi)) * kDnIn;
+      if (bf2f(xr[6 + d]) <= 0.0f) continue;
+      const float delta = fabsf(a.Y[(static_cast<size_t>(b) * 512 + vi) * kDnOut + d] - bf2f(xr[d]));
+      sum += delta;
+      ++cnt;
+    }
+  s_sum[threadIdx.x] = sum;
+  s_cnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + off];
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  const bool keep_plain = p == kInvalid || s_cnt[0] == 0 || s_sum[0] / static_cast<float>(s_cnt[0]) > a.block_max_delta;
+  if (keep_plain) {
+    if (threadIdx.x == 0) atomicAdd(&a.counters[1], 1ull);
+    return;
+  }
+  unsigned long long replaced = 0;
+  uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
+  for (int vi = threadIdx.x; vi < 512; vi += 256) {
+    const int x = vi & 7, y = (vi >> 3) & 7, z = vi >> 6;
+    const F3 centre{(static_cast<float>(kx * 8 + x) + 0.5f) * a.g.voxel, (static_cast<float>(ky * 8 + y) + 0.5f) * a.g.voxel,
+                    (static_cast<float>(kz * 8 + z) + 0.5f) * a.g.voxel};
+    for (int d = 0; d < kDirs; ++d) {
+      float ts, w;
+      int s;
+      voxelFeature(blk, vi, d, centre, a.g.inv_trunc, a.g.inv_maxw, ts, w, s);
+      if (s < 0 || w < a.min_w) continue;
+      const float yv = a.Y[(static_cast<size_t>(b) * 512 + vi) * kDnOut + d];
+      const uint16_t* xr = a.X + (static_cast<size_t>(b) * kGridVox + interiorIndex(vi)) * kDnIn;
+      if (fabsf(yv - bf2f(xr[d])) > a.max_delta) continue;
+      const F3 n = unpackNormalOct(blk[fieldOffset(d, 3, s) + vi]);
+      const float yc = sclamp(yv, -1.0f, 1.0f);
+      blk[fieldOffset(d, 2, s) + vi] = __float_as_uint(dot3(n, centre) - yc * a.trunc);
+      ++replaced;
+    }
+  }
+  if (replaced) atomicAdd(&a.counters[0], replaced);
+  if (threadIdx.x == 0) V.dirty[p] = 1u;
+}
+
+// 16x16x16 bf16 MFMA layout probe: A[i][k] = i + 16 k / 256, B = identity;
+// out[lane * 4 + r] = D value held by (lane, r).
+__global__ void dnMfmaProbeKernel(float* out) {
+  const int lane = threadIdx.x;
+  v4s a, b;
+  for (int j = 0; j < 4; ++j) {
+    const int i = lane & 15, k = (lane >> 4) * 4 + j;
+    a[j] = static_cast<short>(f2bf(static_cast<float>(i) + static_cast<float>(k) / 16.0f));
+    b[j] = static_cast<short>(f2bf((k == (lane & 15)) ? 1.0f : 0.0f));
+  }
+  v4f c = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = c[r];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers --
+
+hipError_t dnLaunchGather(const VolumeView& V, const int32_t* keys, uint32_t n, float voxel, float trunc,
+                          float max_weight, uint16_t* X, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  dnGatherKernel<<<n, 256, 0, s>>>(V, keys, DnGeom{voxel, 1.0f / trunc, 1.0f / max_weight}, X);
+  return hipGetLastError();
+}
+hipError_t dnLaunchTarget(const VolumeView& V, const int32_t* keys, uint32_t n, float voxel, float trunc,
+                          float max_weight, float* T, float* M, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  dnTargetKernel<<<n, 256, 0, s>>>(V, keys, DnGeom{voxel, 1.0f / trunc, 1.0f / max_weight}, T, M);
+  return hipGetLastError();
+}
+hipError_t dnLaunchPack(const float* P, uint16_t* W1t, uint16_t* W2t, uint16_t* W3t, uint16_t* W2b, uint16_t* W3b,
+                        hipStream_t s) {
+  dnPackKernel<<<(27 * 32 * 32 + 255) / 256, 256, 0, s>>>(P, W1t, W2t, W3t, W2b, W3b);
+  return hipGetLastError();
+}
+hipError_t dnLaunchForward(const uint16_t* X, const DnWeights& w, uint32_t n, float* Y, uint16_t* H1, uint16_t* H2,
+                           const float* T, const float* M, float* dY, float* loss, float inv_count, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  DnFwdArgs a{X, w, Y, H1, H2, T, M, dY, loss, inv_count};
+  dnForwardKernel<<<n, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+hipError_t dnLaunchBackward(const uint16_t* X, const uint16_t* H1, const uint16_t* H2, const float* dY,
+                            uint16_t* dA2, uint16_t* dA1, const DnWeights& w, uint32_t n, int groups, float* partial,
+                            float* grad, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  DnBwdArgs a{static_cast<int>(n), X, H1, H2, dY, dA2, dA1, w, partial};
+  // each layer kernel writes every partial entry of its layer, so the
+  // reduction never reads stale values
+  dnBackward3Kernel<<<groups, 256, 0, s>>>(a);
+  dnBackward2Kernel<<<groups, 256, 0, s>>>(a);
+  dnBackward1Kernel<<<groups, 256, 0, s>>>(a);
+  dnReduceKernel<<<(kNumParams + 255) / 256, 256, 0, s>>>(partial, groups, grad);
+  return hipGetLastError();
+}
+hipError_t dnLaunchAdam(float* P, const float* G, float* M, float* V, float lr, float b1, float b2, float eps,
+                        int step, hipStream_t s) {
+  const float c1 = 1.0f - powf(b1, static_cast<float>(step)), c2 = 1.0f - powf(b2, static_cast<float>(step));
+  dnAdamKernel<<<(kNumParams + 255) / 256, 256, 0, s>>>(P, G, M, V, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+hipError_t dnLaunchApply(const VolumeView& V, const int32_t* keys, uint32_t n, const float* Y, const uint16_t* X,
+                         float voxel, float trunc, float max_weight, float min_w, float max_delta,
+                         float block_max_delta, unsigned long long* counters, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  DnApplyArgs a{keys, Y, X, DnGeom{voxel, 1.0f / trunc, 1.0f / max_weight}, trunc, min_w, max_delta,
+                block_max_delta, counters};
+  dnApplyKernel<<<n, 256, 0, s>>>(V, a);
+  return hipGetLastError();
+}
+hipError_t dnLaunchProbe(float* out, hipStream_t s) {
+  dnMfmaProbeKernel<<<1, 64, 0, s>>>(out);
+  return hipGetLastError();
+}
+int dnNumParams() { return kNumParams; }
+
+}  // namespace vhm

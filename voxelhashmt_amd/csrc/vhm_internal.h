@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/vhm.h"
 #include "vhm_math.h"
 
 namespace vhm {
@@ -135,3 +136,39 @@ hipError_t launchSynthRender(const SynthDeviceScene& sc, const Intrinsics& K, co
                              uint32_t first_frame, const SynthNoise& nz, uint16_t* out, hipStream_t s);
 
 }  // namespace vhm
+
+namespace vhm {
+// Denoiser launchers (vhm_denoise.hip).
+struct DnWeights {       // packed device weights
+  const uint16_t* W1t;   // [27][32 n][16 c]
+  const uint16_t* W2t;   // [27][32 n][32 c]
+  const uint16_t* W3t;   // [27][16 n][32 c]   (n >= 6 zero)
+  const uint16_t* W2b;   // [27][32 n=c][32 k=o]  flipped taps, for dh1 = conv(da2, .)
+  const uint16_t* W3b;   // [27][32 n=c][16 k=o]  flipped taps, for dh2 = conv(dy, .)
+  const float* b;        // fp32 master parameters (biases are read from here)
+};
+hipError_t dnLaunchGather(const VolumeView& V, const int32_t* keys, uint32_t n, float voxel, float trunc,
+                          float max_weight, uint16_t* X, hipStream_t s);
+hipError_t dnLaunchTarget(const VolumeView& V, const int32_t* keys, uint32_t n, float voxel, float trunc,
+                          float max_weight, float* T, float* M, hipStream_t s);
+hipError_t dnLaunchPack(const float* P, uint16_t* W1t, uint16_t* W2t, uint16_t* W3t, uint16_t* W2b, uint16_t* W3b,
+                        hipStream_t s);
+hipError_t dnLaunchForward(const uint16_t* X, const DnWeights& w, uint32_t n, float* Y, uint16_t* H1, uint16_t* H2,
+                           const float* T, const float* M, float* dY, float* loss, float inv_count, hipStream_t s);
+hipError_t dnLaunchBackward(const uint16_t* X, const uint16_t* H1, const uint16_t* H2, const float* dY,
+                            uint16_t* dA2, uint16_t* dA1, const DnWeights& w, uint32_t n, int groups, float* partial,
+                            float* grad, hipStream_t s);
+hipError_t dnLaunchAdam(float* P, const float* G, float* M, float* V, float lr, float b1, float b2, float eps,
+                        int step, hipStream_t s);
+hipError_t dnLaunchApply(const VolumeView& V, const int32_t* keys, uint32_t n, const float* Y, const uint16_t* X,
+                         float voxel, float trunc, float max_weight, float min_w, float max_delta,
+                         float block_max_delta, unsigned long long* counters, hipStream_t s);
+hipError_t dnLaunchProbe(float* out, hipStream_t s);
+int dnNumParams();
+}  // namespace vhm
+
+// Context internals the denoiser needs (defined in vhm_capi.hip).
+struct vhm_ctx;
+vhm::VolumeView vhmCtxView(vhm_ctx* c);
+hipStream_t vhmCtxStream(vhm_ctx* c);
+const vhm_params* vhmCtxParams(vhm_ctx* c);
