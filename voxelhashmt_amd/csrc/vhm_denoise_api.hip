@@ -14,6 +14,7 @@ using namespace vhm;
 
 struct vhm_denoiser {
   int capacity = 0;  // blocks in the training/inference set
+  int batch_capacity = 0;  // blocks per training step
   int count = 0;
   hipStream_t stream = nullptr;
   int groups = 512;  // backward workgroups (partial gradient rows)
@@ -80,6 +81,7 @@ int vhm_denoiser_create(int capacity, int batch_capacity, vhm_denoiser** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return VHM_ERR_NO_DEVICE;
   auto* d = new vhm_denoiser();
   d->capacity = capacity;
+  d->batch_capacity = batch_capacity;
   const size_t np = dnNumParams();
   const size_t cap = capacity, bc = batch_capacity;
   hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
@@ -210,6 +212,8 @@ int vhm_denoiser_forward(vhm_denoiser* d, size_t offset, size_t n, float* y_out)
 // lr <= 0 computes loss and gradient without updating the parameters.
 int vhm_denoiser_train_step(vhm_denoiser* d, size_t offset, size_t n, float lr, float* loss_out, float* grad_out) {
   if (!d || n == 0 || offset + n > static_cast<size_t>(d->count)) return VHM_ERR_INVALID_ARGUMENT;
+  if (n > static_cast<size_t>(d->batch_capacity))
+    return dfail(d, VHM_ERR_INVALID_ARGUMENT, "denoiser train step failed: batch larger than batch_capacity");
   // mask count (host side, from the device masks) -> 1 / max(sum M, 1)
   std::vector<float> mh(n * kYElems);
   DN_TRY(d, hipMemcpyAsync(mh.data(), d->M + offset * kYElems, mh.size() * 4, hipMemcpyDeviceToHost, d->stream),
