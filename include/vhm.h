@@ -219,6 +219,44 @@ int vhm_denoiser_samples(vhm_denoiser* d, size_t offset, size_t n, uint16_t* x_o
  * assume: out[4 * lane + r] = D element held by (lane, r). */
 int vhm_mfma_layout_probe(float* out256);
 
+/* --- batched multi-scene fusion (no reference counterpart) -------------- */
+
+/* S independent volumes (one vhm_ctx each, with its own params) advance one
+ * frame per step: one launch per pipeline stage covers every scene (filter,
+ * allocation, integration, incremental marching cubes of the step's dirty
+ * blocks), and the whole step is captured once as a hipGraph and replayed.
+ * Results per scene equal vhm_integrate_u16 frame by frame followed by
+ * vhm_extract_mesh_incremental.  Each scene's frames (uint16, device,
+ * n_frames x H x W) and poses are given once; scenes with shorter sequences
+ * stop integrating once past their last frame.  The volumes are driven by the
+ * batch: call vhm_batch_synchronize before reading them through the vhm_ctx
+ * functions, and do not integrate into them while batch steps are in flight. */
+typedef struct vhm_batch vhm_batch;
+int vhm_batch_create(int num_scenes, const vhm_params* params /* [num_scenes] */, vhm_batch** out);
+int vhm_batch_destroy(vhm_batch* b);
+const char* vhm_batch_last_error(const vhm_batch* b);
+vhm_ctx* vhm_batch_volume(vhm_batch* b, int scene);
+/* frames_device must stay valid while steps run; tri_capacity: triangles
+ * kept per step's incremental mesh. */
+int vhm_batch_set_sequence(vhm_batch* b, int scene, const uint16_t* frames_device, int n_frames, float depth_scale,
+                           const vhm_intrinsics* K, const vhm_pose* poses, size_t tri_capacity);
+/* After every set_sequence; grids <= 0 keep the current values; use_graph
+ * captures the step as a hipGraph. */
+int vhm_batch_prepare(vhm_batch* b, int integrate_grid, int mesh_grid, int use_graph);
+/* Enqueues n steps (asynchronous). */
+int vhm_batch_step(vhm_batch* b, int n_steps);
+int vhm_batch_synchronize(vhm_batch* b);
+int vhm_batch_steps_done(const vhm_batch* b);
+/* Last step's incremental mesh of a scene: blocks re-meshed, triangle total
+ * (may exceed tri_capacity), and up to capacity triangles (9 floats, host). */
+int vhm_batch_mesh_output(vhm_batch* b, int scene, uint32_t* blocks, uint32_t* triangles, float* tris_host,
+                          size_t capacity);
+/* Runs one step per candidate (multiples of the CU count) with direct
+ * launches and keeps, per persistent kernel, the grid with the least time
+ * per unit of work (a captured graph is re-captured with them). */
+int vhm_batch_autotune(vhm_batch* b, const int* candidates, int n_candidates, int* integrate_grid_out,
+                       int* mesh_grid_out);
+
 /* --- synthetic depth (no reference counterpart; benchmark/test input) ---- */
 
 /* Analytic scene: the interior of an axis-aligned room plus spheres

@@ -20,9 +20,11 @@ REL_TOL = 1e-6  # per-voxel layer tolerance (relative)
 
 
 def _params(w, **kw):
-    return vhm.default_params(w.voxel_size, w.truncation, filter_sigma_spatial=w.filter_sigma_spatial,
-                              filter_sigma_range=w.filter_sigma_range, hash_log2_buckets=w.hash_log2_buckets,
-                              max_blocks=w.max_blocks, max_width=w.K.width, max_height=w.K.height, **kw)
+    base = dict(filter_sigma_spatial=w.filter_sigma_spatial, filter_sigma_range=w.filter_sigma_range,
+                hash_log2_buckets=w.hash_log2_buckets, max_blocks=w.max_blocks, max_width=w.K.width,
+                max_height=w.K.height)
+    base.update(kw)
+    return vhm.default_params(w.voxel_size, w.truncation, **base)
 
 
 def _metres(frames_u16):

@@ -116,6 +116,20 @@ _SIGS = {
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vhm_denoiser_samples": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vhm_mfma_layout_probe": (C.c_int, [C.c_void_p]),
+    "vhm_batch_create": (C.c_int, [C.c_int, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "vhm_batch_destroy": (C.c_int, [C.c_void_p]),
+    "vhm_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "vhm_batch_volume": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "vhm_batch_set_sequence": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float,
+                                         C.POINTER(Intrinsics), C.POINTER(Pose), C.c_size_t]),
+    "vhm_batch_prepare": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "vhm_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "vhm_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "vhm_batch_steps_done": (C.c_int, [C.c_void_p]),
+    "vhm_batch_mesh_output": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.c_void_p, C.c_size_t]),
+    "vhm_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
 }
 
 
@@ -202,14 +216,24 @@ class Volume:
 
     def __init__(self, params: Params):
         self.params = params
+        self._owned = True
         self._h = C.c_void_p()
         rc = lib().vhm_create(C.byref(params), C.byref(self._h))
         if rc != 0:
             raise VhmError(rc, "vhm_create failed")
 
+    @classmethod
+    def _borrow(cls, handle: int, params: Params, owner) -> "Volume":
+        """A view of a volume owned by a Batch (not destroyed by close())."""
+        v = cls.__new__(cls)
+        v.params, v._owned, v._owner = params, False, owner
+        v._h = C.c_void_p(handle)
+        return v
+
     def close(self):
         if self._h:
-            lib().vhm_destroy(self._h)
+            if self._owned:
+                lib().vhm_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -420,6 +444,87 @@ class Denoiser:
         m = np.zeros((n, 512, 6), np.float32)
         self._check(lib().vhm_denoiser_samples(self._h, offset, n, x.ctypes.data, t.ctypes.data, m.ctypes.data))
         return x, t, m
+
+
+class Batch:
+    """Batched multi-scene fusion (vhm_batch_*): S volumes advance one frame
+    per step, one launch per stage for all scenes, the step replayed as a
+    captured hipGraph."""
+
+    def __init__(self, params_list):
+        self.params = list(params_list)
+        arr = (Params * len(self.params))(*self.params)
+        self._h = C.c_void_p()
+        rc = lib().vhm_batch_create(len(self.params), arr, C.byref(self._h))
+        if rc != 0:
+            raise VhmError(rc, "vhm_batch_create failed")
+        self._keep = []  # device frame tensors must outlive the steps
+        self.volumes = [Volume._borrow(lib().vhm_batch_volume(self._h, i), p, self)
+                        for i, p in enumerate(self.params)]
+
+    def close(self):
+        if self._h:
+            lib().vhm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VhmError(rc, lib().vhm_batch_last_error(self._h).decode())
+
+    def set_sequence(self, scene: int, frames, K: Intrinsics, poses, depth_scale: float = 0.001,
+                     tri_capacity: int = 1 << 20):
+        """frames: device int16/uint16 tensor (n, H, W)."""
+        if not getattr(frames, "is_cuda", False):
+            raise ValueError("batch frames must be a device tensor")
+        n = int(frames.shape[0])
+        P = (Pose * n)(*poses[:n])
+        self._keep.append(frames)
+        self._check(lib().vhm_batch_set_sequence(self._h, scene, frames.data_ptr(), n, depth_scale, C.byref(K), P,
+                                                 tri_capacity))
+
+    def prepare(self, integrate_grid: int = 0, mesh_grid: int = 0, use_graph: bool = True):
+        self._check(lib().vhm_batch_prepare(self._h, integrate_grid, mesh_grid, int(use_graph)))
+
+    def step(self, n: int = 1):
+        self._check(lib().vhm_batch_step(self._h, n))
+
+    def synchronize(self):
+        self._check(lib().vhm_batch_synchronize(self._h))
+
+    @property
+    def steps_done(self) -> int:
+        return lib().vhm_batch_steps_done(self._h)
+
+    def mesh_output(self, scene: int, fetch: bool = True):
+        """(blocks re-meshed, triangle total, triangles (n,3,3) or None) of
+        the last step's incremental mesh of one scene."""
+        blocks, tris = C.c_uint32(), C.c_uint32()
+        self._check(lib().vhm_batch_mesh_output(self._h, scene, C.byref(blocks), C.byref(tris), None, 0))
+        out = None
+        if fetch:
+            out = np.zeros((tris.value, 3, 3), np.float32)
+            if tris.value:
+                self._check(lib().vhm_batch_mesh_output(self._h, scene, None, None, out.ctypes.data, tris.value))
+        return blocks.value, tris.value, out
+
+    def autotune(self, candidates=(2, 4, 8, 16)) -> tuple[int, int]:
+        c = (C.c_int * len(candidates))(*candidates)
+        gi, gm = C.c_int(), C.c_int()
+        self._check(lib().vhm_batch_autotune(self._h, c, len(candidates), C.byref(gi), C.byref(gm)))
+        return gi.value, gm.value
 
 
 def mfma_layout_probe() -> np.ndarray:

@@ -21,7 +21,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_obj"
 LIB = PKG / "libvhm.so"
 SOURCES = ["vhm_filter.hip", "vhm_alloc.hip", "vhm_integrate.hip", "vhm_mesh.hip", "vhm_synth.hip",
-           "vhm_capi.hip", "vhm_denoise.hip", "vhm_denoise_api.hip"]
+           "vhm_capi.hip", "vhm_denoise.hip", "vhm_denoise_api.hip", "vhm_batch.hip"]
 ARCH = os.environ.get("VHM_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

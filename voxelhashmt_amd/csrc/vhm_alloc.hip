@@ -81,7 +81,9 @@ __device__ __forceinline__ int worldToVoxel1(float x, float voxel) {
   return static_cast<int>(floorf(x / voxel));
 }
 
-__global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ depth, VolumeView V, AllocArgs a) {
+// One 16x16 sample tile of the allocation rule for one frame.
+__device__ void allocTile(const float* __restrict__ depth, const VolumeView& V, const AllocArgs& a, int tile_x,
+                          int tile_y) {
   __shared__ uint64_t centres[kCentreSlots];
   __shared__ uint64_t centre_list[3 * 256];
   __shared__ uint64_t nbrs[kNbrSlots];
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ 
   __syncthreads();
 
   // 1. centre keys of this lane's pixel (voxel_hash_tsdf.h:645-661).
-  const int sx = blockIdx.x * kTile + tid % kTile, sy = blockIdx.y * kTile + tid / kTile;
+  const int sx = tile_x * kTile + tid % kTile, sy = tile_y * kTile + tid / kTile;
   const int u = sx * a.stride, v = sy * a.stride;
   if (u < a.K.width && v < a.K.height) {
     const float d = depth[static_cast<size_t>(v) * a.K.width + u];
@@ -159,12 +161,83 @@ __global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ 
   }
 }
 
+__global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ depth, VolumeView V, AllocArgs a) {
+  allocTile(depth, V, a, blockIdx.x, blockIdx.y);
+}
+
 __global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) {
   hdr->active_count = 0u;
   hdr->frame = frame;
 }
 
+// ---- batched steps ----
+
+__global__ void batchFrameBeginKernel(const BatchScene* __restrict__ scenes, int S, const BatchState* __restrict__ st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  scenes[s].V.hdr->active_count = 0u;
+  scenes[s].V.hdr->frame = st->step + 1u;
+}
+
+__global__ __launch_bounds__(256) void allocateBatchKernel(const BatchScene* __restrict__ scenes, int S,
+                                                           const uint32_t* __restrict__ tile_prefix,
+                                                           const BatchState* __restrict__ st) {
+  const uint32_t step = st->step;
+  const int s = findSegment(tile_prefix, S, blockIdx.x);
+  const BatchScene& sc = scenes[s];
+  if (step >= sc.n_frames) return;
+  AllocArgs a = sc.aa;
+  a.T_wc = sc.poses[step];
+  a.frame = step + 1u;
+  const uint32_t t = blockIdx.x - tile_prefix[s];
+  const int sw = (a.K.width + a.stride - 1) / a.stride;
+  const uint32_t tiles_x = (sw + kTile - 1) / kTile;
+  allocTile(sc.filtered, sc.V, a, t % tiles_x, t / tiles_x);
+}
+
+// prefix[s] = number of active blocks of scenes < s this step (S + 1 entries).
+__global__ void activePrefixKernel(const BatchScene* __restrict__ scenes, int S, const BatchState* __restrict__ st,
+                                   uint32_t* __restrict__ prefix) {
+  if (threadIdx.x != 0) return;
+  const uint32_t step = st->step;
+  uint32_t acc = 0;
+  for (int s = 0; s < S; ++s) {
+    prefix[s] = acc;
+    if (step < scenes[s].n_frames) acc += scenes[s].V.hdr->active_count;
+  }
+  prefix[S] = acc;
+}
+
+__global__ void stepIncrementKernel(BatchState* st) { st->step += 1u; }
+
 }  // namespace
+
+uint32_t allocTiles(const Intrinsics& K, int stride) {
+  const int sw = (K.width + stride - 1) / stride, sh = (K.height + stride - 1) / stride;
+  return static_cast<uint32_t>(((sw + kTile - 1) / kTile) * ((sh + kTile - 1) / kTile));
+}
+
+hipError_t launchBatchFrameBegin(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s) {
+  batchFrameBeginKernel<<<(S + 63) / 64, 64, 0, s>>>(scenes, S, st);
+  return hipGetLastError();
+}
+
+hipError_t launchAllocateBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
+                               const BatchState* st, hipStream_t s) {
+  allocateBatchKernel<<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  return hipGetLastError();
+}
+
+hipError_t launchActivePrefix(const BatchScene* scenes, int S, const BatchState* st, uint32_t* prefix,
+                              hipStream_t s) {
+  activePrefixKernel<<<1, 64, 0, s>>>(scenes, S, st, prefix);
+  return hipGetLastError();
+}
+
+hipError_t launchStepIncrement(BatchState* st, hipStream_t s) {
+  stepIncrementKernel<<<1, 1, 0, s>>>(st);
+  return hipGetLastError();
+}
 
 hipError_t launchFrameBegin(DevHeader* hdr, uint32_t frame, hipStream_t s) {
   frameBeginKernel<<<1, 1, 0, s>>>(hdr, frame);

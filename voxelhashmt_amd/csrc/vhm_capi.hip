@@ -42,6 +42,7 @@ struct vhm_ctx {
   uint32_t* pool = nullptr;
   uint32_t* obs = nullptr;
   uint32_t* dirty = nullptr;
+  uint32_t* dirmask = nullptr;
   uint32_t* active = nullptr;
   DevHeader* hdr = nullptr;
   // per-frame buffers
@@ -105,6 +106,7 @@ VolumeView view(vhm_ctx* c) {
   V.pool = c->pool;
   V.obs = c->obs;
   V.dirty = c->dirty;
+  V.dirmask = c->dirmask;
   V.max_blocks = c->P.max_blocks;
   V.active = c->active;
   V.hdr = c->hdr;
@@ -301,6 +303,7 @@ int initVolume(vhm_ctx* c) {
   VHM_TRY(c, hipMemsetAsync(c->pool, 0, size_t(c->P.max_blocks) * kBlockWords * 4u, s), "hipMemset pool");
   VHM_TRY(c, hipMemsetAsync(c->obs, 0, size_t(c->P.max_blocks) * 16u * 4u, s), "hipMemset observed");
   VHM_TRY(c, hipMemsetAsync(c->dirty, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset dirty");
+  VHM_TRY(c, hipMemsetAsync(c->dirmask, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset direction masks");
   VHM_TRY(c, hipMemsetAsync(c->remesh_flags, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset remesh flags");
   VHM_TRY(c, hipMemsetAsync(c->hdr, 0, sizeof(DevHeader), s), "hipMemset header");
   VHM_TRY(c, hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -401,6 +404,7 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->pool, size_t(p->max_blocks) * kBlockWords * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->obs, size_t(p->max_blocks) * 16u * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->dirty, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->dirmask, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->hdr, sizeof(DevHeader));
   if (e == hipSuccess) e = hipMalloc(&c->depth_in, px * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->filtered, px * 4u);
@@ -428,7 +432,7 @@ int vhm_destroy(vhm_ctx* c) {
     hipEventDestroy(ev.b);
   }
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
-                  c->obs,          c->dirty,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
+                  c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
                   c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner};
   for (void* b : bufs)
@@ -793,3 +797,7 @@ void vhm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 vhm::VolumeView vhmCtxView(vhm_ctx* c) { return view(c); }
 hipStream_t vhmCtxStream(vhm_ctx* c) { return c->stream; }
 const vhm_params* vhmCtxParams(vhm_ctx* c) { return &c->P; }
+void vhmCtxAdvance(vhm_ctx* c, uint32_t last_frame_id, uint64_t frames) {
+  if (last_frame_id > c->frame) c->frame = last_frame_id;
+  c->frames_integrated += frames;
+}

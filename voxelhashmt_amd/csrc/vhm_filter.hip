@@ -38,15 +38,20 @@ __device__ __forceinline__ float loadDepth<uint16_t>(const uint16_t* in, size_t 
   return d > 0.0f ? d : 0.0f;
 }
 
+}  // namespace
+
+// One 32x8 output tile (tile_x, tile_y) of the filter; the whole workgroup
+// (256 lanes) takes part.
 template <typename In>
-__global__ __launch_bounds__(256) void filterKernel(const In* __restrict__ in, float* __restrict__ out, FilterArgs a) {
+__device__ void filterTile(const In* __restrict__ in, float* __restrict__ out, const FilterArgs& a, int tile_x,
+                           int tile_y) {
   __shared__ float sd[kDH][kDW];
   __shared__ float snx[kNH][kNW], sny[kNH][kNW], snz[kNH][kNW];
   __shared__ uint8_t sok[kNH][kNW];
 
   const int W = a.K.width, H = a.K.height;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
+  const int x0 = tile_x * kTX, y0 = tile_y * kTY;
   const int dx0 = x0 - (kR + 1), dy0 = y0 - (kR + 1);
 
   for (int i = tid; i < kDW * kDH; i += 256) {
@@ -92,7 +97,7 @@ __global__ __launch_bounds__(256) void filterKernel(const In* __restrict__ in, f
 
   const int tx = tid % kTX, ty = tid / kTX;
   const int u = x0 + tx, v = y0 + ty;
-  if (u >= W || v >= H) return;
+  if (u >= W || v >= H) return;  // last use of the shared tile: no barrier follows
   const int cdx = tx + kR + 1, cdy = ty + kR + 1;  // centre in the depth tile
   const int cnx = tx + kR, cny = ty + kR;          // centre in the normal tile
   const float c = sd[cdy][cdx];
@@ -130,7 +135,39 @@ __global__ __launch_bounds__(256) void filterKernel(const In* __restrict__ in, f
   out[static_cast<size_t>(v) * W + u] = result;
 }
 
+namespace {
+
+template <typename In>
+__global__ __launch_bounds__(256) void filterKernel(const In* __restrict__ in, float* __restrict__ out, FilterArgs a) {
+  filterTile<In>(in, out, a, blockIdx.x, blockIdx.y);
+}
+
+// Batched: workgroup i filters tile i of the concatenation of every scene's
+// tiles (tile_prefix[s] = first tile of scene s), frame `step` of each scene.
+__global__ __launch_bounds__(256) void filterBatchKernel(const BatchScene* __restrict__ scenes, int S,
+                                                         const uint32_t* __restrict__ tile_prefix,
+                                                         const BatchState* __restrict__ st) {
+  const uint32_t step = st->step;
+  const int s = findSegment(tile_prefix, S, blockIdx.x);
+  const BatchScene& sc = scenes[s];
+  if (step >= sc.n_frames) return;
+  const uint32_t t = blockIdx.x - tile_prefix[s];
+  const uint32_t tiles_x = (sc.fa.K.width + kTX - 1) / kTX;
+  filterTile<uint16_t>(sc.frames + static_cast<size_t>(step) * sc.frame_elems, sc.filtered, sc.fa, t % tiles_x,
+                       t / tiles_x);
+}
+
 }  // namespace
+
+uint32_t filterTiles(const Intrinsics& K) {
+  return ((K.width + kTX - 1) / kTX) * ((K.height + kTY - 1) / kTY);
+}
+
+hipError_t launchFilterBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
+                             const BatchState* st, hipStream_t s) {
+  filterBatchKernel<<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  return hipGetLastError();
+}
 
 hipError_t launchFilterF32(const float* in, float* out, const FilterArgs& a, hipStream_t s) {
   const dim3 grid((a.K.width + kTX - 1) / kTX, (a.K.height + kTY - 1) / kTY);

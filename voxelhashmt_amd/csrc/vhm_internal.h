@@ -11,7 +11,8 @@
 //                contiguous voxels, so a wavefront's 64 lanes read 256 B.
 //                147,456 B per block, as the reference's Voxel array.
 //                pool_keys[max_blocks] u64, obs[max_blocks][16] u32 observed
-//                bitmask, dirty[max_blocks] u32.
+//                bitmask, dirty[max_blocks] u32, dirmask[max_blocks] u32
+//                (directions with observations).
 //   per frame    active[2^k] u32 hash slots touched this frame,
 //                depth staging, filtered depth (W x H f32).
 #pragma once
@@ -59,6 +60,7 @@ struct VolumeView {
   uint32_t* pool;  // float fields are stored as their bit patterns
   uint32_t* obs;
   uint32_t* dirty;
+  uint32_t* dirmask;  // per block: bit d set once any voxel has an observed slot in direction d
   uint32_t max_blocks;
   uint32_t* active;
   DevHeader* hdr;
@@ -122,6 +124,59 @@ hipError_t launchCompactFlags(const uint32_t* flags, const uint32_t* pos, uint32
                               uint32_t* flags_out, hipStream_t s);
 hipError_t launchListKeys(const VolumeView& V, const uint32_t* list, uint32_t n, int32_t* out, hipStream_t s);
 
+// ---- batched multi-scene step (vhm_batch.hip) ----
+struct BatchState {
+  uint32_t step;  // frame index every scene integrates in the current step
+  uint32_t pad[3];
+};
+
+struct BatchScene {
+  VolumeView V;
+  FilterArgs fa;              // intrinsics, scale and filter constants
+  AllocArgs aa;               // pose and frame id are taken per step
+  IntegrateArgs ia;           // poses are taken per step
+  const uint16_t* frames;     // [n_frames][H][W] uint16, device
+  size_t frame_elems;         // H * W
+  uint32_t n_frames;
+  float* filtered;            // H * W
+  const Pose34* poses;        // [n_frames] T_wc
+  const Pose34* poses_inv;    // [n_frames] T_cw
+  // incremental meshing of the step
+  uint32_t* remesh_flags;     // [max_blocks]
+  uint32_t* remesh_list;      // [max_blocks]
+  uint32_t* mesh_counts;      // [6 * max_blocks]
+  uint32_t* mesh_offsets;     // [6 * max_blocks]
+  float* tris;                // [tri_capacity][9]
+  uint32_t tri_capacity;
+  uint32_t* out;              // [4]: blocks re-meshed, triangles (total, may exceed capacity)
+  MeshArgs ma;
+};
+
+// Index s with prefix[s] <= i < prefix[s + 1] (prefix has S + 1 entries).
+VHM_HD int findSegment(const uint32_t* prefix, int S, uint32_t i) {
+  int lo = 0, hi = S - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+uint32_t filterTiles(const Intrinsics& K);
+uint32_t allocTiles(const Intrinsics& K, int stride);
+hipError_t launchFilterBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
+                             const BatchState* st, hipStream_t s);
+hipError_t launchBatchFrameBegin(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s);
+hipError_t launchAllocateBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
+                               const BatchState* st, hipStream_t s);
+hipError_t launchActivePrefix(const BatchScene* scenes, int S, const BatchState* st, uint32_t* prefix,
+                              hipStream_t s);
+hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t* prefix, const BatchState* st,
+                                int grid, hipStream_t s);
+hipError_t launchStepIncrement(BatchState* st, hipStream_t s);
+// Incremental meshing of every scene's dirty blocks (prefix: S + 1 scratch).
+hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, hipStream_t s);
+
 struct SynthDeviceScene {
   float room_min[3], room_max[3];
   int num_primitives;
@@ -172,3 +227,6 @@ struct vhm_ctx;
 vhm::VolumeView vhmCtxView(vhm_ctx* c);
 hipStream_t vhmCtxStream(vhm_ctx* c);
 const vhm_params* vhmCtxParams(vhm_ctx* c);
+// Records frames integrated outside the context's own calls (batched steps):
+// later frame ids continue after last_frame_id.
+void vhmCtxAdvance(vhm_ctx* c, uint32_t last_frame_id, uint64_t frames);

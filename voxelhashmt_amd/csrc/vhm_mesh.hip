@@ -35,6 +35,7 @@ struct MeshShared {
   float nx[kSlots][kRegion], ny[kSlots][kRegion], nz[kSlots][kRegion], po[kSlots][kRegion];
   uint8_t bits[kRegion];  // bit k: slot k observed; bit 3: voxel's block allocated
   uint32_t src[8];        // pool index of self + 7 positive neighbours (kInvalid if absent)
+  uint32_t any_dir;       // OR over the 8 source blocks' direction masks
   float cand[kMaxCand][256];
   uint32_t scan[256];
   uint32_t total;
@@ -54,14 +55,21 @@ __device__ uint32_t lookupPool(const VolumeView& V, int x, int y, int z) {
   return kInvalid;
 }
 
-__device__ void stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, uint32_t self, int dir) {
+// Stages the 9x9x9 corner region of direction `dir`.  Returns false (and
+// stages nothing) when none of the 8 source blocks has an observed slot in
+// that direction: then every corner is invalid and no cube yields a triangle.
+__device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, uint32_t self, int dir) {
   const int tid = threadIdx.x;
+  if (tid == 0) S.any_dir = 0u;
+  __syncthreads();
   if (tid < 8) {
     const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
     uint32_t p = tid == 0 ? self : lookupPool(V, keyX(key) + ox, keyY(key) + oy, keyZ(key) + oz);
     S.src[tid] = (p < V.max_blocks) ? p : kInvalid;
+    if (p < V.max_blocks && (V.dirmask[p] & (1u << dir))) atomicOr(&S.any_dir, 1u);
   }
   __syncthreads();
+  if (S.any_dir == 0u) return false;
   for (int r = tid; r < kRegion; r += 256) {
     const int x = r % kR, y = (r / kR) % kR, z = r / (kR * kR);
     const int sb = (x == 8 ? 1 : 0) | (y == 8 ? 2 : 0) | (z == 8 ? 4 : 0);
@@ -87,6 +95,7 @@ __device__ void stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
     S.bits[r] = b;
   }
   __syncthreads();
+  return true;
 }
 
 __device__ __forceinline__ float planeSD(const MeshShared& S, int k, int r, F3 p) {
@@ -275,18 +284,52 @@ __device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
   return incl - x;
 }
 
+// Triangle count of (block b, direction dir); the workgroup takes part.
+__device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a) {
+  const uint64_t key = V.pool_keys[b];
+  if (!stageRegion(S, V, key, b, dir)) {
+    __syncthreads();  // S.any_dir is rewritten by the next item
+    return 0u;
+  }
+  uint32_t c = processCube(S, key, threadIdx.x, a, nullptr) + processCube(S, key, threadIdx.x + 256, a, nullptr);
+  blockExclusiveScan(S, c);
+  const uint32_t total = S.total;
+  __syncthreads();  // S.total / S.scan are reused by the next item
+  return total;
+}
+
+// Emits the triangles of (block b, direction dir) at tris[base ..] in cube
+// order; tri_owner (optional) receives owner for each triangle.
+__device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a, uint32_t base,
+                         float* tris, uint32_t* tri_owner, uint32_t owner, uint32_t capacity) {
+  const uint64_t key = V.pool_keys[b];
+  if (!stageRegion(S, V, key, b, dir)) {
+    __syncthreads();
+    return;
+  }
+  // cubes 0..255 precede 256..511 in the reference's lz, ly, lx loop order
+  for (int half = 0; half < 2; ++half) {
+    const int cube = threadIdx.x + 256 * half;
+    const uint32_t c = processCube(S, key, cube, a, nullptr);
+    const uint32_t start = base + blockExclusiveScan(S, c);
+    if (c > 0 && start + c <= capacity) {
+      processCube(S, key, cube, a, tris + 9ull * start);
+      if (tri_owner)
+        for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = owner;
+    }
+    base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
+  }
+  __syncthreads();
+}
+
 // Grid (n, 6): workgroup (i, dir) meshes block list[i] (or pool index i
 // when list is null) in direction dir.
 __global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
                                                        MeshArgs a, uint32_t* __restrict__ counts) {
   __shared__ MeshShared S;
   const uint32_t i = blockIdx.x, dir = blockIdx.y;
-  const uint32_t b = list ? list[i] : i;
-  const uint64_t key = V.pool_keys[b];
-  stageRegion(S, V, key, b, static_cast<int>(dir));
-  uint32_t c = processCube(S, key, threadIdx.x, a, nullptr) + processCube(S, key, threadIdx.x + 256, a, nullptr);
-  blockExclusiveScan(S, c);
-  if (threadIdx.x == 0) counts[static_cast<size_t>(dir) * n + i] = S.total;
+  const uint32_t total = countItem(S, V, list ? list[i] : i, static_cast<int>(dir), a);
+  if (threadIdx.x == 0) counts[static_cast<size_t>(dir) * n + i] = total;
 }
 
 // Writes the triangles of workgroup (i, dir) at offsets[dir * n + i] in cube
@@ -297,22 +340,8 @@ __global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, const uint32
                                                       uint32_t capacity) {
   __shared__ MeshShared S;
   const uint32_t i = blockIdx.x, dir = blockIdx.y;
-  const uint32_t b = list ? list[i] : i;
-  const uint64_t key = V.pool_keys[b];
-  uint32_t base = offsets[static_cast<size_t>(dir) * n + i];
-  stageRegion(S, V, key, b, static_cast<int>(dir));
-  // cubes 0..255 precede 256..511 in the reference's lz, ly, lx loop order
-  for (int half = 0; half < 2; ++half) {
-    const int cube = threadIdx.x + 256 * half;
-    const uint32_t c = processCube(S, key, cube, a, nullptr);
-    const uint32_t start = base + blockExclusiveScan(S, c);
-    if (c > 0 && start + c <= capacity) {
-      processCube(S, key, cube, a, tris + 9ull * start);
-      if (tri_owner)
-        for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = i;
-    }
-    base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
-  }
+  emitItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, offsets[static_cast<size_t>(dir) * n + i], tris,
+           tri_owner, i, capacity);
 }
 
 // Blocks whose cubes read a dirty block: the dirty block itself and its
@@ -419,6 +448,118 @@ __global__ __launch_bounds__(256) void scanAddKernel(uint32_t* __restrict__ out,
     if (base + i < n) out[base + i] += add;
 }
 
+// ---- batched incremental meshing (all scenes of a batch step) ----
+
+__device__ void markRemeshScene(const VolumeView& V, uint32_t* flags, uint32_t start, uint32_t stride) {
+  const uint32_t nb = V.hdr->num_blocks < V.max_blocks ? V.hdr->num_blocks : V.max_blocks;
+  for (uint32_t p = start; p < nb; p += stride) {
+    if (V.dirty[p] == 0u) continue;
+    V.dirty[p] = 0u;
+    const uint64_t key = V.pool_keys[p];
+    for (int o = 0; o < 8; ++o) {
+      const uint32_t q = o == 0 ? p : lookupPool(V, keyX(key) - (o & 1), keyY(key) - ((o >> 1) & 1),
+                                                 keyZ(key) - (o >> 2));
+      if (q < V.max_blocks) flags[q] = 1u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void markRemeshBatchKernel(const BatchScene* __restrict__ scenes) {
+  const BatchScene& sc = scenes[blockIdx.y];
+  markRemeshScene(sc.V, sc.remesh_flags, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
+}
+
+// Sequential-chunk exclusive scan by one workgroup: out[i] = sum(in[0..i)),
+// returns the total (all lanes).  With `list`, instead compacts the indices of
+// non-zero entries of `in` into list (in order) and clears them.
+__device__ uint32_t workgroupScan(uint32_t* in, uint32_t n, uint32_t* out, uint32_t* list) {
+  __shared__ uint32_t s[256];
+  __shared__ uint32_t carry;
+  const int tid = threadIdx.x;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += 256) {
+    const uint32_t i = base + tid;
+    uint32_t x = i < n ? in[i] : 0u;
+    if (list) x = x ? 1u : 0u;
+    s[tid] = x;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const uint32_t y = tid >= off ? s[tid - off] : 0u;
+      __syncthreads();
+      s[tid] += y;
+      __syncthreads();
+    }
+    const uint32_t excl = carry + s[tid] - x;
+    if (i < n) {
+      if (list) {
+        if (x) {
+          list[excl] = i;
+          in[i] = 0u;
+        }
+      } else {
+        out[i] = excl;
+      }
+    }
+    __syncthreads();
+    if (tid == 255) carry += s[255];
+    __syncthreads();
+  }
+  return carry;
+}
+
+__global__ __launch_bounds__(256) void compactRemeshBatchKernel(const BatchScene* __restrict__ scenes) {
+  const BatchScene& sc = scenes[blockIdx.x];
+  const VolumeView& V = sc.V;
+  const uint32_t nb = V.hdr->num_blocks < V.max_blocks ? V.hdr->num_blocks : V.max_blocks;
+  const uint32_t m = workgroupScan(sc.remesh_flags, nb, nullptr, sc.remesh_list);
+  if (threadIdx.x == 0) sc.out[0] = m;
+}
+
+__global__ void meshPrefixKernel(const BatchScene* __restrict__ scenes, int S, uint32_t* __restrict__ prefix) {
+  if (threadIdx.x != 0) return;
+  uint32_t acc = 0;
+  for (int s = 0; s < S; ++s) {
+    prefix[s] = acc;
+    acc += 6u * scenes[s].out[0];
+  }
+  prefix[S] = acc;
+}
+
+__global__ __launch_bounds__(256) void meshCountBatchKernel(const BatchScene* __restrict__ scenes, int S,
+                                                            const uint32_t* __restrict__ prefix) {
+  __shared__ MeshShared M;
+  const uint32_t total = prefix[S];
+  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
+    const int s = findSegment(prefix, S, i);
+    const BatchScene& sc = scenes[s];
+    const uint32_t m = sc.out[0], j = i - prefix[s];
+    const uint32_t dir = j / m, idx = j % m;
+    const uint32_t c = countItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma);
+    if (threadIdx.x == 0) sc.mesh_counts[dir * m + idx] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void meshScanBatchKernel(const BatchScene* __restrict__ scenes) {
+  const BatchScene& sc = scenes[blockIdx.x];
+  const uint32_t t = workgroupScan(sc.mesh_counts, 6u * sc.out[0], sc.mesh_offsets, nullptr);
+  if (threadIdx.x == 0) sc.out[1] = t;
+}
+
+__global__ __launch_bounds__(256) void meshEmitBatchKernel(const BatchScene* __restrict__ scenes, int S,
+                                                           const uint32_t* __restrict__ prefix) {
+  __shared__ MeshShared M;
+  const uint32_t total = prefix[S];
+  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
+    const int s = findSegment(prefix, S, i);
+    const BatchScene& sc = scenes[s];
+    const uint32_t m = sc.out[0], j = i - prefix[s];
+    const uint32_t dir = j / m, idx = j % m;
+    emitItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, sc.mesh_offsets[dir * m + idx], sc.tris,
+             nullptr, 0u, sc.tri_capacity);
+  }
+}
+
 }  // namespace
 
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
@@ -462,6 +603,16 @@ hipError_t launchCompactFlags(const uint32_t* flags, const uint32_t* pos, uint32
                               uint32_t* flags_out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   compactFlagsKernel<<<gridFor(n), 256, 0, s>>>(flags, pos, n, list, flags_out);
+  return hipGetLastError();
+}
+
+hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, hipStream_t s) {
+  markRemeshBatchKernel<<<dim3(64, S), 256, 0, s>>>(scenes);
+  compactRemeshBatchKernel<<<S, 256, 0, s>>>(scenes);
+  meshPrefixKernel<<<1, 64, 0, s>>>(scenes, S, prefix);
+  meshCountBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix);
+  meshScanBatchKernel<<<S, 256, 0, s>>>(scenes);
+  meshEmitBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix);
   return hipGetLastError();
 }
 
