@@ -76,22 +76,144 @@ def cpu_baseline(w, frame0_u16, gpu_updates_frame0: int) -> dict | None:
             "seconds": dt}
 
 
+def calibrate_copy(nbytes: int = 1 << 30, reps: int = 5):
+    """A plain device-to-device copy of known size: read nbytes, write nbytes
+    per launch.  Its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc give the
+    counters' scale (KiB per byte moved)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": nbytes, "copy_bytes_written_per_launch": nbytes,
+                      "launches": reps}))
+
+
+def bench_batched(args):
+    """Batched plain method: 48 seeded rooms (target block counts log-uniform
+    in 1k-60k, resolutions from {320x240, 640x480, 1280x720}), 60 frames
+    each, voxel 1 cm; one step = one frame of every scene through filter,
+    allocation, integration and incremental marching cubes, replayed as one
+    captured hipGraph.  Grid sizes come from the autotuner; an untimed full
+    pass precedes the timed pass (the sequences are rewound in between)."""
+    import numpy as np
+    import torch
+    import voxelhashmt_amd as vhm
+    from voxelhashmt_amd import synth
+
+    ws = synth.batched_scenes()
+    n_frames = len(ws[0].poses)
+    K = min(args.steps, n_frames)
+    ok, msg = vhm.device_available()
+    if not ok:
+        raise SystemExit(f"no GPU: {msg}")
+    log(f"[bench-batched] {msg}; {len(ws)} scenes x {n_frames} frames, timed steps {K}")
+    dev = torch.device("cuda:0")
+    frames = []
+    for w in ws:
+        t = torch.empty((n_frames, w.K.height, w.K.width), dtype=torch.int16, device=dev)
+        synth.render(w.scene, w.K, w.poses, w.noise, 0, t)
+        frames.append(t)
+    torch.cuda.synchronize()
+    log("[bench-batched] rendered all frames into HBM")
+    params = [vhm.default_params(w.voxel_size, w.truncation, filter_sigma_spatial=w.filter_sigma_spatial,
+                                 filter_sigma_range=w.filter_sigma_range, hash_log2_buckets=w.hash_log2_buckets,
+                                 max_blocks=w.max_blocks, max_width=w.K.width, max_height=w.K.height)
+              for w in ws]
+    caps = [200 * w.target_blocks + 65536 for w in ws]
+    b = vhm.Batch(params)
+    for i, w in enumerate(ws):
+        b.set_sequence(i, frames[i], w.K, w.poses, tri_capacity=caps[i])
+    b.prepare(use_graph=True)
+    gi, gm = b.autotune((2, 4, 8, 16))
+    log(f"[bench-batched] autotuned grids: integrate {gi}, mesh {gm}")
+    b.rewind()
+    if args.warmup:
+        b.step(n_frames)
+        b.synchronize()
+        b.rewind()
+        log("[bench-batched] warm-up pass done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(K)
+    b.synchronize()
+    elapsed = time.perf_counter() - t0
+    updates = sum(v.stats()["updates"] for v in b.volumes)
+    blocks = np.array([v.block_count() for v in b.volumes])
+    log(f"[bench-batched] timed {K} steps in {elapsed * 1e3:.1f} ms; updates={updates}; "
+        f"blocks min/median/max {blocks.min()}/{int(np.median(blocks))}/{blocks.max()}")
+
+    # stage breakdown: the same steps again with direct launches and events
+    b.rewind()
+    tot = {"front_ms": 0.0, "integrate_ms": 0.0, "mesh_ms": 0.0, "updates": 0, "blocks_remeshed": 0}
+    overflow = 0
+    for _ in range(K):
+        r = b.timed_step()
+        for k in tot:
+            tot[k] += r[k]
+        for i in range(len(ws)):
+            if b.mesh_output(i, fetch=False)[1] > caps[i]:
+                overflow += 1
+    achieved = BYTES_PER_UPDATE * tot["updates"] / max(tot["integrate_ms"] * 1e-3, 1e-12) / 1e9
+    out = {
+        "metric": "voxel updates/s (% HBM peak)",
+        "value": updates / elapsed,
+        "unit": "voxel updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": n_frames if args.warmup else 0,
+        "ms_per_step": elapsed * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (48 seeded analytic rooms, GPU-rendered uint16 depth resident in HBM)",
+        "config": {"workload": "batched_48_rooms", "scenes": len(ws), "frames_per_scene": n_frames,
+                   "resolutions": sorted({f"{w.K.width}x{w.K.height}" for w in ws}), "voxel_size_m": 0.01,
+                   "truncation_m": 0.04, "blocks_min": int(blocks.min()), "blocks_median": int(np.median(blocks)),
+                   "blocks_max": int(blocks.max()), "blocks_total": int(blocks.sum()),
+                   "target_blocks_min": min(w.target_blocks for w in ws),
+                   "target_blocks_max": max(w.target_blocks for w in ws),
+                   "integrate_grid": gi, "mesh_grid": gm, "hipgraph": True},
+        "frames_per_s": len(ws) * K / elapsed,
+        "stage_ms_direct_launches": {k: round(v, 3) for k, v in tot.items() if k.endswith("_ms")},
+        "blocks_remeshed": tot["blocks_remeshed"],
+        "mesh_capacity_overflows": overflow,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "integrateBatchKernel",
+                     "avg_launch_ms": round(tot["integrate_ms"] / K, 4),
+                     "updates_per_launch": tot["updates"] / K},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=480)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="plain_scale", choices=["plain_scale", "plain_small"])
+    ap.add_argument("--workload", default="plain_scale", choices=["plain_scale", "plain_small", "batched"])
     ap.add_argument("--max-blocks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--mesh-every", type=int, default=10, help="incremental MC period in frames (0 = off)")
     ap.add_argument("--mesh-tri-capacity", type=int, default=8_000_000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the sharded design is described in "
                                    "DESIGN.md (--gpus must be 1)"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.workload == "batched":
+        bench_batched(args)
+        return
 
     import numpy as np
     import torch

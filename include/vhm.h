@@ -247,10 +247,17 @@ int vhm_batch_prepare(vhm_batch* b, int integrate_grid, int mesh_grid, int use_g
 int vhm_batch_step(vhm_batch* b, int n_steps);
 int vhm_batch_synchronize(vhm_batch* b);
 int vhm_batch_steps_done(const vhm_batch* b);
+/* Clears every volume and restarts all sequences at their first frame
+ * (reservations, captured graph and grid sizes are kept). */
+int vhm_batch_rewind(vhm_batch* b);
 /* Last step's incremental mesh of a scene: blocks re-meshed, triangle total
  * (may exceed tri_capacity), and up to capacity triangles (9 floats, host). */
 int vhm_batch_mesh_output(vhm_batch* b, int scene, uint32_t* blocks, uint32_t* triangles, float* tris_host,
                           size_t capacity);
+/* One step with direct launches, timed per stage: ms_out[0] filter +
+ * allocation, [1] integration, [2] incremental meshing; the step's voxel
+ * updates and re-meshed blocks over all scenes.  Synchronous. */
+int vhm_batch_timed_step(vhm_batch* b, float ms_out[3], uint64_t* updates, uint64_t* remeshed);
 /* Runs one step per candidate (multiples of the CU count) with direct
  * launches and keeps, per persistent kernel, the grid with the least time
  * per unit of work (a captured graph is re-captured with them). */

@@ -126,6 +126,9 @@ _SIGS = {
     "vhm_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "vhm_batch_synchronize": (C.c_int, [C.c_void_p]),
     "vhm_batch_steps_done": (C.c_int, [C.c_void_p]),
+    "vhm_batch_rewind": (C.c_int, [C.c_void_p]),
+    "vhm_batch_timed_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]),
     "vhm_batch_mesh_output": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                         C.c_void_p, C.c_size_t]),
     "vhm_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
@@ -146,6 +149,14 @@ def lib():
         if not Path(path).exists():
             raise RuntimeError(f"libvhm.so not built ({path}); run `python -m voxelhashmt_amd.build` "
                                "(there is no CPU fallback)")
+        # One HSA runtime per process: torch's ROCm wheel brings its own HIP
+        # and HSA runtimes, and an HSA runtime loaded before it (ours, via
+        # libvhm.so) makes torch find no GPU.  Loading torch first lets
+        # libvhm.so's HIP runtime bind to the HSA runtime torch loaded.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(path)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
@@ -504,6 +515,9 @@ class Batch:
     def synchronize(self):
         self._check(lib().vhm_batch_synchronize(self._h))
 
+    def rewind(self):
+        self._check(lib().vhm_batch_rewind(self._h))
+
     @property
     def steps_done(self) -> int:
         return lib().vhm_batch_steps_done(self._h)
@@ -519,6 +533,14 @@ class Batch:
             if tris.value:
                 self._check(lib().vhm_batch_mesh_output(self._h, scene, None, None, out.ctypes.data, tris.value))
         return blocks.value, tris.value, out
+
+    def timed_step(self) -> dict:
+        """One step with direct launches, timed per stage (synchronous)."""
+        ms = (C.c_float * 3)()
+        u, r = C.c_uint64(), C.c_uint64()
+        self._check(lib().vhm_batch_timed_step(self._h, ms, C.byref(u), C.byref(r)))
+        return {"front_ms": ms[0], "integrate_ms": ms[1], "mesh_ms": ms[2], "updates": u.value,
+                "blocks_remeshed": r.value}
 
     def autotune(self, candidates=(2, 4, 8, 16)) -> tuple[int, int]:
         c = (C.c_int * len(candidates))(*candidates)

@@ -334,11 +334,73 @@ int vhm_batch_mesh_output(vhm_batch* b, int scene, uint32_t* blocks, uint32_t* t
   return VHM_OK;
 }
 
-// Times one step per candidate grid size (direct launches with events around
-// the two persistent kernels' stages) and keeps the fastest per stage,
-// normalised by work (updates for integration, re-meshed blocks for meshing).
-// Consumes len(candidates) steps of the sequence (results do not depend on
-// grid sizes).  candidates: multipliers of the CU count.
+}  // extern "C"
+
+namespace {
+
+// One step with direct launches and events: ms[0] frame begin + filter +
+// allocation + prefix, ms[1] integration, ms[2] incremental meshing;
+// *updates / *remeshed: the step's voxel updates and re-meshed blocks over
+// all scenes.  Synchronous.
+int timedStep(vhm_batch* b, int gi, int gm, float ms[3], uint64_t* updates, uint64_t* remeshed) {
+  const int S = b->S;
+  uint64_t u0 = 0;
+  for (vhm_ctx* v : b->vols) {
+    uint64_t u = 0;
+    const int rc = vhm_stats(v, &u, nullptr, nullptr);
+    if (rc) return bfail(b, rc, vhm_last_error(v));
+    u0 += u;
+  }
+  hipEvent_t ev[4] = {};
+  for (auto& e : ev) B_TRY(b, hipEventCreate(&e), "hipEventCreate");
+  hipError_t e = hipEventRecord(ev[0], b->stream);
+  if (e == hipSuccess) e = launchBatchFrameBegin(b->scenes, S, b->state, b->stream);
+  if (e == hipSuccess) e = launchFilterBatch(b->scenes, S, b->filter_prefix, b->filter_total, b->state, b->stream);
+  if (e == hipSuccess) e = launchAllocateBatch(b->scenes, S, b->alloc_prefix, b->alloc_total, b->state, b->stream);
+  if (e == hipSuccess) e = launchActivePrefix(b->scenes, S, b->state, b->active_prefix, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
+  if (e == hipSuccess) e = launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, gi, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);
+  if (e == hipSuccess) e = launchMeshBatch(b->scenes, S, b->mesh_prefix, gm, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev[3], b->stream);
+  if (e == hipSuccess) e = launchStepIncrement(b->state, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e == hipSuccess) {
+    for (int i = 0; i < 3; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  B_TRY(b, e, "batch timed step");
+  noteStep(b);
+  uint64_t u1 = 0, rm = 0;
+  for (int s = 0; s < S; ++s) {
+    uint64_t u = 0;
+    const int rc = vhm_stats(b->vols[s], &u, nullptr, nullptr);
+    if (rc) return bfail(b, rc, vhm_last_error(b->vols[s]));
+    u1 += u;
+    uint32_t o[4];
+    B_TRY(b, hipMemcpy(o, b->host_scenes[s].out, 16, hipMemcpyDeviceToHost), "hipMemcpy mesh outputs");
+    rm += o[0];
+  }
+  if (updates) *updates = u1 - u0;
+  if (remeshed) *remeshed = rm;
+  return VHM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// One step with direct launches and per-stage timing (see timedStep), with
+// the current grid sizes.
+int vhm_batch_timed_step(vhm_batch* b, float ms_out[3], uint64_t* updates, uint64_t* remeshed) {
+  if (!b || !b->prepared || !ms_out) return VHM_ERR_INVALID_ARGUMENT;
+  return timedStep(b, b->integrate_grid, b->mesh_grid, ms_out, updates, remeshed);
+}
+
+// Times one step per candidate grid size and keeps the fastest per stage,
+// normalised by work (updates for integration, re-meshed blocks for
+// meshing).  Consumes n_candidates steps of the sequences (results do not
+// depend on grid sizes).  candidates: multipliers of the CU count.
 int vhm_batch_autotune(vhm_batch* b, const int* candidates, int n_candidates, int* integrate_grid_out,
                        int* mesh_grid_out) {
   if (!b || !b->prepared || !candidates || n_candidates <= 0) return VHM_ERR_INVALID_ARGUMENT;
@@ -346,46 +408,12 @@ int vhm_batch_autotune(vhm_batch* b, const int* candidates, int n_candidates, in
   int bi = b->integrate_grid, bm = b->mesh_grid;
   for (int c = 0; c < n_candidates; ++c) {
     const int g = candidates[c] * b->num_cus;
-    if (g <= 0) continue;
-    uint64_t upd0 = 0, upd1 = 0;
-    for (vhm_ctx* v : b->vols) {
-      uint64_t u = 0;
-      (void)vhm_stats(v, &u, nullptr, nullptr);
-      upd0 += u;
-    }
-    hipEvent_t e0, e1, e2;
-    B_TRY(b, hipEventCreate(&e0), "hipEventCreate");
-    B_TRY(b, hipEventCreate(&e1), "hipEventCreate");
-    B_TRY(b, hipEventCreate(&e2), "hipEventCreate");
-    const int S = b->S;
-    B_TRY(b, launchBatchFrameBegin(b->scenes, S, b->state, b->stream), "launch");
-    B_TRY(b, launchFilterBatch(b->scenes, S, b->filter_prefix, b->filter_total, b->state, b->stream), "launch");
-    B_TRY(b, launchAllocateBatch(b->scenes, S, b->alloc_prefix, b->alloc_total, b->state, b->stream), "launch");
-    B_TRY(b, launchActivePrefix(b->scenes, S, b->state, b->active_prefix, b->stream), "launch");
-    B_TRY(b, hipEventRecord(e0, b->stream), "hipEventRecord");
-    B_TRY(b, launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, g, b->stream), "launch");
-    B_TRY(b, hipEventRecord(e1, b->stream), "hipEventRecord");
-    B_TRY(b, launchMeshBatch(b->scenes, S, b->mesh_prefix, g, b->stream), "launch");
-    B_TRY(b, hipEventRecord(e2, b->stream), "hipEventRecord");
-    B_TRY(b, launchStepIncrement(b->state, b->stream), "launch");
-    B_TRY(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-    noteStep(b);
-    float ti = 0.0f, tm = 0.0f;
-    (void)hipEventElapsedTime(&ti, e0, e1);
-    (void)hipEventElapsedTime(&tm, e1, e2);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
-    uint64_t remeshed = 0;
-    for (int s = 0; s < S; ++s) {
-      uint64_t u = 0;
-      (void)vhm_stats(b->vols[s], &u, nullptr, nullptr);
-      upd1 += u;
-      uint32_t o[4];
-      B_TRY(b, hipMemcpy(o, b->host_scenes[s].out, 16, hipMemcpyDeviceToHost), "hipMemcpy");
-      remeshed += o[0];
-    }
-    const double per_i = ti / static_cast<double>(upd1 - upd0 + 1), per_m = tm / static_cast<double>(remeshed + 1);
+    if (g <= 0 || g > (1 << 20)) return bfail(b, VHM_ERR_INVALID_ARGUMENT, "batch autotune failed: bad candidate");
+    float ms[3] = {0.0f, 0.0f, 0.0f};
+    uint64_t upd = 0, rm = 0;
+    const int rc = timedStep(b, g, g, ms, &upd, &rm);
+    if (rc) return rc;
+    const double per_i = ms[1] / static_cast<double>(upd + 1), per_m = ms[2] / static_cast<double>(rm + 1);
     if (per_i < best_i) {
       best_i = per_i;
       bi = g;
@@ -400,6 +428,26 @@ int vhm_batch_autotune(vhm_batch* b, const int* candidates, int n_candidates, in
   if (integrate_grid_out) *integrate_grid_out = bi;
   if (mesh_grid_out) *mesh_grid_out = bm;
   if (b->exec) return vhm_batch_prepare(b, 0, 0, 1);  // re-capture with the chosen grids
+  return VHM_OK;
+}
+
+// Clears every volume and restarts the sequences at frame 0, keeping the
+// reservations, the captured graph and the tuned grid sizes.
+int vhm_batch_rewind(vhm_batch* b) {
+  if (!b) return VHM_ERR_INVALID_ARGUMENT;
+  B_TRY(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+  for (int s = 0; s < b->S; ++s) {
+    const int rc = vhm_reset(b->vols[s]);
+    if (rc != VHM_OK) return bfail(b, rc, std::string("scene ") + std::to_string(s) + ": " + vhm_last_error(b->vols[s]));
+    const BatchScene& sc = b->host_scenes[s];
+    if (!b->ready[s]) continue;
+    B_TRY(b, hipMemsetAsync(sc.remesh_flags, 0, 4 * size_t(vhmCtxParams(b->vols[s])->max_blocks), b->stream),
+          "hipMemset remesh flags");
+    B_TRY(b, hipMemsetAsync(sc.out, 0, 16, b->stream), "hipMemset mesh outputs");
+  }
+  B_TRY(b, hipMemsetAsync(b->state, 0, sizeof(BatchState), b->stream), "hipMemset batch state");
+  B_TRY(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+  b->steps_done = 0;
   return VHM_OK;
 }
 

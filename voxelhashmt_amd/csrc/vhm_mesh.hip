@@ -31,9 +31,15 @@ constexpr int kR = 9;  // corner region edge (8 voxels + 1 from the + neighbours
 constexpr int kRegion = kR * kR * kR;
 constexpr int kMaxCand = 24;
 
+// Per-voxel bits of the staged region.
+constexpr uint8_t kBitAllocated = 8;   // the voxel's block is allocated
+constexpr uint8_t kBitBelow = 16;      // some observed slot's corner value may be < iso
+constexpr uint8_t kBitAbove = 32;      // some observed slot's corner value may be >= iso
+
 struct MeshShared {
   float nx[kSlots][kRegion], ny[kSlots][kRegion], nz[kSlots][kRegion], po[kSlots][kRegion];
-  uint8_t bits[kRegion];  // bit k: slot k observed; bit 3: voxel's block allocated
+  float sdv[kSlots][kRegion];  // plane signed distance of slot k at the voxel's own centre
+  uint8_t bits[kRegion];       // bit k: slot k observed; kBit* above
   uint32_t src[8];        // pool index of self + 7 positive neighbours (kInvalid if absent)
   uint32_t any_dir;       // OR over the 8 source blocks' direction masks
   float cand[kMaxCand][256];
@@ -58,7 +64,17 @@ __device__ uint32_t lookupPool(const VolumeView& V, int x, int y, int z) {
 // Stages the 9x9x9 corner region of direction `dir`.  Returns false (and
 // stages nothing) when none of the 8 source blocks has an observed slot in
 // that direction: then every corner is invalid and no cube yields a triangle.
-__device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, uint32_t self, int dir) {
+//
+// Besides the unpacked slots, each voxel gets its slots' plane distances at
+// its own centre -- exactly the values a cube uses for that corner
+// (voxel_hash_tsdf.h:925) -- and two bits saying whether any of them may lie
+// below / at-or-above iso.  A cube's corner values, including the ones
+// completeCubeValuesWithFallback fills in (means of valid values), are
+// drawn from these, so a cube whose 8 corners never have both bits can
+// produce no triangle.  The margin (1e-6 |iso| + 1e-30) exceeds the rounding
+// of those means, and NaN sets both bits.
+__device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, uint32_t self, int dir,
+                            const MeshArgs& a) {
   const int tid = threadIdx.x;
   if (tid == 0) S.any_dir = 0u;
   __syncthreads();
@@ -70,25 +86,35 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
   }
   __syncthreads();
   if (S.any_dir == 0u) return false;
+  const float margin = fabsf(a.iso) * 1e-6f + 1e-30f;
+  const float below = a.iso + margin, above = a.iso - margin;
+  const int bx = keyX(key) * kBlockSide, by = keyY(key) * kBlockSide, bz = keyZ(key) * kBlockSide;
   for (int r = tid; r < kRegion; r += 256) {
     const int x = r % kR, y = (r / kR) % kR, z = r / (kR * kR);
     const int sb = (x == 8 ? 1 : 0) | (y == 8 ? 2 : 0) | (z == 8 ? 4 : 0);
     const uint32_t p = S.src[sb];
     uint8_t b = 0;
     if (p != kInvalid) {
-      b = 8;
+      b = kBitAllocated;
       const int vi = (x & 7) + kBlockSide * ((y & 7) + kBlockSide * (z & 7));
       const uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
+      const F3 centre{(static_cast<float>(bx + x) + 0.5f) * a.voxel, (static_cast<float>(by + y) + 0.5f) * a.voxel,
+                      (static_cast<float>(bz + z) + 0.5f) * a.voxel};
 #pragma unroll
       for (int k = 0; k < kSlots; ++k) {
         const float w = __uint_as_float(blk[fieldOffset(dir, 0, k) + vi]);
         if (w > 0.0f) {
           b |= static_cast<uint8_t>(1u << k);
           const F3 n = unpackNormalOct(blk[fieldOffset(dir, 3, k) + vi]);
+          const float po = __uint_as_float(blk[fieldOffset(dir, 2, k) + vi]);
+          const float v = dot3(n, centre) - po;
           S.nx[k][r] = n.x;
           S.ny[k][r] = n.y;
           S.nz[k][r] = n.z;
-          S.po[k][r] = __uint_as_float(blk[fieldOffset(dir, 2, k) + vi]);
+          S.po[k][r] = po;
+          S.sdv[k][r] = v;
+          if (!(v >= below)) b |= kBitBelow;
+          if (!(v <= above)) b |= kBitAbove;
         }
       }
     }
@@ -146,32 +172,56 @@ __device__ __forceinline__ int cubeCase(const float val[8], float iso) {
   return ci;
 }
 
-// polygoniseCube (marching_cubes.h:48-94): writes the case's triangles.
-__device__ void emitCube(const F3 pos[8], const float val[8], float iso, float* out) {
-  const int ci = cubeCase(val, iso);
-  const int mask = c_mc.edge_mask[ci];
-  F3 ev[12];
+// Corner c of the cube at offset ((kCornerX >> c) & 1, (kCornerY >> c) & 1,
+// (kCornerZ >> c) & 1), and edge e joining corners (kEdgeC0 >> 4e) & 15 and
+// (kEdgeC1 >> 4e) & 15: bit-packed forms of kMcCornerOffset / kMcEdgeCorners
+// for lane-varying indices.
+constexpr uint32_t packCornerAxis(int axis) {
+  uint32_t m = 0;
+  for (int c = 0; c < 8; ++c) m |= static_cast<uint32_t>(kMcCornerOffset[c][axis]) << c;
+  return m;
+}
+constexpr uint64_t packEdgeCorner(int end) {
+  uint64_t m = 0;
+  for (int e = 0; e < 12; ++e) m |= static_cast<uint64_t>(kMcEdgeCorners[e][end]) << (4 * e);
+  return m;
+}
+constexpr uint32_t kCornerX = packCornerAxis(0), kCornerY = packCornerAxis(1), kCornerZ = packCornerAxis(2);
+constexpr uint64_t kEdgeC0 = packEdgeCorner(0), kEdgeC1 = packEdgeCorner(1);
+
+__device__ __forceinline__ float pick8(const float v[8], int c) {
+  float r = v[0];
 #pragma unroll
-  for (int e = 0; e < 12; ++e) {
-    ev[e] = F3{0.0f, 0.0f, 0.0f};
-    if ((mask & (1 << e)) == 0) continue;
-    const int c0 = kMcEdgeCorners[e][0], c1 = kMcEdgeCorners[e][1];
-    const float delta = val[c1] - val[c0];
-    if (fabsf(delta) < 1e-6f) {
-      ev[e] = pos[c0];
-      continue;
-    }
-    const float t = sclamp((iso - val[c0]) / delta, 0.0f, 1.0f);
-    ev[e] = add3(pos[c0], mul3(sub3(pos[c1], pos[c0]), t));
-  }
+  for (int m = 1; m < 8; ++m) r = (c == m) ? v[m] : r;
+  return r;
+}
+
+// Centre of the cube corner c of the cube at voxel (ox, oy, oz): the same
+// expression as the corner positions of voxel_hash_tsdf.h:918-922.
+__device__ __forceinline__ F3 cornerPos(int ox, int oy, int oz, int c, float voxel) {
+  return F3{(static_cast<float>(ox + static_cast<int>((kCornerX >> c) & 1u)) + 0.5f) * voxel,
+            (static_cast<float>(oy + static_cast<int>((kCornerY >> c) & 1u)) + 0.5f) * voxel,
+            (static_cast<float>(oz + static_cast<int>((kCornerZ >> c) & 1u)) + 0.5f) * voxel};
+}
+
+// polygoniseCube (marching_cubes.h:48-94): writes the case's triangles.
+// Each triangle vertex interpolates its edge (interpolateVertex,
+// marching_cubes.h:34-46) on the spot, so no 12-edge vertex array is held.
+__device__ __forceinline__ void emitCube(int ox, int oy, int oz, float voxel, const float val[8], float iso, float* out) {
+  const int ci = cubeCase(val, iso);
   const int nt = c_mc.num_tris[ci];
   for (int i = 0; i < nt; ++i)
     for (int j = 0; j < 3; ++j) {
       const int e = c_mc.tri[ci][3 * i + j];
-      F3 q = ev[0];
-#pragma unroll
-      for (int m = 1; m < 12; ++m)
-        if (m == e) q = ev[m];
+      const int c0 = static_cast<int>((kEdgeC0 >> (4 * e)) & 15u), c1 = static_cast<int>((kEdgeC1 >> (4 * e)) & 15u);
+      const float v0 = pick8(val, c0), v1 = pick8(val, c1);
+      const F3 p0 = cornerPos(ox, oy, oz, c0, voxel);
+      F3 q = p0;
+      const float delta = v1 - v0;
+      if (!(fabsf(delta) < 1e-6f)) {
+        const float t = sclamp((iso - v0) / delta, 0.0f, 1.0f);
+        q = add3(p0, mul3(sub3(cornerPos(ox, oy, oz, c1, voxel), p0), t));
+      }
       out[9 * i + 3 * j + 0] = q.x;
       out[9 * i + 3 * j + 1] = q.y;
       out[9 * i + 3 * j + 2] = q.z;
@@ -179,8 +229,10 @@ __device__ void emitCube(const F3 pos[8], const float val[8], float iso, float* 
 }
 
 // One cube of extractDirectionalMultiLayerMesh (voxel_hash_tsdf.h:853-935).
-// Returns the triangle count; emits them at `out` when out != nullptr.
-__device__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a, float* out) {
+// Returns the triangle count; with kEmit also writes them at `out`.
+template <bool kEmit>
+__device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
+                                                float* out) {
   const int tid = threadIdx.x;
   const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
   const int ox = keyX(key) * kBlockSide + lx, oy = keyY(key) * kBlockSide + ly, oz = keyZ(key) * kBlockSide + lz;
@@ -188,12 +240,17 @@ __device__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const Mes
   const F3 cc{(static_cast<float>(ox) + 0.5f) * a.voxel + hv, (static_cast<float>(oy) + 0.5f) * a.voxel + hv,
               (static_cast<float>(oz) + 0.5f) * a.voxel + hv};
   int rc[8];
+  uint32_t span = 0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c)
+  for (int c = 0; c < 8; ++c) {
     rc[c] = (lx + kMcCornerOffset[c][0]) + kR * ((ly + kMcCornerOffset[c][1]) + kR * (lz + kMcCornerOffset[c][2]));
+    span |= S.bits[rc[c]];
+  }
+  if ((span & (kBitBelow | kBitAbove)) != (kBitBelow | kBitAbove)) return 0;  // no sign change possible
 
   // candidates, kept sorted ascending (std::sort, voxel_hash_tsdf.h:887)
   int nc = 0;
+#pragma unroll 1
   for (int c = 0; c < 8; ++c) {
     const uint8_t b = S.bits[rc[c]];
     if (!(b & 8)) continue;
@@ -210,13 +267,6 @@ __device__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const Mes
     }
   }
   if (nc == 0) return 0;
-
-  F3 pos[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c)
-    pos[c] = F3{(static_cast<float>(ox + kMcCornerOffset[c][0]) + 0.5f) * a.voxel,
-                (static_cast<float>(oy + kMcCornerOffset[c][1]) + 0.5f) * a.voxel,
-                (static_cast<float>(oz + kMcCornerOffset[c][2]) + 0.5f) * a.voxel};
 
   uint32_t ntri = 0;
   float centre = 0.0f;
@@ -250,12 +300,12 @@ __device__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const Mes
           }
         }
         if (best < 0) continue;
-        val[c] = planeSD(S, best, rc[c], pos[c]);
+        val[c] = S.sdv[best][rc[c]];  // planeSD at the corner's centre, staged
         valid |= 1u << c;
       }
       if (completeCube(val, valid)) {
         const uint32_t t = c_mc.num_tris[cubeCase(val, a.iso)];
-        if (out != nullptr && t > 0) emitCube(pos, val, a.iso, out + 9u * ntri);
+        if (kEmit && t > 0) emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
         ntri += t;
       }
     }
@@ -287,11 +337,11 @@ __device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
 // Triangle count of (block b, direction dir); the workgroup takes part.
 __device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a) {
   const uint64_t key = V.pool_keys[b];
-  if (!stageRegion(S, V, key, b, dir)) {
+  if (!stageRegion(S, V, key, b, dir, a)) {
     __syncthreads();  // S.any_dir is rewritten by the next item
     return 0u;
   }
-  uint32_t c = processCube(S, key, threadIdx.x, a, nullptr) + processCube(S, key, threadIdx.x + 256, a, nullptr);
+  uint32_t c = processCube<false>(S, key, threadIdx.x, a, nullptr) + processCube<false>(S, key, threadIdx.x + 256, a, nullptr);
   blockExclusiveScan(S, c);
   const uint32_t total = S.total;
   __syncthreads();  // S.total / S.scan are reused by the next item
@@ -303,17 +353,17 @@ __device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, in
 __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a, uint32_t base,
                          float* tris, uint32_t* tri_owner, uint32_t owner, uint32_t capacity) {
   const uint64_t key = V.pool_keys[b];
-  if (!stageRegion(S, V, key, b, dir)) {
+  if (!stageRegion(S, V, key, b, dir, a)) {
     __syncthreads();
     return;
   }
   // cubes 0..255 precede 256..511 in the reference's lz, ly, lx loop order
   for (int half = 0; half < 2; ++half) {
     const int cube = threadIdx.x + 256 * half;
-    const uint32_t c = processCube(S, key, cube, a, nullptr);
+    const uint32_t c = processCube<false>(S, key, cube, a, nullptr);
     const uint32_t start = base + blockExclusiveScan(S, c);
     if (c > 0 && start + c <= capacity) {
-      processCube(S, key, cube, a, tris + 9ull * start);
+      processCube<true>(S, key, cube, a, tris + 9ull * start);
       if (tri_owner)
         for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = owner;
     }

@@ -120,23 +120,24 @@ class SceneSpec:
         return d
 
 
-def random_scene(seed: int, room_min, room_max, n_prims: int, path: np.ndarray, clearance=0.3) -> SceneSpec:
-    """n_prims spheres (radius U[0.1,0.5]) / boxes (sides U[0.2,0.8]) with
-    p = 1/2 each, inside the room, at least `clearance` from every camera
-    position in `path`."""
+def random_scene(seed: int, room_min, room_max, n_prims: int, path: np.ndarray, clearance=0.3,
+                 size_scale=1.0) -> SceneSpec:
+    """n_prims spheres (radius U[0.1,0.5] x size_scale) / boxes (sides
+    U[0.2,0.8] x size_scale) with p = 1/2 each, inside the room, at least
+    `clearance` from every camera position in `path`."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lo, hi = np.array(room_min, float), np.array(room_max, float)
     prims, tries = [], 0
     while len(prims) < n_prims and tries < 100000:
         tries += 1
         if rng.uniform() < 0.5:
-            r = rng.uniform(0.1, 0.5)
+            r = rng.uniform(0.1, 0.5) * size_scale
             c = rng.uniform(lo + r, hi - r)
             if (np.linalg.norm(path - c, axis=-1) - r).min() < clearance:
                 continue
             prims.append((0, c, r, (0.0, 0.0, 0.0)))
         else:
-            h = rng.uniform(0.2, 0.8, size=3) / 2
+            h = rng.uniform(0.2, 0.8, size=3) / 2 * size_scale
             c = rng.uniform(lo + h, hi - h)
             if _point_box_distance(path, c, h).min() < clearance:
                 continue
@@ -181,6 +182,7 @@ class Workload:
     filter_sigma_range: float
     hash_log2_buckets: int
     max_blocks: int
+    target_blocks: int = 0
 
 
 def plain_small(n_frames: int = 100, seed: int = 0) -> Workload:
@@ -203,6 +205,41 @@ def plain_scale(n_frames: int = 500, seed: int = 1) -> Workload:
     scene = random_scene(seed, room_min, room_max, 150, eyes)
     return Workload("plain_scale_1280x720", make_intrinsics(1280, 720, 1050.0, 1050.0, 639.5, 359.5), poses, scene,
                     sensor_noise(seed), 0.005, 0.02, 4.5, 0.03, 22, 700_000)
+
+
+BATCH_RESOLUTIONS = ((320, 240), (640, 480), (1280, 720))
+
+
+def room_for_blocks(seed: int, target_blocks: int, width: int, height: int, n_frames: int = 60) -> Workload:
+    """A plain-small-style room (walls + 20 s^2 primitives scaled by s, circular
+    trajectory) whose size s is chosen so that fusing n_frames at voxel 1 cm
+    allocates about target_blocks blocks (measured on these scenes:
+    blocks ~= 42,130 s^2.15 for s in [0.2, 1.3])."""
+    s = (target_blocks / 42130.0) ** (1.0 / 2.15)
+    L, H = 4.0 * s, 3.0 * s
+    room_min, room_max = (-L / 2, 0.0, -L / 2), (L / 2, H, L / 2)
+    poses = circular_poses(n_frames, 0.375 * L, 0.5 * H, (0.0, H / 3, 0.0))
+    path = np.array([list(p.t) for p in poses])
+    scene = random_scene(seed, room_min, room_max, max(2, int(round(20 * s * s))), path, clearance=0.3 * s,
+                         size_scale=s)
+    f = 525.0 * width / 640.0
+    log2 = max(14, int(math.ceil(math.log2(max(target_blocks, 1) * 4))))
+    return Workload(f"room{target_blocks}_{width}x{height}",
+                    make_intrinsics(width, height, f, f, (width - 1) / 2, (height - 1) / 2), poses, scene,
+                    sensor_noise(seed), 0.01, 0.04, 4.5, 0.03, log2, int(target_blocks * 1.6 + 2048), target_blocks)
+
+
+def batched_scenes(n_scenes: int = 48, seed: int = 2, n_frames: int = 60) -> list[Workload]:
+    """Batched plain method: n_scenes independent rooms with target block
+    counts log-uniform in [1k, 60k] and resolutions drawn from
+    {320x240, 640x480, 1280x720}; n_frames each, voxel 1 cm."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_scenes):
+        target = int(round(math.exp(rng.uniform(math.log(1000.0), math.log(60000.0)))))
+        w, h = BATCH_RESOLUTIONS[int(rng.integers(0, len(BATCH_RESOLUTIONS)))]
+        out.append(room_for_blocks(seed * 1000 + i, target, w, h, n_frames))
+    return out
 
 
 def tiny(n_frames: int = 4, seed: int = 7, width=96, height=72) -> Workload:
