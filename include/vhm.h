@@ -145,6 +145,12 @@ int vhm_stats(vhm_ctx* ctx, uint64_t* updates, uint64_t* visited, uint64_t* fram
  * allocated keys in pool order. */
 int vhm_block_keys(vhm_ctx* ctx, int32_t* keys_out, size_t capacity, size_t* n_out);
 
+/* Keys of the blocks holding at least one observed voxel (the blocks
+ * blockHasObservedVoxels accepts, voxel_hash_tsdf.h:206-224), in pool order;
+ * O(blocks) on the device from the per-block observed bitmasks.  Writes
+ * min(capacity, n) keys; *n_out = n.  Synchronous. */
+int vhm_observed_block_keys(vhm_ctx* ctx, int32_t* keys_out, size_t capacity, size_t* n_out);
+
 /* Voxel layers of the given blocks in the reference's AoS Voxel layout
  * (voxel_hash_tsdf.h:18-31; 288 B per voxel, 147,456 B per block, local
  * index x + 8(y + 8z)) -- what copyBlocksToFlat (voxel_hash_tsdf.h:147-164)

@@ -91,6 +91,7 @@ _SIGS = {
     "vhm_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vhm_block_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vhm_download_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "vhm_observed_block_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vhm_extract_mesh": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vhm_extract_mesh_incremental": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t,
                                                C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
@@ -315,6 +316,20 @@ class Volume:
         m = C.c_size_t()
         self._check(lib().vhm_block_keys(self._h, out.ctypes.data, n, C.byref(m)))
         return out[:n]
+
+    def observed_block_keys(self) -> np.ndarray:
+        """Keys of blocks with any observed voxel (pool order)."""
+        n = C.c_size_t()
+        self._check(lib().vhm_observed_block_keys(self._h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 3), np.int32)
+        if n.value:
+            m = C.c_size_t()
+            self._check(lib().vhm_observed_block_keys(self._h, out.ctypes.data, n.value, C.byref(m)))
+        return out
+
+    def reset(self):
+        """Clears the volume, keeping its reservations."""
+        self._check(lib().vhm_reset(self._h))
 
     def download_blocks(self, keys: np.ndarray) -> np.ndarray:
         """Blocks in the reference's AoS layout: (n, 512, 6, 3) of VOXEL_DTYPE."""

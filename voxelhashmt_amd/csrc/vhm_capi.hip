@@ -53,6 +53,8 @@ struct vhm_ctx {
   uint32_t* mesh_offsets = nullptr;
   uint32_t* mesh_partials = nullptr;
   size_t mesh_counts_cap = 0, mesh_offsets_cap = 0, mesh_partials_cap = 0;
+  uint8_t* cube_counts = nullptr;  // per-cube triangle counts of the count pass
+  size_t cube_counts_cap = 0;
   float* tris = nullptr;
   size_t tris_cap = 0;
   int32_t* key_buf = nullptr;
@@ -434,7 +436,7 @@ int vhm_destroy(vhm_ctx* c) {
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
-                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner};
+                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -563,6 +565,35 @@ int vhm_block_keys(vhm_ctx* c, int32_t* keys_out, size_t capacity, size_t* n_out
   return VHM_OK;
 }
 
+int vhm_observed_block_keys(vhm_ctx* c, int32_t* keys_out, size_t capacity, size_t* n_out) {
+  if (!c || !n_out) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  int rc = readHeader(c, &h);
+  if (rc) return rc;
+  *n_out = 0;
+  const uint32_t nb = h.num_blocks < c->P.max_blocks ? h.num_blocks : c->P.max_blocks;
+  if (nb == 0) return VHM_OK;
+  // the remesh flags are all zero between calls; the compaction clears them again
+  VHM_TRY(c, launchObservedFlags(view(c), nb, c->remesh_flags, c->stream), "observed flags kernel launch");
+  VHM_TRY(c, launchScan(c->remesh_flags, c->remesh_pos, nb, c->remesh_partials, c->stream), "observed scan launch");
+  VHM_TRY(c, launchCompactFlags(c->remesh_flags, c->remesh_pos, nb, c->remesh_list, c->remesh_flags, c->stream),
+          "observed compact launch");
+  uint32_t m = 0;
+  VHM_TRY(c, hipMemcpyAsync(&m, c->remesh_partials + (scanPartialsSize(nb) - 1), sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H observed count");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *n_out = m;
+  const size_t k = capacity < m ? capacity : m;
+  if (k == 0 || !keys_out) return VHM_OK;
+  if ((rc = ensure(c, &c->key_buf, &c->key_buf_cap, 3 * size_t(m), "hipMalloc key buffer"))) return rc;
+  VHM_TRY(c, launchListKeys(view(c), c->remesh_list, m, c->key_buf, c->stream), "observed key launch");
+  VHM_TRY(c, hipMemcpyAsync(keys_out, c->key_buf, 3 * k * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H observed keys");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return VHM_OK;
+}
+
 int vhm_download_blocks(vhm_ctx* c, const int32_t* keys, size_t n, void* out_host) {
   if (!c) return VHM_ERR_INVALID_ARGUMENT;
   if (n == 0) return VHM_OK;
@@ -598,6 +629,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   const size_t entries = size_t(nb) * kDirs;
   if (entries > 0xffffffffull) return fail(c, VHM_ERR_CAPACITY, "mesh extraction failed: too many blocks");
   if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
+  if ((rc = ensure(c, &c->cube_counts, &c->cube_counts_cap, entries * 512, "hipMalloc cube counts"))) return rc;
   if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
   if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
                    "hipMalloc mesh partials")))
@@ -609,7 +641,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
   hipEvent_t ev{};
   beginTimed(c, 3, &ev);
-  VHM_TRY(c, launchMeshCount(view(c), nullptr, nb, ma, c->mesh_counts, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchMeshCount(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->stream), "mesh count kernel launch");
   VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
           "mesh scan kernel launch");
   uint32_t total = 0;
@@ -620,7 +652,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   *n_out = total;
   if (tris_out && capacity > 0 && total > 0) {
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
-    VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_offsets, c->tris, nullptr, total, c->stream),
+    VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, nullptr, total, c->stream),
             "mesh emit kernel launch");
     const size_t m = capacity < total ? capacity : total;
     VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, m * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
@@ -662,6 +694,7 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
   }
   const size_t entries = size_t(m) * kDirs;
   if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
+  if ((rc = ensure(c, &c->cube_counts, &c->cube_counts_cap, entries * 512, "hipMalloc cube counts"))) return rc;
   if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
   if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
                    "hipMalloc mesh partials")))
@@ -671,7 +704,7 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
   ma.iso = iso;
   ma.extract_tol = c->P.voxel_size * c->P.layer_extract_scale;
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
-  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->stream), "mesh count kernel launch");
   VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
           "mesh scan kernel launch");
   uint32_t total = 0;
@@ -690,13 +723,13 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
   if (total > 0 && tris_out && tri_capacity > 0 && out_memkind == VHM_DEVICE) {
     // straight into the caller's device buffers; triangles past the capacity are dropped
     const uint32_t cap = static_cast<uint32_t>(tri_capacity < total ? tri_capacity : total);
-    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_offsets, tris_out, tri_block_out, cap,
+    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, tris_out, tri_block_out, cap,
                               c->stream),
             "mesh emit kernel launch");
   } else if (total > 0 && tris_out && tri_capacity > 0) {
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
     if ((rc = ensure(c, &c->tri_owner, &c->tri_owner_cap, size_t(total), "hipMalloc triangle owners"))) return rc;
-    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_offsets, c->tris, c->tri_owner, total,
+    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, c->tri_owner, total,
                               c->stream),
             "mesh emit kernel launch");
     const size_t t = tri_capacity < total ? tri_capacity : total;

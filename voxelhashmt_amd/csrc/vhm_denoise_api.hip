@@ -35,6 +35,7 @@ struct vhm_denoiser {
   float* T = nullptr;     // [cap][512][6]
   float* M = nullptr;
   int32_t* keys = nullptr;  // [cap][3]
+  std::vector<uint32_t> mcount;  // per sample: number of target cells with M = 1 (host)
   // per-batch scratch
   float* Y = nullptr;
   float* dY = nullptr;
@@ -181,12 +182,26 @@ int vhm_denoiser_append(vhm_denoiser* d, vhm_ctx* input_vol, vhm_ctx* target_vol
   DN_TRY(d, dnLaunchGather(vhmCtxView(input_vol), k, static_cast<uint32_t>(n), P->voxel_size, P->truncation_distance,
                            P->max_weight, d->X + size_t(d->count) * kXElems, d->stream),
          "denoiser gather launch");
+  if (d->mcount.size() < static_cast<size_t>(d->capacity)) d->mcount.resize(d->capacity);
   if (target_vol) {
     const vhm_params* Q = vhmCtxParams(target_vol);
     DN_TRY(d, dnLaunchTarget(vhmCtxView(target_vol), k, static_cast<uint32_t>(n), Q->voxel_size,
                              Q->truncation_distance, Q->max_weight, d->T + size_t(d->count) * kYElems,
                              d->M + size_t(d->count) * kYElems, d->stream),
            "denoiser target launch");
+    // per-sample mask counts, once, for the loss normalisation of training steps
+    std::vector<float> mh(n * kYElems);
+    DN_TRY(d, hipMemcpyAsync(mh.data(), d->M + size_t(d->count) * kYElems, mh.size() * 4, hipMemcpyDeviceToHost,
+                             d->stream),
+           "hipMemcpy masks");
+    DN_TRY(d, hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < n; ++i) {
+      uint32_t c = 0;
+      for (size_t j = 0; j < kYElems; ++j) c += mh[i * kYElems + j] != 0.0f ? 1u : 0u;
+      d->mcount[d->count + i] = c;
+    }
+  } else {
+    for (size_t i = 0; i < n; ++i) d->mcount[d->count + i] = 0;
   }
   DN_TRY(d, hipStreamSynchronize(d->stream), "hipStreamSynchronize");
   d->count += static_cast<int>(n);
@@ -214,13 +229,9 @@ int vhm_denoiser_train_step(vhm_denoiser* d, size_t offset, size_t n, float lr, 
   if (!d || n == 0 || offset + n > static_cast<size_t>(d->count)) return VHM_ERR_INVALID_ARGUMENT;
   if (n > static_cast<size_t>(d->batch_capacity))
     return dfail(d, VHM_ERR_INVALID_ARGUMENT, "denoiser train step failed: batch larger than batch_capacity");
-  // mask count (host side, from the device masks) -> 1 / max(sum M, 1)
-  std::vector<float> mh(n * kYElems);
-  DN_TRY(d, hipMemcpyAsync(mh.data(), d->M + offset * kYElems, mh.size() * 4, hipMemcpyDeviceToHost, d->stream),
-         "hipMemcpy masks");
-  DN_TRY(d, hipStreamSynchronize(d->stream), "hipStreamSynchronize");
+  // mask count (counted per sample at append) -> 1 / max(sum M, 1)
   double msum = 0.0;
-  for (float v : mh) msum += v;
+  for (size_t i = 0; i < n; ++i) msum += static_cast<double>(d->mcount[offset + i]);
   const float inv = 1.0f / static_cast<float>(msum > 1.0 ? msum : 1.0);
   const uint16_t* X = d->X + offset * kXElems;
   DN_TRY(d, dnLaunchForward(X, weights(d), static_cast<uint32_t>(n), d->Y + offset * kYElems, d->H1, d->H2,

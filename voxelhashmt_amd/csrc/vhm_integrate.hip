@@ -38,121 +38,188 @@ __device__ __forceinline__ NormalParams normalParams(const IntegrateArgs& a) {
   return np;
 }
 
-// Integrates pool block p with the whole workgroup (2 voxels per lane).
-// Lane 0 of every wavefront accumulates its updates / newly observed voxels.
-__device__ void integrateOneBlock(const DepthView<float>& D, const NormalParams& np, const VolumeView& V,
-                                  const IntegrateArgs& a, uint32_t p, unsigned long long& n_updates,
-                                  unsigned long long& n_new_obs) {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
+// Lane compaction.  Only about a quarter of a visited block's voxels pass
+// the depth test (observeDepth) and go on to the expensive part (the
+// 21-tap surface normal, the weight and the 3-slot update); run in place,
+// that part would execute with most lanes of every wavefront idle.  So each
+// workgroup first runs observeDepth over whole blocks (2 voxels per lane),
+// appends the passing voxels -- with the projection and depth they need --
+// to a queue in LDS, and runs the expensive part 256 queued voxels at a
+// time, one per lane.  Voxels are independent within a frame (every block
+// is in the active list once), so the order of the updates does not matter.
+constexpr int kQueueCap = 768;  // < 256 left over + 512 appended per block
+
+struct IntegrateQueue {
+  uint32_t p[kQueueCap];
+  uint32_t vi[kQueueCap];
+  float u[kQueueCap], v[kQueueCap], dep[kQueueCap];
+  float pcx[kQueueCap], pcy[kQueueCap], pcz[kQueueCap];
+  uint32_t n;
+};
+
+struct IntegrateTally {
+  unsigned long long updates = 0, new_obs = 0, visited = 0;
+};
+
+// Expensive part for queue entry i (one lane).
+__device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, const DepthView<float>& D,
+                                                const NormalParams& np, const VolumeView& V, const IntegrateArgs& a,
+                                                IntegrateTally& t) {
+  const uint32_t p = Q.p[i], vi = Q.vi[i];
+  const F3 pc{Q.pcx[i], Q.pcy[i], Q.pcz[i]};
+  const VoxelObservation o = observeSurface(D, np, a.T_wc, pc, Q.u[i], Q.v[i], Q.dep[i], a.trunc, a.min_view);
+  if (o.dir < 0) return;
+  ++t.updates;
+  const int d = o.dir;
+  uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
+  Slots L;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    L.w[k] = __uint_as_float(blk[fieldOffset(d, 0, k) + vi]);
+    L.sc[k] = __uint_as_float(blk[fieldOffset(d, 1, k) + vi]);
+    L.po[k] = __uint_as_float(blk[fieldOffset(d, 2, k) + vi]);
+    L.pn[k] = blk[fieldOffset(d, 3, k) + vi];
+  }
+  if (!integrateSample(L, o.coord, o.pw, o.nw, o.weight, a.merge_tol, a.min_cos, a.max_weight)) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    blk[fieldOffset(d, 0, k) + vi] = __float_as_uint(L.w[k]);
+    blk[fieldOffset(d, 1, k) + vi] = __float_as_uint(L.sc[k]);
+    blk[fieldOffset(d, 2, k) + vi] = __float_as_uint(L.po[k]);
+    blk[fieldOffset(d, 3, k) + vi] = L.pn[k];
+  }
+  // observed bitmask (keeps observedVoxelCount O(1)), dirty flag and
+  // direction mask for incremental meshing
+  const uint32_t bit = 1u << (vi & 31u);
+  const uint32_t old = atomicOr(&V.obs[static_cast<size_t>(p) * 16u + (vi >> 5)], bit);
+  if (!(old & bit)) ++t.new_obs;
+  V.dirty[p] = 1u;
+  if (!(V.dirmask[p] & (1u << d))) atomicOr(&V.dirmask[p], 1u << d);
+}
+
+// Runs the expensive part on queue entries [Q.n - m, Q.n) with m = min(Q.n, 256)
+// and removes them.  Workgroup-wide.
+__device__ __forceinline__ void drainQueue(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
+                                           const VolumeView& V, const IntegrateArgs& a, IntegrateTally& t) {
+  const uint32_t n = Q.n;
+  const uint32_t m = n < 256u ? n : 256u;
+  if (threadIdx.x < m) integrateQueued(Q, static_cast<int>(n - m + threadIdx.x), D, np, V, a, t);
+  __syncthreads();
+  if (threadIdx.x == 0) Q.n = n - m;
+  __syncthreads();
+}
+
+// Cheap part for all 512 voxels of pool block p: append the voxels that pass
+// observeDepth to the queue.  Requires Q.n < 256 on entry.  Workgroup-wide.
+__device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
+                                             const VolumeView& V, const IntegrateArgs& a, uint32_t p) {
+  const int tid = threadIdx.x, lane = tid & 63;
   const uint64_t key = V.pool_keys[p];
   const int bx = keyX(key) * kBlockSide, by = keyY(key) * kBlockSide, bz = keyZ(key) * kBlockSide;
-  uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
-  bool any_write = false;
-  uint32_t dirs_written = 0;  // wavefront-uniform after the ballots below
+#pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int vi = tid + 256 * half;
     const int lx = vi & 7, ly = (vi >> 3) & 7, lz = vi >> 6;
-    bool reached = false, wrote = false;
     // voxel centre, voxelToWorld (voxel_hash_tsdf.h:603-609)
     const F3 pw{(static_cast<float>(bx + lx) + 0.5f) * a.voxel, (static_cast<float>(by + ly) + 0.5f) * a.voxel,
                 (static_cast<float>(bz + lz) + 0.5f) * a.voxel};
-    const VoxelObservation o = observeVoxel(D, np, a.T_wc, a.T_cw, pw, a.trunc, a.min_view);
-    if (o.dir >= 0) {
-      reached = true;
-      const int d = o.dir;
-      Slots L;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        L.w[k] = __uint_as_float(blk[fieldOffset(d, 0, k) + vi]);
-        L.sc[k] = __uint_as_float(blk[fieldOffset(d, 1, k) + vi]);
-        L.po[k] = __uint_as_float(blk[fieldOffset(d, 2, k) + vi]);
-        L.pn[k] = blk[fieldOffset(d, 3, k) + vi];
-      }
-      if (integrateSample(L, o.coord, o.pw, o.nw, o.weight, a.merge_tol, a.min_cos, a.max_weight)) {
-        wrote = true;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          blk[fieldOffset(d, 0, k) + vi] = __float_as_uint(L.w[k]);
-          blk[fieldOffset(d, 1, k) + vi] = __float_as_uint(L.sc[k]);
-          blk[fieldOffset(d, 2, k) + vi] = __float_as_uint(L.po[k]);
-          blk[fieldOffset(d, 3, k) + vi] = L.pn[k];
-        }
-      }
+    F3 pc;
+    float u, v, dep;
+    const bool pass = observeDepth(D, np, a.T_cw, pw, a.trunc, pc, u, v, dep);
+    const unsigned long long mask = __ballot(pass);
+    uint32_t base = 0;
+    if (lane == 0 && mask != 0ull) base = atomicAdd(&Q.n, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, 0);
+    if (pass) {
+      const uint32_t i = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+      Q.p[i] = p;
+      Q.vi[i] = static_cast<uint32_t>(vi);
+      Q.u[i] = u;
+      Q.v[i] = v;
+      Q.dep[i] = dep;
+      Q.pcx[i] = pc.x;
+      Q.pcy[i] = pc.y;
+      Q.pcz[i] = pc.z;
     }
-    // Per-wavefront bookkeeping: this wavefront owns voxels [vi - lane, vi - lane + 63].
-    const unsigned long long reached_mask = __ballot(reached);
-    const unsigned long long wrote_mask = __ballot(wrote);
-    if (wrote_mask != 0ull)
-#pragma unroll
-      for (int d = 0; d < kDirs; ++d)
-        if (__ballot(wrote && o.dir == d) != 0ull) dirs_written |= 1u << d;
-    if (lane == 0) {
-      n_updates += __popcll(reached_mask);
-      if (wrote_mask != 0ull) {
-        uint32_t* ow = V.obs + static_cast<size_t>(p) * 16u + static_cast<uint32_t>(vi >> 5);
-        const unsigned long long old = static_cast<unsigned long long>(ow[0]) |
-                                       (static_cast<unsigned long long>(ow[1]) << 32);
-        const unsigned long long now = old | wrote_mask;
-        n_new_obs += __popcll(now & ~old);
-        ow[0] = static_cast<uint32_t>(now);
-        ow[1] = static_cast<uint32_t>(now >> 32);
-      }
-    }
-    any_write = any_write || (wrote_mask != 0ull);
   }
-  if (any_write && lane == 0) {
-    V.dirty[p] = 1u;
-    if ((V.dirmask[p] & dirs_written) != dirs_written) atomicOr(&V.dirmask[p], dirs_written);
-  }
+  __syncthreads();
 }
 
+// Integrates active-list entries [begin, end) of one volume (then drains
+// the queue completely).  Workgroup-wide; Q.n == 0 on entry and exit.
+__device__ __forceinline__ void integrateRange(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
+                               const VolumeView& V, const IntegrateArgs& a, uint32_t begin, uint32_t end,
+                               IntegrateTally& t) {
+  for (uint32_t i = begin; i < end; ++i) {
+    const uint32_t p = V.vals[V.active[i]];
+    if (p >= V.max_blocks) continue;  // pool exhausted for this key (error already flagged)
+    if (threadIdx.x == 0) t.visited += kBlockVoxels;
+    enqueueBlock(Q, D, np, V, a, p);
+    while (Q.n >= 256u) drainQueue(Q, D, np, V, a, t);
+  }
+  while (Q.n > 0u) drainQueue(Q, D, np, V, a, t);
+}
+
+__device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& t) {
+  // one atomic per wavefront: reduce the lanes' counts first
+  unsigned long long u = t.updates, o = t.new_obs;
+  for (int off = 32; off > 0; off >>= 1) {
+    u += __shfl_down(u, off);
+    o += __shfl_down(o, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (u) atomicAdd(&V.hdr->updates, u);
+    if (o) atomicAdd(&V.hdr->observed, o);
+  }
+  if (threadIdx.x == 0 && t.visited) atomicAdd(&V.hdr->visited, t.visited);
+  t = IntegrateTally{};
+}
+
+// Workgroup w integrates the contiguous chunk of the active list
+// [w * c, (w + 1) * c), c = ceil(n / grid).
 __global__ __launch_bounds__(256) void integrateKernel(const float* __restrict__ depth, VolumeView V,
                                                        IntegrateArgs a) {
+  __shared__ IntegrateQueue Q;
+  if (threadIdx.x == 0) Q.n = 0;
+  __syncthreads();
   const DepthView<float> D{depth, a.K.width, a.K.height};
   const NormalParams np = normalParams(a);
   const uint32_t n_active = V.hdr->active_count;
-  const int lane = threadIdx.x & 63;
-  unsigned long long n_updates = 0, n_new_obs = 0, n_visited = 0;
-  for (uint32_t i = blockIdx.x; i < n_active; i += gridDim.x) {
-    const uint32_t p = V.vals[V.active[i]];
-    if (p >= V.max_blocks) continue;  // pool exhausted for this key (error already flagged)
-    if (threadIdx.x == 0) n_visited += kBlockVoxels;
-    integrateOneBlock(D, np, V, a, p, n_updates, n_new_obs);
-  }
-  if (lane == 0) {
-    if (n_updates) atomicAdd(&V.hdr->updates, n_updates);
-    if (n_new_obs) atomicAdd(&V.hdr->observed, n_new_obs);
-    if (n_visited) atomicAdd(&V.hdr->visited, n_visited);
-  }
+  const uint32_t chunk = (n_active + gridDim.x - 1) / gridDim.x;
+  const uint32_t begin = blockIdx.x * chunk;
+  const uint32_t end = begin + chunk < n_active ? begin + chunk : n_active;
+  IntegrateTally t;
+  if (begin < end) integrateRange(Q, D, np, V, a, begin, end, t);
+  flushTally(V, t);
 }
 
-// Batched: a persistent grid strides over the concatenation of every
-// scene's active list (prefix[s] = first item of scene s).
+// Batched: the same over the concatenation of every scene's active list
+// (prefix[s] = first entry of scene s); a chunk crossing scenes is split
+// per scene, so queued voxels always belong to the scene being processed.
 __global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __restrict__ scenes, int S,
                                                             const uint32_t* __restrict__ prefix,
                                                             const BatchState* __restrict__ st) {
+  __shared__ IntegrateQueue Q;
+  if (threadIdx.x == 0) Q.n = 0;
+  __syncthreads();
   const uint32_t step = st->step;
   const uint32_t total = prefix[S];
-  const int lane = threadIdx.x & 63;
-  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
+  const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+  uint32_t i = blockIdx.x * chunk;
+  const uint32_t end = i + chunk < total ? i + chunk : total;
+  while (i < end) {
     const int s = findSegment(prefix, S, i);
     const BatchScene& sc = scenes[s];
-    const VolumeView V = sc.V;
-    const uint32_t p = V.vals[V.active[i - prefix[s]]];
-    if (p >= V.max_blocks) continue;
+    const uint32_t seg_end = prefix[s + 1] < end ? prefix[s + 1] : end;
     IntegrateArgs a = sc.ia;
     a.T_wc = sc.poses[step];
     a.T_cw = sc.poses_inv[step];
     const DepthView<float> D{sc.filtered, a.K.width, a.K.height};
     const NormalParams np = normalParams(a);
-    unsigned long long n_updates = 0, n_new_obs = 0;
-    integrateOneBlock(D, np, V, a, p, n_updates, n_new_obs);
-    if (lane == 0) {
-      if (n_updates) atomicAdd(&V.hdr->updates, n_updates);
-      if (n_new_obs) atomicAdd(&V.hdr->observed, n_new_obs);
-      if (threadIdx.x == 0) atomicAdd(&V.hdr->visited, static_cast<unsigned long long>(kBlockVoxels));
-    }
+    IntegrateTally t;
+    integrateRange(Q, D, np, sc.V, a, i - prefix[s], seg_end - prefix[s], t);
+    flushTally(sc.V, t);
+    i = seg_end;
   }
 }
 

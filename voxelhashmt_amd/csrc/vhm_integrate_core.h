@@ -134,8 +134,11 @@ VHM_HD bool surfaceNormalReference(const D& dv, const NormalParams& a, float u, 
       const float delta = sdep - cd;
       const float c = sclamp(dot3(cn, sn), -1.0f, 1.0f);
       const float ad = 1.0f - c;
-      const float w = a.spatial_w[duu * duu + dvv * dvv] * gaussianWeight(delta * delta, a.inv_r) *
-                      gaussianWeight(ad * ad, a.inv_a);
+      // spatial_w[duu^2 + dvv^2] without a lane-varying index into the
+      // parameter block (which would force it into scratch memory)
+      const int s2 = duu * duu + dvv * dvv;
+      const float sw = s2 == 0 ? a.spatial_w[0] : (s2 == 1 ? a.spatial_w[1] : a.spatial_w[2]);
+      const float w = sw * gaussianWeight(delta * delta, a.inv_r) * gaussianWeight(ad * ad, a.inv_a);
       acc = add3(acc, mul3(sn, w));
       ws += w;
     }
@@ -384,21 +387,31 @@ struct VoxelObservation {
   float weight;
 };
 
-// Everything integrateBlocks does for one voxel centre before
-// integrateDirectionalSample (voxel_hash_tsdf.h:1001-1046).
+// First half of a voxel's observation (voxel_hash_tsdf.h:1001-1021):
+// camera-space centre, projection, bilinear depth and the truncation test.
+// Returns false when the voxel is skipped; the outputs feed observeSurface.
 template <class D>
-VHM_HD VoxelObservation observeVoxel(const D& dv, const NormalParams& a, const Pose34& T_wc, const Pose34& T_cw,
-                                     F3 pw, float trunc, float min_view) {
+VHM_HD bool observeDepth(const D& dv, const NormalParams& a, const Pose34& T_cw, F3 pw, float trunc, F3& pc,
+                         float& u, float& v, float& dep) {
+  pc = transformPoint(T_cw, pw);
+  if (pc.z <= 0.0f) return false;
+  u = 0.0f;
+  v = 0.0f;
+  if (!projectPoint(pc, a.K, u, v)) return false;
+  dep = 0.0f;
+  if (!sampleBilinear(dv, a, u, v, dep)) return false;
+  const float sd = dep - pc.z;
+  return !(sd <= -trunc);
+}
+
+// Second half (voxel_hash_tsdf.h:1023-1046): surface normal, weight,
+// surface point and direction of a voxel that passed observeDepth.
+template <class D>
+VHM_HD VoxelObservation observeSurface(const D& dv, const NormalParams& a, const Pose34& T_wc, F3 pc, float u,
+                                       float v, float dep, float trunc, float min_view) {
   VoxelObservation o;
   o.dir = -1;
-  const F3 pc = transformPoint(T_cw, pw);
-  if (pc.z <= 0.0f) return o;
-  float u = 0.0f, v = 0.0f;
-  if (!projectPoint(pc, a.K, u, v)) return o;
-  float dep = 0.0f;
-  if (!sampleBilinear(dv, a, u, v, dep)) return o;
   const float sd = dep - pc.z;
-  if (sd <= -trunc) return o;
   const F3 ray = normalized3(pc);
   float vaw = 1.0f;
   F3 nc = mul3(ray, -1.0f);
@@ -415,6 +428,21 @@ VHM_HD VoxelObservation observeVoxel(const D& dv, const NormalParams& a, const P
   o.nw = nw;
   o.weight = w;
   return o;
+}
+
+// Everything integrateBlocks does for one voxel centre before
+// integrateDirectionalSample (voxel_hash_tsdf.h:1001-1046).
+template <class D>
+VHM_HD VoxelObservation observeVoxel(const D& dv, const NormalParams& a, const Pose34& T_wc, const Pose34& T_cw,
+                                     F3 pw, float trunc, float min_view) {
+  F3 pc;
+  float u, v, dep;
+  if (!observeDepth(dv, a, T_cw, pw, trunc, pc, u, v, dep)) {
+    VoxelObservation o;
+    o.dir = -1;
+    return o;
+  }
+  return observeSurface(dv, a, T_wc, pc, u, v, dep, trunc, min_view);
 }
 
 }  // namespace vhm

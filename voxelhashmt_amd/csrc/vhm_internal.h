@@ -110,19 +110,21 @@ hipError_t launchIntegrate(const float* depth, const VolumeView& V, const Integr
 hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeView& V, uint32_t* out,
                                  hipStream_t s);
 // Marching cubes over block `list` (pool indices; null = all of 0..n-1):
-// counts[dir * n + i], then triangles at the scanned offsets.
+// counts[dir * n + i] and per-cube counts cube_counts[512 * (dir * n + i) ..]
+// (6 * n * 512 bytes), then triangles at the scanned offsets.
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           uint32_t* counts, hipStream_t s);
+                           uint32_t* counts, uint8_t* cube_counts, hipStream_t s);
 hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s);
 size_t scanPartialsSize(uint32_t n);
 hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                          const uint32_t* offsets, float* tris, uint32_t* tri_owner, uint32_t capacity,
-                          hipStream_t s);
+                          const uint32_t* counts, const uint8_t* cube_counts, const uint32_t* offsets, float* tris,
+                          uint32_t* tri_owner, uint32_t capacity, hipStream_t s);
 // Incremental meshing: dirty blocks -> flags of blocks to re-mesh -> list.
 hipError_t launchMarkRemesh(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s);
 hipError_t launchCompactFlags(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* list,
                               uint32_t* flags_out, hipStream_t s);
 hipError_t launchListKeys(const VolumeView& V, const uint32_t* list, uint32_t n, int32_t* out, hipStream_t s);
+hipError_t launchObservedFlags(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s);
 
 // ---- batched multi-scene step (vhm_batch.hip) ----
 struct BatchState {
@@ -146,6 +148,7 @@ struct BatchScene {
   uint32_t* remesh_list;      // [max_blocks]
   uint32_t* mesh_counts;      // [6 * max_blocks]
   uint32_t* mesh_offsets;     // [6 * max_blocks]
+  uint8_t* cube_counts;       // [6 * max_blocks][512]
   float* tris;                // [tri_capacity][9]
   uint32_t tri_capacity;
   uint32_t* out;              // [4]: blocks re-meshed, triangles (total, may exceed capacity)

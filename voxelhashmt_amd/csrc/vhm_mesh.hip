@@ -230,6 +230,15 @@ __device__ __forceinline__ void emitCube(int ox, int oy, int oz, float voxel, co
 
 // One cube of extractDirectionalMultiLayerMesh (voxel_hash_tsdf.h:853-935).
 // Returns the triangle count; with kEmit also writes them at `out`.
+//
+// Three steps keep the wavefront's lanes in step: (1) the candidate plane
+// distances at the cell centre, kept per (corner, slot) in registers and
+// insertion-sorted into the lane's LDS column (std::sort,
+// voxel_hash_tsdf.h:887); (2) the greedy clustering (888-907), a cheap pass
+// that overwrites the column with the cluster centres; (3) per cluster, the
+// closest slot of each corner, the corner values, completion and the
+// triangles (909-934), in ascending cluster order as the reference
+// processes them.
 template <bool kEmit>
 __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
                                                 float* out) {
@@ -248,70 +257,77 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
   }
   if ((span & (kBitBelow | kBitAbove)) != (kBitBelow | kBitAbove)) return 0;  // no sign change possible
 
-  // candidates, kept sorted ascending (std::sort, voxel_hash_tsdf.h:887)
+  // (1) candidates
+  constexpr float kNone = __builtin_huge_valf();
+  float sc[8][kSlots];
   int nc = 0;
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const uint8_t b = S.bits[rc[c]];
-    if (!(b & 8)) continue;
+    const uint32_t b = S.bits[rc[c]];
+#pragma unroll
     for (int k = 0; k < kSlots; ++k) {
-      if (!(b & (1u << k))) continue;
-      const float s = planeSD(S, k, rc[c], cc);
-      int j = nc;
-      while (j > 0 && S.cand[j - 1][tid] > s) {
-        S.cand[j][tid] = S.cand[j - 1][tid];
-        --j;
+      float s = kNone;
+      if ((b & kBitAllocated) && (b & (1u << k))) {
+        s = planeSD(S, k, rc[c], cc);
+        int j = nc;
+        while (j > 0 && S.cand[j - 1][tid] > s) {
+          S.cand[j][tid] = S.cand[j - 1][tid];
+          --j;
+        }
+        S.cand[j][tid] = s;
+        ++nc;
       }
-      S.cand[j][tid] = s;
-      ++nc;
+      sc[c][k] = s;
     }
   }
   if (nc == 0) return 0;
 
-  uint32_t ntri = 0;
-  float centre = 0.0f;
-  int count = 0;
-  // Greedy clustering (voxel_hash_tsdf.h:888-907); a cluster is processed
-  // (909-934) when the next one opens or the candidates end.
-  for (int i = 0; i <= nc; ++i) {
-    const bool last = (i == nc);
-    const float sk = last ? 0.0f : S.cand[i][tid];
-    if (!last && count > 0 && !(fabsf(sk - centre) > a.extract_tol)) {
+  // (2) clusters: a candidate joins the open cluster unless it lies more than
+  // extract_tol from its running mean; centres overwrite the column (cluster
+  // j's centre is written only after candidate j has been read).
+  int ncl = 0;
+  float centre = S.cand[0][tid];
+  int count = 1;
+  for (int i = 1; i < nc; ++i) {
+    const float sk = S.cand[i][tid];
+    if (!(fabsf(sk - centre) > a.extract_tol)) {
       centre = (centre * static_cast<float>(count) + sk) / static_cast<float>(count + 1);
       ++count;
-      continue;
-    }
-    if (count > 0) {
-      float val[8];
-      uint32_t valid = 0;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        val[c] = 0.0f;
-        const uint8_t b = S.bits[rc[c]];
-        if (!(b & 8)) continue;
-        int best = -1;
-        float best_delta = 3.402823466e+38f;
-        for (int k = 0; k < kSlots; ++k) {
-          if (!(b & (1u << k))) continue;
-          const float delta = fabsf(planeSD(S, k, rc[c], cc) - centre);
-          if (delta <= a.extract_tol && delta < best_delta) {
-            best_delta = delta;
-            best = k;
-          }
-        }
-        if (best < 0) continue;
-        val[c] = S.sdv[best][rc[c]];  // planeSD at the corner's centre, staged
-        valid |= 1u << c;
-      }
-      if (completeCube(val, valid)) {
-        const uint32_t t = c_mc.num_tris[cubeCase(val, a.iso)];
-        if (kEmit && t > 0) emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
-        ntri += t;
-      }
-    }
-    if (!last) {
+    } else {
+      S.cand[ncl++][tid] = centre;
       centre = sk;
       count = 1;
+    }
+  }
+  S.cand[ncl++][tid] = centre;
+
+  // (3) per cluster
+  uint32_t ntri = 0;
+  for (int j = 0; j < ncl; ++j) {
+    const float cen = S.cand[j][tid];
+    float val[8];
+    uint32_t valid = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      val[c] = 0.0f;
+      int best = -1;
+      float best_delta = 3.402823466e+38f;
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) {
+        const float delta = fabsf(sc[c][k] - cen);  // +inf for a missing candidate
+        if (delta <= a.extract_tol && delta < best_delta) {
+          best_delta = delta;
+          best = k;
+        }
+      }
+      if (best < 0) continue;
+      val[c] = S.sdv[best][rc[c]];  // planeSD at the corner's centre, staged
+      valid |= 1u << c;
+    }
+    if (completeCube(val, valid)) {
+      const uint32_t t = c_mc.num_tris[cubeCase(val, a.iso)];
+      if (kEmit && t > 0) emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
+      ntri += t;
     }
   }
   return ntri;
@@ -335,23 +351,34 @@ __device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
 }
 
 // Triangle count of (block b, direction dir); the workgroup takes part.
-__device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a) {
+// Per-cube counts go to cube_counts[0..511] (one byte per cube: at most
+// 24 clusters x 5 triangles) for the emit pass.
+__device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a,
+                              uint8_t* __restrict__ cube_counts) {
   const uint64_t key = V.pool_keys[b];
   if (!stageRegion(S, V, key, b, dir, a)) {
+    cube_counts[threadIdx.x] = 0;
+    cube_counts[threadIdx.x + 256] = 0;
     __syncthreads();  // S.any_dir is rewritten by the next item
     return 0u;
   }
-  uint32_t c = processCube<false>(S, key, threadIdx.x, a, nullptr) + processCube<false>(S, key, threadIdx.x + 256, a, nullptr);
-  blockExclusiveScan(S, c);
+  const uint32_t c0 = processCube<false>(S, key, threadIdx.x, a, nullptr);
+  const uint32_t c1 = processCube<false>(S, key, threadIdx.x + 256, a, nullptr);
+  cube_counts[threadIdx.x] = static_cast<uint8_t>(c0);
+  cube_counts[threadIdx.x + 256] = static_cast<uint8_t>(c1);
+  blockExclusiveScan(S, c0 + c1);
   const uint32_t total = S.total;
   __syncthreads();  // S.total / S.scan are reused by the next item
   return total;
 }
 
-// Emits the triangles of (block b, direction dir) at tris[base ..] in cube
-// order; tri_owner (optional) receives owner for each triangle.
+// Emits the `total` triangles of (block b, direction dir) at tris[base ..] in
+// cube order, running only the cubes the count pass found triangles in;
+// tri_owner (optional) receives owner for each triangle.
 __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a, uint32_t base,
-                         float* tris, uint32_t* tri_owner, uint32_t owner, uint32_t capacity) {
+                         uint32_t total, const uint8_t* __restrict__ cube_counts, float* tris, uint32_t* tri_owner,
+                         uint32_t owner, uint32_t capacity) {
+  if (total == 0) return;  // workgroup-uniform
   const uint64_t key = V.pool_keys[b];
   if (!stageRegion(S, V, key, b, dir, a)) {
     __syncthreads();
@@ -360,7 +387,7 @@ __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir
   // cubes 0..255 precede 256..511 in the reference's lz, ly, lx loop order
   for (int half = 0; half < 2; ++half) {
     const int cube = threadIdx.x + 256 * half;
-    const uint32_t c = processCube<false>(S, key, cube, a, nullptr);
+    const uint32_t c = cube_counts[cube];
     const uint32_t start = base + blockExclusiveScan(S, c);
     if (c > 0 && start + c <= capacity) {
       processCube<true>(S, key, cube, a, tris + 9ull * start);
@@ -373,25 +400,30 @@ __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir
 }
 
 // Grid (n, 6): workgroup (i, dir) meshes block list[i] (or pool index i
-// when list is null) in direction dir.
+// when list is null) in direction dir.  Item dir * n + i: total in
+// counts[item], per-cube counts in cube_counts[512 * item ..].
 __global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
-                                                       MeshArgs a, uint32_t* __restrict__ counts) {
+                                                       MeshArgs a, uint32_t* __restrict__ counts,
+                                                       uint8_t* __restrict__ cube_counts) {
   __shared__ MeshShared S;
   const uint32_t i = blockIdx.x, dir = blockIdx.y;
-  const uint32_t total = countItem(S, V, list ? list[i] : i, static_cast<int>(dir), a);
-  if (threadIdx.x == 0) counts[static_cast<size_t>(dir) * n + i] = total;
+  const size_t item = static_cast<size_t>(dir) * n + i;
+  const uint32_t total = countItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, cube_counts + 512 * item);
+  if (threadIdx.x == 0) counts[item] = total;
 }
 
 // Writes the triangles of workgroup (i, dir) at offsets[dir * n + i] in cube
 // order; tri_owner (optional) receives i for each triangle.
 __global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
-                                                      MeshArgs a, const uint32_t* __restrict__ offsets,
-                                                      float* __restrict__ tris, uint32_t* __restrict__ tri_owner,
-                                                      uint32_t capacity) {
+                                                      MeshArgs a, const uint32_t* __restrict__ counts,
+                                                      const uint8_t* __restrict__ cube_counts,
+                                                      const uint32_t* __restrict__ offsets, float* __restrict__ tris,
+                                                      uint32_t* __restrict__ tri_owner, uint32_t capacity) {
   __shared__ MeshShared S;
   const uint32_t i = blockIdx.x, dir = blockIdx.y;
-  emitItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, offsets[static_cast<size_t>(dir) * n + i], tris,
-           tri_owner, i, capacity);
+  const size_t item = static_cast<size_t>(dir) * n + i;
+  emitItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, offsets[item], counts[item], cube_counts + 512 * item,
+           tris, tri_owner, i, capacity);
 }
 
 // Blocks whose cubes read a dirty block: the dirty block itself and its
@@ -407,6 +439,18 @@ __global__ __launch_bounds__(256) void markRemeshKernel(VolumeView V, uint32_t n
                                                  keyZ(key) - (o >> 2));
       if (q < V.max_blocks) flags[q] = 1u;
     }
+  }
+}
+
+// flags[p] = 1 for blocks with any observed voxel (non-zero observed bitmask).
+__global__ __launch_bounds__(256) void observedFlagsKernel(VolumeView V, uint32_t num_blocks,
+                                                           uint32_t* __restrict__ flags) {
+  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < num_blocks; p += gridDim.x * 256u) {
+    const uint32_t* o = V.obs + static_cast<size_t>(p) * 16u;
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) any |= o[i];
+    flags[p] = any ? 1u : 0u;
   }
 }
 
@@ -585,7 +629,8 @@ __global__ __launch_bounds__(256) void meshCountBatchKernel(const BatchScene* __
     const BatchScene& sc = scenes[s];
     const uint32_t m = sc.out[0], j = i - prefix[s];
     const uint32_t dir = j / m, idx = j % m;
-    const uint32_t c = countItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma);
+    const uint32_t c = countItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma,
+                                 sc.cube_counts + 512ull * (dir * m + idx));
     if (threadIdx.x == 0) sc.mesh_counts[dir * m + idx] = c;
   }
 }
@@ -604,18 +649,18 @@ __global__ __launch_bounds__(256) void meshEmitBatchKernel(const BatchScene* __r
     const int s = findSegment(prefix, S, i);
     const BatchScene& sc = scenes[s];
     const uint32_t m = sc.out[0], j = i - prefix[s];
-    const uint32_t dir = j / m, idx = j % m;
-    emitItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, sc.mesh_offsets[dir * m + idx], sc.tris,
-             nullptr, 0u, sc.tri_capacity);
+    const uint32_t dir = j / m, idx = j % m, item = dir * m + idx;
+    emitItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, sc.mesh_offsets[item],
+             sc.mesh_counts[item], sc.cube_counts + 512ull * item, sc.tris, nullptr, 0u, sc.tri_capacity);
   }
 }
 
 }  // namespace
 
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           uint32_t* counts, hipStream_t s) {
+                           uint32_t* counts, uint8_t* cube_counts, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  meshCountKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts);
+  meshCountKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts, cube_counts);
   return hipGetLastError();
 }
 
@@ -631,10 +676,11 @@ hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* p
 }
 
 hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                          const uint32_t* offsets, float* tris, uint32_t* tri_owner, uint32_t capacity,
-                          hipStream_t s) {
+                          const uint32_t* counts, const uint8_t* cube_counts, const uint32_t* offsets, float* tris,
+                          uint32_t* tri_owner, uint32_t capacity, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  meshEmitKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, offsets, tris, tri_owner, capacity);
+  meshEmitKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts, cube_counts, offsets, tris, tri_owner,
+                                                 capacity);
   return hipGetLastError();
 }
 
@@ -646,6 +692,12 @@ static uint32_t gridFor(uint32_t n) {
 hipError_t launchMarkRemesh(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s) {
   if (num_blocks == 0) return hipSuccess;
   markRemeshKernel<<<gridFor(num_blocks), 256, 0, s>>>(V, num_blocks, flags);
+  return hipGetLastError();
+}
+
+hipError_t launchObservedFlags(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s) {
+  if (num_blocks == 0) return hipSuccess;
+  observedFlagsKernel<<<gridFor(num_blocks), 256, 0, s>>>(V, num_blocks, flags);
   return hipGetLastError();
 }
 

@@ -60,14 +60,35 @@ def gt_surface_points(scene, K, poses, stride: int = 8, max_points: int = 400_00
     return pts
 
 
-def chamfer_mm(triangles: np.ndarray, scene, gt_points: np.ndarray) -> dict:
+def _unique_vertices(tris32: np.ndarray) -> np.ndarray:
+    """Distinct vertices (exact float32 bit patterns) of a triangle soup, as float64."""
+    v = np.ascontiguousarray(tris32, np.float32).reshape(-1, 3)
+    b = v.view(np.uint32)
+    k1 = (b[:, 0].astype(np.uint64) << np.uint64(32)) | b[:, 1].astype(np.uint64)
+    k2 = b[:, 2]
+    order = np.lexsort((k2, k1))
+    s1, s2 = k1[order], k2[order]
+    keep = np.ones(len(order), bool)
+    keep[1:] = (s1[1:] != s1[:-1]) | (s2[1:] != s2[:-1])
+    return v[order[keep]].astype(np.float64)
+
+
+def chamfer_mm(triangles: np.ndarray, scene, gt_points: np.ndarray, max_accuracy_samples: int = 500_000,
+               seed: int = 0) -> dict:
+    """accuracy over (a seeded random subset of at most max_accuracy_samples)
+    unique mesh vertices; completeness over all gt_points against all unique
+    vertices."""
     from scipy.spatial import cKDTree
-    verts = np.asarray(triangles, np.float64).reshape(-1, 3)
+    verts = np.asarray(triangles).reshape(-1, 3)
     if len(verts) == 0 or len(gt_points) == 0:
-        return {"chamfer_mm": float("inf"), "accuracy_mm": float("inf"), "completeness_mm": float("inf")}
-    verts = np.unique(verts, axis=0)
-    acc = np.abs(scene.sdf(verts)).mean() * 1e3
-    comp_d, _ = cKDTree(verts).query(gt_points, k=1)
+        return {"chamfer_mm": float("inf"), "accuracy_mm": float("inf"), "completeness_mm": float("inf"),
+                "vertices": 0}
+    verts = _unique_vertices(verts)
+    sample = verts
+    if len(verts) > max_accuracy_samples:
+        sample = verts[np.random.default_rng(seed).choice(len(verts), max_accuracy_samples, replace=False)]
+    acc = np.abs(scene.sdf(sample)).mean() * 1e3
+    comp_d, _ = cKDTree(verts, balanced_tree=False, compact_nodes=False).query(gt_points, k=1)
     comp = comp_d.mean() * 1e3
     return {"chamfer_mm": float((acc + comp) / 2), "accuracy_mm": float(acc), "completeness_mm": float(comp),
             "vertices": int(len(verts))}
