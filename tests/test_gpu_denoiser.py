@@ -145,3 +145,48 @@ def test_training_reduces_loss_and_apply_fallback(gpu):
         assert r["cells_replaced"] > 0
     few.close()
     many.close()
+
+
+def _pair(seed, frames_in=2, frames_target=12, sigma=0.004):
+    """(few-frame volume, many-frame half-noise volume, their common observed
+    keys) for one small seeded room rendered at 160x120."""
+    w = synth.tiny(n_frames=frames_target, seed=seed, width=160, height=120)
+    noisy = vhm.Noise(a=sigma, b=0.0, c=0.4, dropout=0.01, max_depth=0.0, seed=seed, enabled=1)
+    clean = vhm.Noise(a=sigma / 2, b=0.0, c=0.4, dropout=0.01, max_depth=0.0, seed=seed + 7919, enabled=1)
+    f_in = synth.render(w.scene, w.K, w.poses, noisy)
+    f_t = synth.render(w.scene, w.K, w.poses, clean)
+    vp = vhm.default_params(w.voxel_size, w.truncation, hash_log2_buckets=16, max_blocks=16384,
+                            max_width=w.K.width, max_height=w.K.height)
+    few, many = vhm.Volume(vp), vhm.Volume(vp)
+    for i in range(frames_target):
+        if i % (frames_target // frames_in) == 0:
+            few.integrate(f_in[i], w.K, w.poses[i])
+        many.integrate(f_t[i], w.K, w.poses[i])
+    kf = {tuple(k) for k in few.observed_block_keys()}
+    keys = np.array([k for k in O.sorted_keys(many.observed_block_keys()) if tuple(k) in kf], np.int32)
+    return few, many, keys
+
+
+def test_trained_network_beats_plain_few_frame_fusion(gpu):
+    """The gain test: trained on pairs from 6 rooms, the network's masked
+    error against the many-frame target on 2 unseen rooms is below that of
+    plain few-frame fusion (the identity: zero residual head) -- what the
+    learned part adds over the method without it."""
+    import voxelhashmt_amd.learned as L
+    with vhm.Denoiser(8192, 1024) as dn:
+        dn.set_params(L.init_params(3))
+        for seed in range(40, 48):
+            few, many, keys = _pair(seed)
+            dn.append(few, keys[:600], many)
+            few.close()
+            many.close()
+            if seed == 45:
+                n_train = dn.count
+        print(f"[gain] {n_train} training blocks, {dn.count - n_train} held out", flush=True)
+        net0, ident = L.held_out_loss(dn, n_train, dn.count - n_train, 1024)
+        losses = L.train(dn, 300, n_train, batch=1024, lr=2e-3, seed=1)
+        net, ident2 = L.held_out_loss(dn, n_train, dn.count - n_train, 1024)
+        print(f"[gain] train loss {losses[0]:.5f} -> {np.mean(losses[-10:]):.5f}; held-out: identity {ident:.5f}, "
+              f"untrained {net0:.5f}, trained {net:.5f}", flush=True)
+        assert abs(ident - ident2) <= 1e-4 * max(ident, 1e-12) + 1e-9  # the identity ignores W1, W2
+        assert net < 0.9 * ident

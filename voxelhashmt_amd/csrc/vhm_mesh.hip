@@ -45,6 +45,10 @@ struct MeshShared {
   float cand[kMaxCand][256];
   uint32_t scan[256];
   uint32_t total;
+  // cubes that may produce triangles, compacted so that busy lanes are dense
+  uint16_t list_cube[kBlockVoxels];
+  uint32_t list_start[kBlockVoxels];
+  uint32_t nlist;
 };
 
 __device__ uint32_t lookupPool(const VolumeView& V, int x, int y, int z) {
@@ -76,7 +80,10 @@ __device__ uint32_t lookupPool(const VolumeView& V, int x, int y, int z) {
 __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, uint32_t self, int dir,
                             const MeshArgs& a) {
   const int tid = threadIdx.x;
-  if (tid == 0) S.any_dir = 0u;
+  if (tid == 0) {
+    S.any_dir = 0u;
+    S.nlist = 0u;
+  }
   __syncthreads();
   if (tid < 8) {
     const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
@@ -189,11 +196,14 @@ constexpr uint64_t packEdgeCorner(int end) {
 constexpr uint32_t kCornerX = packCornerAxis(0), kCornerY = packCornerAxis(1), kCornerZ = packCornerAxis(2);
 constexpr uint64_t kEdgeC0 = packEdgeCorner(0), kEdgeC1 = packEdgeCorner(1);
 
+// v[c] for a lane-varying c as a 3-level select tree on the bits of c (a
+// compare chain would be folded back into an indexed access, which forces
+// v out of registers).
 __device__ __forceinline__ float pick8(const float v[8], int c) {
-  float r = v[0];
-#pragma unroll
-  for (int m = 1; m < 8; ++m) r = (c == m) ? v[m] : r;
-  return r;
+  const bool b0 = (c & 1) != 0, b1 = (c & 2) != 0, b2 = (c & 4) != 0;
+  const float a01 = b0 ? v[1] : v[0], a23 = b0 ? v[3] : v[2], a45 = b0 ? v[5] : v[4], a67 = b0 ? v[7] : v[6];
+  const float a03 = b1 ? a23 : a01, a47 = b1 ? a67 : a45;
+  return b2 ? a47 : a03;
 }
 
 // Centre of the cube corner c of the cube at voxel (ox, oy, oz): the same
@@ -228,8 +238,20 @@ __device__ __forceinline__ void emitCube(int ox, int oy, int oz, float voxel, co
     }
 }
 
-// One cube of extractDirectionalMultiLayerMesh (voxel_hash_tsdf.h:853-935).
-// Returns the triangle count; with kEmit also writes them at `out`.
+// Whether any corner of `cube` may lie below iso and any at or above it
+// (see stageRegion); a cube failing this yields no triangle.
+__device__ __forceinline__ bool cubeMayCross(const MeshShared& S, int cube) {
+  const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
+  const int r0 = lx + kR * (ly + kR * lz);
+  const uint32_t span = S.bits[r0] | S.bits[r0 + 1] | S.bits[r0 + kR] | S.bits[r0 + kR + 1] |
+                        S.bits[r0 + kR * kR] | S.bits[r0 + kR * kR + 1] | S.bits[r0 + kR * kR + kR] |
+                        S.bits[r0 + kR * kR + kR + 1];
+  return (span & (kBitBelow | kBitAbove)) == (kBitBelow | kBitAbove);
+}
+
+// One cube of extractDirectionalMultiLayerMesh (voxel_hash_tsdf.h:853-935)
+// that passed cubeMayCross.  Returns the triangle count; with kEmit also
+// writes them at `out`.
 //
 // Three steps keep the wavefront's lanes in step: (1) the candidate plane
 // distances at the cell centre, kept per (corner, slot) in registers and
@@ -249,13 +271,9 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
   const F3 cc{(static_cast<float>(ox) + 0.5f) * a.voxel + hv, (static_cast<float>(oy) + 0.5f) * a.voxel + hv,
               (static_cast<float>(oz) + 0.5f) * a.voxel + hv};
   int rc[8];
-  uint32_t span = 0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
+  for (int c = 0; c < 8; ++c)
     rc[c] = (lx + kMcCornerOffset[c][0]) + kR * ((ly + kMcCornerOffset[c][1]) + kR * (lz + kMcCornerOffset[c][2]));
-    span |= S.bits[rc[c]];
-  }
-  if ((span & (kBitBelow | kBitAbove)) != (kBitBelow | kBitAbove)) return 0;  // no sign change possible
 
   // (1) candidates
   constexpr float kNone = __builtin_huge_valf();
@@ -350,9 +368,25 @@ __device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
   return incl - x;
 }
 
+// Appends `cube` (with `start`) to the compacted list when `take`; the
+// whole wavefront calls it together.
+__device__ __forceinline__ void listAppend(MeshShared& S, bool take, int cube, uint32_t start) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long mask = __ballot(take);
+  uint32_t base = 0;
+  if (lane == 0 && mask != 0ull) base = atomicAdd(&S.nlist, static_cast<uint32_t>(__popcll(mask)));
+  base = __shfl(base, 0);
+  if (take) {
+    const uint32_t i = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    S.list_cube[i] = static_cast<uint16_t>(cube);
+    S.list_start[i] = start;
+  }
+}
+
 // Triangle count of (block b, direction dir); the workgroup takes part.
 // Per-cube counts go to cube_counts[0..511] (one byte per cube: at most
-// 24 clusters x 5 triangles) for the emit pass.
+// 24 clusters x 5 triangles) for the emit pass.  The cubes passing
+// cubeMayCross are compacted first and then spread over the lanes.
 __device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a,
                               uint8_t* __restrict__ cube_counts) {
   const uint64_t key = V.pool_keys[b];
@@ -362,19 +396,32 @@ __device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, in
     __syncthreads();  // S.any_dir is rewritten by the next item
     return 0u;
   }
-  const uint32_t c0 = processCube<false>(S, key, threadIdx.x, a, nullptr);
-  const uint32_t c1 = processCube<false>(S, key, threadIdx.x + 256, a, nullptr);
-  cube_counts[threadIdx.x] = static_cast<uint8_t>(c0);
-  cube_counts[threadIdx.x + 256] = static_cast<uint8_t>(c1);
-  blockExclusiveScan(S, c0 + c1);
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int cube = threadIdx.x + 256 * half;
+    const bool take = cubeMayCross(S, cube);
+    if (!take) cube_counts[cube] = 0;
+    listAppend(S, take, cube, 0u);
+  }
+  __syncthreads();
+  const uint32_t n = S.nlist;
+  uint32_t mine = 0;
+  for (uint32_t j = threadIdx.x; j < n; j += 256) {
+    const int cube = S.list_cube[j];
+    const uint32_t c = processCube<false>(S, key, cube, a, nullptr);
+    cube_counts[cube] = static_cast<uint8_t>(c);
+    mine += c;
+  }
+  blockExclusiveScan(S, mine);
   const uint32_t total = S.total;
-  __syncthreads();  // S.total / S.scan are reused by the next item
+  __syncthreads();  // S.total / S.scan / S.nlist are reused by the next item
   return total;
 }
 
 // Emits the `total` triangles of (block b, direction dir) at tris[base ..] in
-// cube order, running only the cubes the count pass found triangles in;
-// tri_owner (optional) receives owner for each triangle.
+// cube order, running only the cubes the count pass found triangles in
+// (compacted over the lanes); tri_owner (optional) receives owner for each
+// triangle.
 __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a, uint32_t base,
                          uint32_t total, const uint8_t* __restrict__ cube_counts, float* tris, uint32_t* tri_owner,
                          uint32_t owner, uint32_t capacity) {
@@ -384,17 +431,25 @@ __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir
     __syncthreads();
     return;
   }
-  // cubes 0..255 precede 256..511 in the reference's lz, ly, lx loop order
+  // offsets in the reference's cube order (lz, ly, lx: cubes 0..255, then 256..511)
   for (int half = 0; half < 2; ++half) {
     const int cube = threadIdx.x + 256 * half;
     const uint32_t c = cube_counts[cube];
     const uint32_t start = base + blockExclusiveScan(S, c);
-    if (c > 0 && start + c <= capacity) {
+    listAppend(S, c > 0, cube, start);
+    base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
+  }
+  __syncthreads();
+  const uint32_t n = S.nlist;
+  for (uint32_t j = threadIdx.x; j < n; j += 256) {
+    const int cube = S.list_cube[j];
+    const uint32_t start = S.list_start[j];
+    const uint32_t c = cube_counts[cube];
+    if (start + c <= capacity) {
       processCube<true>(S, key, cube, a, tris + 9ull * start);
       if (tri_owner)
         for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = owner;
     }
-    base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
   }
   __syncthreads();
 }
