@@ -289,12 +289,16 @@ def main():
     integ = kt["integrate"]
     avg_ms = integ["ms"] / max(integ["launches"], 1)
     achieved = BYTES_PER_UPDATE * updates / max(integ["ms"] * 1e-3, 1e-12) / 1e9
+    # HBM traffic per launch: bytes per voxel update measured with
+    # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by a plain copy
+    # kernel's reading), times this run's updates per launch
     traffic = None
     tfile = ROOT / "profiles" / "integrate_traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-        except Exception:
+            bpu = json.loads(tfile.read_text())["bytes_per_update"]
+            traffic = round(bpu * updates / max(integ["launches"], 1))
+        except (KeyError, ValueError):
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -66,6 +66,10 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
                                                 const NormalParams& np, const VolumeView& V, const IntegrateArgs& a,
                                                 IntegrateTally& t) {
   const uint32_t p = Q.p[i], vi = Q.vi[i];
+  // bookkeeping words, loaded early: only this voxel's own update can set its
+  // observed bit this frame (every block is in the active list once)
+  const uint32_t obs_word = V.obs[static_cast<size_t>(p) * 16u + (vi >> 5)];
+  const uint32_t dmask = V.dirmask[p];
   const F3 pc{Q.pcx[i], Q.pcy[i], Q.pcz[i]};
   const VoxelObservation o = observeSurface(D, np, a.T_wc, pc, Q.u[i], Q.v[i], Q.dep[i], a.trunc, a.min_view);
   if (o.dir < 0) return;
@@ -80,21 +84,27 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
     L.po[k] = __uint_as_float(blk[fieldOffset(d, 2, k) + vi]);
     L.pn[k] = blk[fieldOffset(d, 3, k) + vi];
   }
+  const Slots L0 = L;
   if (!integrateSample(L, o.coord, o.pw, o.nw, o.weight, a.merge_tol, a.min_cos, a.max_weight)) return;
+  // write back only the words that changed (usually the one updated slot)
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    blk[fieldOffset(d, 0, k) + vi] = __float_as_uint(L.w[k]);
-    blk[fieldOffset(d, 1, k) + vi] = __float_as_uint(L.sc[k]);
-    blk[fieldOffset(d, 2, k) + vi] = __float_as_uint(L.po[k]);
-    blk[fieldOffset(d, 3, k) + vi] = L.pn[k];
+    if (__float_as_uint(L.w[k]) != __float_as_uint(L0.w[k])) blk[fieldOffset(d, 0, k) + vi] = __float_as_uint(L.w[k]);
+    if (__float_as_uint(L.sc[k]) != __float_as_uint(L0.sc[k]))
+      blk[fieldOffset(d, 1, k) + vi] = __float_as_uint(L.sc[k]);
+    if (__float_as_uint(L.po[k]) != __float_as_uint(L0.po[k]))
+      blk[fieldOffset(d, 2, k) + vi] = __float_as_uint(L.po[k]);
+    if (L.pn[k] != L0.pn[k]) blk[fieldOffset(d, 3, k) + vi] = L.pn[k];
   }
   // observed bitmask (keeps observedVoxelCount O(1)), dirty flag and
   // direction mask for incremental meshing
   const uint32_t bit = 1u << (vi & 31u);
-  const uint32_t old = atomicOr(&V.obs[static_cast<size_t>(p) * 16u + (vi >> 5)], bit);
-  if (!(old & bit)) ++t.new_obs;
+  if (!(obs_word & bit)) {
+    atomicOr(&V.obs[static_cast<size_t>(p) * 16u + (vi >> 5)], bit);
+    ++t.new_obs;
+  }
   V.dirty[p] = 1u;
-  if (!(V.dirmask[p] & (1u << d))) atomicOr(&V.dirmask[p], 1u << d);
+  if (!(dmask & (1u << d))) atomicOr(&V.dirmask[p], 1u << d);
 }
 
 // Runs the expensive part on queue entries [Q.n - m, Q.n) with m = min(Q.n, 256)

@@ -43,7 +43,15 @@ def unpack_octahedral(packed: np.ndarray) -> np.ndarray:
 def gt_surface_points(scene, K, poses, stride: int = 8, max_points: int = 400_000, seed: int = 0) -> np.ndarray:
     """Noise-free depth of every `stride`-th pixel of the given frames, back-projected to the world."""
     from . import synth
-    depth = synth.render(scene, K, poses, None).astype(np.float64) * 1e-3
+    try:  # the GPU ray caster when a device is present (same pixels as the host one)
+        import torch
+        if not torch.cuda.is_available():
+            raise ImportError
+        out = torch.empty((len(poses), K.height, K.width), dtype=torch.int16, device="cuda:0")
+        synth.render(scene, K, poses, None, 0, out)
+        depth = out.cpu().numpy().view(np.uint16).astype(np.float64) * 1e-3
+    except ImportError:
+        depth = synth.render(scene, K, poses, None).astype(np.float64) * 1e-3
     pts = []
     vv, uu = np.mgrid[0:K.height:stride, 0:K.width:stride]
     for i, p in enumerate(poses):
@@ -88,7 +96,7 @@ def chamfer_mm(triangles: np.ndarray, scene, gt_points: np.ndarray, max_accuracy
     if len(verts) > max_accuracy_samples:
         sample = verts[np.random.default_rng(seed).choice(len(verts), max_accuracy_samples, replace=False)]
     acc = np.abs(scene.sdf(sample)).mean() * 1e3
-    comp_d, _ = cKDTree(verts, balanced_tree=False, compact_nodes=False).query(gt_points, k=1)
+    comp_d, _ = cKDTree(verts, balanced_tree=False, compact_nodes=False).query(gt_points, k=1, workers=-1)
     comp = comp_d.mean() * 1e3
     return {"chamfer_mm": float((acc + comp) / 2), "accuracy_mm": float(acc), "completeness_mm": float(comp),
             "vertices": int(len(verts))}
