@@ -40,4 +40,13 @@ void vcheck_exp_table(uint64_t* out32) {
   for (int i = 0; i < 32; ++i) out32[i] = vhm::kExp2Tab32[i];
 }
 
+// The product's octahedral dequantisation for every 16-bit code, beside the
+// plain correctly rounded division it replaces.
+void vcheck_dequant_all(float* product, float* division) {
+  for (uint32_t c = 0; c < 65536u; ++c) {
+    product[c] = vhm::dequantOct(c);
+    division[c] = (static_cast<float>(c) / 65535.0f) * 2.0f - 1.0f;
+  }
+}
+
 }  // extern "C"

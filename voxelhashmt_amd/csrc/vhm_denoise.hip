@@ -156,7 +156,7 @@ __device__ void voxelFeature(const uint32_t* blk, int vi, int d, F3 centre, floa
   int bk = -1;
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
-    const float wk = __uint_as_float(blk[fieldOffset(d, 0, k) + vi]);
+    const float wk = __uint_as_float(blk[poolWord(d, 0, k, vi)]);
     if (wk > bw) {
       bw = wk;
       bk = k;
@@ -168,8 +168,8 @@ __device__ void voxelFeature(const uint32_t* blk, int vi, int d, F3 centre, floa
     w = 0.0f;
     return;
   }
-  const F3 n = unpackNormalOct(blk[fieldOffset(d, 3, bk) + vi]);
-  const float sd = dot3(n, centre) - __uint_as_float(blk[fieldOffset(d, 2, bk) + vi]);
+  const F3 n = unpackNormalOct(blk[poolWord(d, 3, bk, vi)]);
+  const float sd = dot3(n, centre) - __uint_as_float(blk[poolWord(d, 2, bk, vi)]);
   tsdf = sclamp(sd * inv_trunc, -1.0f, 1.0f);
   w = bw * inv_maxw;
 }
@@ -670,9 +670,9 @@ __global__ __launch_bounds__(256) void dnApplyKernel(VolumeView V, DnApplyArgs a
       const float yv = a.Y[(static_cast<size_t>(b) * 512 + vi) * kDnOut + d];
       const uint16_t* xr = a.X + (static_cast<size_t>(b) * kGridVox + interiorIndex(vi)) * kDnIn;
       if (fabsf(yv - bf2f(xr[d])) > a.max_delta) continue;
-      const F3 n = unpackNormalOct(blk[fieldOffset(d, 3, s) + vi]);
+      const F3 n = unpackNormalOct(blk[poolWord(d, 3, s, vi)]);
       const float yc = sclamp(yv, -1.0f, 1.0f);
-      blk[fieldOffset(d, 2, s) + vi] = __float_as_uint(dot3(n, centre) - yc * a.trunc);
+      blk[poolWord(d, 2, s, vi)] = __float_as_uint(dot3(n, centre) - yc * a.trunc);
       ++replaced;
     }
   }

@@ -75,26 +75,24 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
   if (o.dir < 0) return;
   ++t.updates;
   const int d = o.dir;
-  uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
+  // the voxel's 3 slots of direction d: 48 contiguous bytes, 3 x 16-byte loads
+  uint4* rec = reinterpret_cast<uint4*>(V.pool + static_cast<size_t>(p) * kBlockWords + poolWord(d, 0, 0, vi));
+  uint4 old[3];
   Slots L;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    L.w[k] = __uint_as_float(blk[fieldOffset(d, 0, k) + vi]);
-    L.sc[k] = __uint_as_float(blk[fieldOffset(d, 1, k) + vi]);
-    L.po[k] = __uint_as_float(blk[fieldOffset(d, 2, k) + vi]);
-    L.pn[k] = blk[fieldOffset(d, 3, k) + vi];
+    old[k] = rec[k];
+    L.w[k] = __uint_as_float(old[k].x);
+    L.sc[k] = __uint_as_float(old[k].y);
+    L.po[k] = __uint_as_float(old[k].z);
+    L.pn[k] = old[k].w;
   }
-  const Slots L0 = L;
   if (!integrateSample(L, o.coord, o.pw, o.nw, o.weight, a.merge_tol, a.min_cos, a.max_weight)) return;
-  // write back only the words that changed (usually the one updated slot)
+  // write back only the slots that changed (usually the one updated slot)
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    if (__float_as_uint(L.w[k]) != __float_as_uint(L0.w[k])) blk[fieldOffset(d, 0, k) + vi] = __float_as_uint(L.w[k]);
-    if (__float_as_uint(L.sc[k]) != __float_as_uint(L0.sc[k]))
-      blk[fieldOffset(d, 1, k) + vi] = __float_as_uint(L.sc[k]);
-    if (__float_as_uint(L.po[k]) != __float_as_uint(L0.po[k]))
-      blk[fieldOffset(d, 2, k) + vi] = __float_as_uint(L.po[k]);
-    if (L.pn[k] != L0.pn[k]) blk[fieldOffset(d, 3, k) + vi] = L.pn[k];
+    const uint4 now = make_uint4(__float_as_uint(L.w[k]), __float_as_uint(L.sc[k]), __float_as_uint(L.po[k]), L.pn[k]);
+    if (now.x != old[k].x || now.y != old[k].y || now.z != old[k].z || now.w != old[k].w) rec[k] = now;
   }
   // observed bitmask (keeps observedVoxelCount O(1)), dirty flag and
   // direction mask for incremental meshing
@@ -261,7 +259,7 @@ __global__ __launch_bounds__(256) void gatherBlocksAoSKernel(const int32_t* __re
     const uint32_t field = i & 3u, rest = i >> 2;
     const uint32_t slotk = rest % 3u, rest2 = rest / 3u;
     const uint32_t dir = rest2 % 6u, voxel = rest2 / 6u;
-    dst[i] = src ? src[fieldOffset(dir, field, slotk) + voxel] : 0u;
+    dst[i] = src ? src[poolWord(dir, field, slotk, voxel)] : 0u;
   }
 }
 

@@ -66,8 +66,13 @@ struct VolumeView {
   DevHeader* hdr;
 };
 
-VHM_HD size_t fieldOffset(int dir, int field, int slot) {
-  return ((static_cast<size_t>(dir) * kFields + field) * kSlots + slot) * kBlockVoxels;
+// Pool word of (direction, field, slot) of voxel vi within a block.  Layout
+// [dir][voxel][slot][field]: the 3 slots of one direction of a voxel are 48
+// contiguous, 16-byte aligned bytes (fields: weight, surface_coord,
+// plane_offset, packed_normal), so a voxel update reads and writes whole
+// 16-byte slots, and a direction of a block is one 24 KiB run.
+VHM_HD size_t poolWord(int dir, int field, int slot, int vi) {
+  return ((static_cast<size_t>(dir) * kBlockVoxels + vi) * kSlots + slot) * kFields + field;
 }
 
 struct FilterArgs {

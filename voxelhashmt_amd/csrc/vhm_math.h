@@ -178,7 +178,17 @@ VHM_HD uint32_t packNormalOct(F3 raw) {
   return static_cast<uint32_t>(lroundi(qx * 65535.0f)) | (static_cast<uint32_t>(lroundi(qy * 65535.0f)) << 16u);
 }
 
-VHM_HD float dequantOct(uint32_t c) { return (static_cast<float>(c & 0xffffu) / 65535.0f) * 2.0f - 1.0f; }
+// (c / 65535.0f) * 2 - 1 for a 16-bit code c.  The quotient is formed as
+// q = c * RN(1/65535), corrected once with the exact residual
+// fma(-q, 65535, c): equal to the correctly rounded c / 65535.0f for all
+// 65,536 codes (checked exhaustively against the division on the host).
+VHM_HD float dequantOct(uint32_t c) {
+  const float x = static_cast<float>(c & 0xffffu);
+  constexpr float kInv = 1.0f / 65535.0f;  // RN(1/65535)
+  const float q = x * kInv;
+  const float quot = __builtin_fmaf(__builtin_fmaf(-q, 65535.0f, x), kInv, q);
+  return quot * 2.0f - 1.0f;
+}
 
 VHM_HD F3 unpackNormalOct(uint32_t packed) {
   F3 n{dequantOct(packed), dequantOct(packed >> 16u), 0.0f};

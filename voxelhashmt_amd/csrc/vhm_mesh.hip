@@ -104,16 +104,18 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
     if (p != kInvalid) {
       b = kBitAllocated;
       const int vi = (x & 7) + kBlockSide * ((y & 7) + kBlockSide * (z & 7));
-      const uint32_t* blk = V.pool + static_cast<size_t>(p) * kBlockWords;
+      // the voxel's 3 slots of direction dir: 48 contiguous bytes
+      const uint4* rec = reinterpret_cast<const uint4*>(V.pool + static_cast<size_t>(p) * kBlockWords +
+                                                        poolWord(dir, 0, 0, vi));
       const F3 centre{(static_cast<float>(bx + x) + 0.5f) * a.voxel, (static_cast<float>(by + y) + 0.5f) * a.voxel,
                       (static_cast<float>(bz + z) + 0.5f) * a.voxel};
 #pragma unroll
       for (int k = 0; k < kSlots; ++k) {
-        const float w = __uint_as_float(blk[fieldOffset(dir, 0, k) + vi]);
-        if (w > 0.0f) {
+        const uint4 slot = rec[k];  // weight, surface_coord, plane_offset, packed_normal
+        if (__uint_as_float(slot.x) > 0.0f) {
           b |= static_cast<uint8_t>(1u << k);
-          const F3 n = unpackNormalOct(blk[fieldOffset(dir, 3, k) + vi]);
-          const float po = __uint_as_float(blk[fieldOffset(dir, 2, k) + vi]);
+          const F3 n = unpackNormalOct(slot.w);
+          const float po = __uint_as_float(slot.z);
           const float v = dot3(n, centre) - po;
           S.nx[k][r] = n.x;
           S.ny[k][r] = n.y;
@@ -249,18 +251,34 @@ __device__ __forceinline__ bool cubeMayCross(const MeshShared& S, int cube) {
   return (span & (kBitBelow | kBitAbove)) == (kBitBelow | kBitAbove);
 }
 
+// Sorting network for the 24 candidate slots (8 corners x 3 slots): Batcher's
+// odd-even merge sort of 32 inputs without the comparators that touch inputs
+// 24..31 (those would hold +inf and never move).  Checked exhaustively by the
+// 0-1 principle on the host when the table was generated.
+constexpr int kSortPairs = 132;
+constexpr uint8_t kSortNet[kSortPairs][2] = {
+    {0,1}, {2,3}, {0,2}, {1,3}, {1,2}, {4,5}, {6,7}, {4,6}, {5,7}, {5,6}, {0,4}, {2,6}, {2,4}, {1,5}, {3,7}, {3,5},
+    {1,2}, {3,4}, {5,6}, {8,9}, {10,11}, {8,10}, {9,11}, {9,10}, {12,13}, {14,15}, {12,14}, {13,15}, {13,14},
+    {8,12}, {10,14}, {10,12}, {9,13}, {11,15}, {11,13}, {9,10}, {11,12}, {13,14}, {0,8}, {4,12}, {4,8}, {2,10},
+    {6,14}, {6,10}, {2,4}, {6,8}, {10,12}, {1,9}, {5,13}, {5,9}, {3,11}, {7,15}, {7,11}, {3,5}, {7,9}, {11,13},
+    {1,2}, {3,4}, {5,6}, {7,8}, {9,10}, {11,12}, {13,14}, {16,17}, {18,19}, {16,18}, {17,19}, {17,18}, {20,21},
+    {22,23}, {20,22}, {21,23}, {21,22}, {16,20}, {18,22}, {18,20}, {17,21}, {19,23}, {19,21}, {17,18}, {19,20},
+    {21,22}, {18,20}, {19,21}, {17,18}, {19,20}, {21,22}, {0,16}, {8,16}, {4,20}, {12,20}, {4,8}, {12,16}, {2,18},
+    {10,18}, {6,22}, {14,22}, {6,10}, {14,18}, {2,4}, {6,8}, {10,12}, {14,16}, {18,20}, {1,17}, {9,17}, {5,21},
+    {13,21}, {5,9}, {13,17}, {3,19}, {11,19}, {7,23}, {15,23}, {7,11}, {15,19}, {3,5}, {7,9}, {11,13}, {15,17},
+    {19,21}, {1,2}, {3,4}, {5,6}, {7,8}, {9,10}, {11,12}, {13,14}, {15,16}, {17,18}, {19,20}, {21,22}};
+
 // One cube of extractDirectionalMultiLayerMesh (voxel_hash_tsdf.h:853-935)
 // that passed cubeMayCross.  Returns the triangle count; with kEmit also
 // writes them at `out`.
 //
 // Three steps keep the wavefront's lanes in step: (1) the candidate plane
-// distances at the cell centre, kept per (corner, slot) in registers and
-// insertion-sorted into the lane's LDS column (std::sort,
-// voxel_hash_tsdf.h:887); (2) the greedy clustering (888-907), a cheap pass
-// that overwrites the column with the cluster centres; (3) per cluster, the
-// closest slot of each corner, the corner values, completion and the
-// triangles (909-934), in ascending cluster order as the reference
-// processes them.
+// distances at the cell centre, per (corner, slot) in registers (+inf for
+// none), sorted by the network above (std::sort, voxel_hash_tsdf.h:887: only
+// the values matter, equal values being interchangeable); (2) the greedy
+// clustering (888-907), whose centres go to the lane's LDS column; (3) per
+// cluster in ascending order, the closest slot of each corner, the corner
+// values, completion and the triangles (909-934).
 template <bool kEmit>
 __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
                                                 float* out) {
@@ -278,6 +296,7 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
   // (1) candidates
   constexpr float kNone = __builtin_huge_valf();
   float sc[8][kSlots];
+  float srt[8 * kSlots];
   int nc = 0;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -287,34 +306,39 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
       float s = kNone;
       if ((b & kBitAllocated) && (b & (1u << k))) {
         s = planeSD(S, k, rc[c], cc);
-        int j = nc;
-        while (j > 0 && S.cand[j - 1][tid] > s) {
-          S.cand[j][tid] = S.cand[j - 1][tid];
-          --j;
-        }
-        S.cand[j][tid] = s;
         ++nc;
       }
       sc[c][k] = s;
+      srt[c * kSlots + k] = s;
     }
   }
   if (nc == 0) return 0;
+#pragma clang loop unroll(full)
+  for (int q = 0; q < kSortPairs; ++q) {
+    const int i = kSortNet[q][0], j = kSortNet[q][1];
+    const float x = srt[i], y = srt[j];
+    const bool sw = y < x;
+    srt[i] = sw ? y : x;
+    srt[j] = sw ? x : y;
+  }
 
   // (2) clusters: a candidate joins the open cluster unless it lies more than
-  // extract_tol from its running mean; centres overwrite the column (cluster
-  // j's centre is written only after candidate j has been read).
+  // extract_tol from its running mean
   int ncl = 0;
-  float centre = S.cand[0][tid];
+  float centre = srt[0];
   int count = 1;
-  for (int i = 1; i < nc; ++i) {
-    const float sk = S.cand[i][tid];
-    if (!(fabsf(sk - centre) > a.extract_tol)) {
-      centre = (centre * static_cast<float>(count) + sk) / static_cast<float>(count + 1);
-      ++count;
-    } else {
-      S.cand[ncl++][tid] = centre;
-      centre = sk;
-      count = 1;
+#pragma clang loop unroll(full)
+  for (int i = 1; i < 8 * kSlots; ++i) {
+    if (i < nc) {
+      const float sk = srt[i];
+      if (!(fabsf(sk - centre) > a.extract_tol)) {
+        centre = (centre * static_cast<float>(count) + sk) / static_cast<float>(count + 1);
+        ++count;
+      } else {
+        S.cand[ncl++][tid] = centre;
+        centre = sk;
+        count = 1;
+      }
     }
   }
   S.cand[ncl++][tid] = centre;
