@@ -293,7 +293,7 @@ def main():
     # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by a plain copy
     # kernel's reading), times this run's updates per launch
     traffic = None
-    tfile = ROOT / "profiles" / "integrate_traffic.json"
+    tfile = ROOT / "profiles" / "integrateKernel_plain_scale_1280x720_traffic.json"
     if tfile.exists():
         try:
             bpu = json.loads(tfile.read_text())["bytes_per_update"]

@@ -237,7 +237,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-scenes", type=int, default=16)
-    ap.add_argument("--eval-seed", type=int, default=5000)
+    ap.add_argument("--eval-seed", type=int, default=2000)
     ap.add_argument("--eval-frames-plain", type=int, default=300)
     ap.add_argument("--eval-frames-learned", type=int, default=30)
     ap.add_argument("--params-in", default=None, help="skip training: load parameters (.npy float32)")
