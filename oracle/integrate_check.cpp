@@ -24,6 +24,10 @@ NormalParams makeParams(int W, int H, const float* K4, const float* p7) {
   a.spatial_w[2] = p7[4];
   a.inv_r = p7[5];
   a.inv_a = p7[6];
+  a.inv_fx = 1.0f / a.K.fx;
+  a.inv_fy = 1.0f / a.K.fy;
+  a.exact_fx = reciprocalDivisionExact(a.K.fx) ? 1 : 0;
+  a.exact_fy = reciprocalDivisionExact(a.K.fy) ? 1 : 0;
   return a;
 }
 

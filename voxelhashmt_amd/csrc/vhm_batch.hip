@@ -201,6 +201,8 @@ int vhm_batch_set_sequence(vhm_batch* b, int scene, const uint16_t* frames_devic
   for (int s = 0; s < 3; ++s) sc.ia.spatial_w[s] = std::exp(-static_cast<float>(s) * inv_n);
   sc.ia.inv_r = invTwoSigmaSq(P.normal_sigma_range);
   sc.ia.inv_a = invTwoSigmaSq(P.normal_sigma_angular);
+  focalDivision(k.fx, &sc.ia.inv_fx, &sc.ia.exact_fx);
+  focalDivision(k.fy, &sc.ia.inv_fy, &sc.ia.exact_fy);
   sc.ma.voxel = P.voxel_size;
   sc.ma.iso = 0.0f;
   sc.ma.extract_tol = P.voxel_size * P.layer_extract_scale;

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -241,6 +243,8 @@ int integrateStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, 
   for (int s = 0; s < 3; ++s) ia.spatial_w[s] = std::exp(-static_cast<float>(s) * inv_s);
   ia.inv_r = invTwoSigmaSq(c->P.normal_sigma_range);
   ia.inv_a = invTwoSigmaSq(c->P.normal_sigma_angular);
+  focalDivision(ia.K.fx, &ia.inv_fx, &ia.exact_fx);
+  focalDivision(ia.K.fy, &ia.inv_fy, &ia.exact_fy);
   hipEvent_t ev{};
   beginTimed(c, 2, &ev);
   const hipError_t e = launchIntegrate(dev_depth, view(c), ia, c->num_cus * 8, c->stream);
@@ -825,6 +829,17 @@ void vhm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 }  // extern "C"
+
+void vhm::focalDivision(float f, float* inv, int* exact) {
+  static std::mutex mu;
+  static std::unordered_map<uint32_t, int> known;
+  *inv = 1.0f / f;
+  const uint32_t key = __builtin_bit_cast(uint32_t, f);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = known.find(key);
+  if (it == known.end()) it = known.emplace(key, reciprocalDivisionExact(f) ? 1 : 0).first;
+  *exact = it->second;
+}
 
 // Context internals for the other host translation units (vhm_internal.h).
 vhm::VolumeView vhmCtxView(vhm_ctx* c) { return view(c); }

@@ -35,6 +35,10 @@ __device__ __forceinline__ NormalParams normalParams(const IntegrateArgs& a) {
   np.spatial_w[2] = a.spatial_w[2];
   np.inv_r = a.inv_r;
   np.inv_a = a.inv_a;
+  np.inv_fx = a.inv_fx;
+  np.inv_fy = a.inv_fy;
+  np.exact_fx = a.exact_fx;
+  np.exact_fy = a.exact_fy;
   return np;
 }
 
@@ -185,7 +189,8 @@ __device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& 
 
 // Workgroup w integrates the contiguous chunk of the active list
 // [w * c, (w + 1) * c), c = ceil(n / grid).
-__global__ __launch_bounds__(256) void integrateKernel(const float* __restrict__ depth, VolumeView V,
+// 4 wavefronts per SIMD: the allocator lands just above 128 VGPRs without the hint.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void integrateKernel(const float* __restrict__ depth, VolumeView V,
                                                        IntegrateArgs a) {
   __shared__ IntegrateQueue Q;
   if (threadIdx.x == 0) Q.n = 0;

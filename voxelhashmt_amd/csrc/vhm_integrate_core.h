@@ -31,7 +31,19 @@ struct NormalParams {
   float disc_base, disc_rel;
   float spatial_w[3];  // exp(-s * inv_s) for s = du^2 + dv^2 in {0, 1, 2}
   float inv_r, inv_a;
+  // RN(1 / fx), RN(1 / fy) and whether divByConst by fx / fy is exact
+  // (reciprocalDivisionExact); otherwise the plain division is used
+  float inv_fx, inv_fy;
+  int exact_fx, exact_fy;
 };
+
+// backProjectContinuous (voxel_hash_tsdf.h:586-593) with the divisions by the
+// focal lengths through divByConst when that is exact for them.
+VHM_HD F3 backProjectN(float u, float v, float d, const NormalParams& a) {
+  const float xn = (u - a.K.cx) * d, yn = (v - a.K.cy) * d;
+  return {a.exact_fx ? divByConst(xn, a.K.fx, a.inv_fx) : xn / a.K.fx,
+          a.exact_fy ? divByConst(yn, a.K.fy, a.inv_fy) : yn / a.K.fy, d};
+}
 
 template <class DepthT>
 struct DepthView {
@@ -106,8 +118,8 @@ VHM_HD bool baseNormal(const D& dv, const NormalParams& a, float u, float v, F3&
   if (!sampleBilinear(dv, a, u - 1.0f, v, dl) || !sampleBilinear(dv, a, u + 1.0f, v, dr) ||
       !sampleBilinear(dv, a, u, v - 1.0f, du) || !sampleBilinear(dv, a, u, v + 1.0f, dd))
     return false;
-  const F3 pl = backProjectCont(u - 1.0f, v, dl, a.K), pr = backProjectCont(u + 1.0f, v, dr, a.K);
-  const F3 pu = backProjectCont(u, v - 1.0f, du, a.K), pd = backProjectCont(u, v + 1.0f, dd, a.K);
+  const F3 pl = backProjectN(u - 1.0f, v, dl, a), pr = backProjectN(u + 1.0f, v, dr, a);
+  const F3 pu = backProjectN(u, v - 1.0f, du, a), pd = backProjectN(u, v + 1.0f, dd, a);
   n = normalized3(cross3(sub3(pd, pu), sub3(pr, pl)));
   return norm3(n) > 1e-6f;
 }
@@ -118,7 +130,7 @@ template <class D>
 VHM_HD bool surfaceNormalReference(const D& dv, const NormalParams& a, float u, float v, float cd, F3& n) {
   F3 cn;
   if (!baseNormal(dv, a, u, v, cn)) return false;
-  const F3 cp = backProjectCont(u, v, cd, a.K);
+  const F3 cp = backProjectN(u, v, cd, a);
   cn = orientTowardCamera(cn, cp);
   F3 acc{0.0f, 0.0f, 0.0f};
   float ws = 0.0f;
@@ -129,7 +141,7 @@ VHM_HD bool surfaceNormalReference(const D& dv, const NormalParams& a, float u, 
       if (!baseNormal(dv, a, su, sv, sn)) continue;
       float sdep = 0.0f;
       if (!sampleBilinear(dv, a, su, sv, sdep)) continue;
-      const F3 sp = backProjectCont(su, sv, sdep, a.K);
+      const F3 sp = backProjectN(su, sv, sdep, a);
       sn = orientTowardCamera(sn, sp);
       const float delta = sdep - cd;
       const float c = sclamp(dot3(cn, sn), -1.0f, 1.0f);
@@ -218,7 +230,7 @@ VHM_HD bool surfaceNormalFast(const D& dv, const NormalParams& a, float u, float
     for (int a2 = -2; a2 <= 2; ++a2) {
       if ((a2 == -2 || a2 == 2) && (b == -2 || b == 2)) continue;
       const int t = tapIndex(a2, b);
-      Pt[t] = backProjectCont(X[a2 + 2], Y[b + 2], S[t], a.K);
+      Pt[t] = backProjectN(X[a2 + 2], Y[b + 2], S[t], a);
     }
   // base normals of the 9 neighbours (du, dv) = (i - 1, j - 1)
   F3 bn[9];
@@ -418,7 +430,7 @@ VHM_HD VoxelObservation observeSurface(const D& dv, const NormalParams& a, const
   if (surfaceNormal(dv, a, u, v, dep, nc)) vaw = fabsf(dot3(nc, ray));
   const float w = observationWeight(dep, sd, trunc, vaw, min_view);
   if (w <= 0.0f) return o;
-  const F3 spw = transformPoint(T_wc, backProjectCont(u, v, dep, a.K));
+  const F3 spw = transformPoint(T_wc, backProjectN(u, v, dep, a));
   const F3 nw = normalized3(rotate(T_wc, nc));
   const int d = dominantDirection(nw);
   if (d < 0) return o;

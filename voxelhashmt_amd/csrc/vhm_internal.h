@@ -99,7 +99,13 @@ struct IntegrateArgs {
   float disc_base, disc_rel;
   float spatial_w[3];  // normal bilateral exp(-s * inv_s), s = 0, 1, 2
   float inv_r, inv_a;
+  float inv_fx, inv_fy;  // RN(1 / fx), RN(1 / fy)
+  int exact_fx, exact_fy;  // divByConst by fx / fy is exact (host-checked)
 };
+
+// RN(1 / f) and whether divByConst by f equals the division for every
+// dividend (reciprocalDivisionExact, cached per value; defined in vhm_capi.hip).
+void focalDivision(float f, float* inv, int* exact);
 
 struct MeshArgs {
   float voxel, iso, extract_tol;

@@ -150,10 +150,45 @@ VHM_HD bool projectPoint(F3 p, const Intrinsics& K, float& u, float& v) {
 
 VHM_HD float discontinuityThreshold(float d, float base, float rel) { return smax(base, rel * smax(d, 0.0f)); }
 
+// orientNormalTowardCamera (voxel_hash_tsdf.h:230-237).  Only the sign of
+// dot(n, p / |p|) matters.  Rounding moves that dot product by at most
+// 4.01 u sum|n_i p_i| (u = 2^-24: the three quotients and the dot's own
+// roundings; the common positive factor 1 / |p| does not change its sign),
+// so when the plain dot(n, p) exceeds 1e-5 sum|n_i p_i| -- far above that
+// bound plus its own rounding -- the signs agree and the divisions are
+// skipped; otherwise the reference sequence runs as written.
 VHM_HD F3 orientTowardCamera(F3 n, F3 p) {
   if (norm3(n) <= 1e-6f || norm3(p) <= 1e-6f) return n;
+  const float d1 = dot3(n, p);
+  const float t = fabsf(n.x * p.x) + fabsf(n.y * p.y) + fabsf(n.z * p.z);
+  if (t > 1e-30f && fabsf(d1) > 1e-5f * t) return d1 > 0.0f ? mul3(n, -1.0f) : n;
   const F3 ray = normalized3(p);
   return dot3(n, ray) > 0.0f ? mul3(n, -1.0f) : n;
+}
+
+// x / d for a divisor d fixed over many calls, from inv = RN(1 / d): the
+// product x * inv corrected once by the exact residual fma(-q, d, x).  Equal
+// to the correctly rounded x / d for every x whenever
+// reciprocalDivisionExact(d) holds (checked once per divisor on the host);
+// tiny |x| takes the plain division (the check covers normal quotients).
+VHM_HD float divByConst(float x, float d, float inv) {
+  if (fabsf(x) < 1e-30f) return x / d;
+  const float q = x * inv;
+  const float e = __builtin_fmaf(-q, d, x);
+  return e == 0.0f ? q : __builtin_fmaf(e, inv, q);
+}
+
+// Host check for divByConst: every mantissa m in [1, 2) (the quotient of
+// m * 2^k is the quotient of m scaled by 2^k while it stays normal, and the
+// sign is symmetric).  About 8.4 M divisions; callers cache the answer.
+inline bool reciprocalDivisionExact(float d) {
+  if (!(d == d) || d == 0.0f || d * 0.0f != 0.0f) return false;  // NaN, zero, inf
+  const float inv = 1.0f / d;
+  for (uint32_t m = 0; m < (1u << 23); ++m) {
+    const float x = __builtin_bit_cast(float, 0x3f800000u | m);
+    if (__builtin_bit_cast(uint32_t, divByConst(x, d, inv)) != __builtin_bit_cast(uint32_t, x / d)) return false;
+  }
+  return true;
 }
 
 VHM_HD float gaussianWeight(float sq, float inv_two_sigma_sq) { return expf_glibc(-sq * inv_two_sigma_sq); }
