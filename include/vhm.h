@@ -164,20 +164,27 @@ int vhm_download_blocks(vhm_ctx* ctx, const int32_t* keys, size_t n, void* out_h
  * capacity 0 only counts.  Synchronous. */
 int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capacity, size_t* n_out);
 
-/* Incremental extraction over the blocks changed since the previous
- * incremental call: every block the integrator wrote (its dirty flag) plus
- * its -x/-y/-z neighbours, whose cubes read it.  Re-meshes that subset only
- * and clears the dirty flags.  Outputs the re-meshed block keys (pool
- * order) and their triangles; tri_block_out[i] is the index (into
- * block_keys_out) of the block owning triangle i.  A caller keeping one mesh
- * per block replaces exactly those blocks.  No reference counterpart (the
- * reference always meshes the whole volume); the per-block output equals
- * vhm_extract_mesh's for those blocks.  block_keys_out is host memory;
- * tris_out / tri_block_out are in out_memkind memory (VHM_DEVICE: written in
- * place on the GPU, nothing copied to the host).  Synchronous. */
-int vhm_extract_mesh_incremental(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out, size_t block_capacity,
-                                 size_t* n_blocks_out, float* tris_out, uint32_t* tri_block_out,
-                                 size_t tri_capacity, int out_memkind, size_t* n_tris_out);
+/* Incremental extraction over the (block, direction) pairs changed since the
+ * previous incremental call: every direction the integrator (or the
+ * denoiser) wrote in a block, in that block and in its -x/-y/-z neighbours,
+ * whose cubes read it.  Re-meshes that subset only and clears the dirty
+ * masks.  Outputs the listed blocks (keys in pool order, and per block the
+ * bit mask of directions re-meshed) and the triangles of those (block,
+ * direction) items: tri_item_out[t] = d * n_blocks + i for triangle t of
+ * direction d of listed block i.  A caller keeping one mesh per (block,
+ * direction) replaces exactly the items in the masks.  No reference
+ * counterpart (the reference always meshes the whole volume); each item's
+ * triangles equal vhm_extract_mesh's for that block and direction.
+ * block_keys_out / block_dirs_out are host memory; tris_out / tri_item_out
+ * are in out_memkind memory (VHM_DEVICE: written in place on the GPU,
+ * nothing copied to the host).  Synchronous. */
+int vhm_extract_mesh_incremental(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out, uint8_t* block_dirs_out,
+                                 size_t block_capacity, size_t* n_blocks_out, float* tris_out,
+                                 uint32_t* tri_item_out, size_t tri_capacity, int out_memkind, size_t* n_tris_out);
+/* Copies again the triangles (and items) of the last incremental extraction
+ * that used VHM_HOST output -- all of them, min(capacity, n), for a caller
+ * whose first buffer was too small; *n_out = n.  Synchronous. */
+int vhm_incremental_mesh_copy(vhm_ctx* ctx, float* tris_out, uint32_t* tri_item_out, size_t capacity, size_t* n_out);
 
 /* Per-kernel GPU time (ms, HIP events on the context stream) accumulated
  * while timing is enabled: [0] filter, [1] allocate, [2] integrate, [3] mesh;

@@ -66,7 +66,7 @@ def _run(torch, use_graph, autotune):
             for i, (w, frames) in enumerate(wl):
                 if step < len(frames):
                     singles[i].integrate(frames[step], w.K, w.poses[step])
-                keys, tris, _ = singles[i].extract_mesh_incremental()
+                keys, _, tris, _ = singles[i].extract_mesh_incremental()
                 nb, nt, btris = b.mesh_output(i)
                 assert nb == len(keys), f"scene {i} step {step}: re-meshed blocks {nb} vs {len(keys)}"
                 assert nt == len(tris), f"scene {i} step {step}: triangles {nt} vs {len(tris)}"

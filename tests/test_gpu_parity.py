@@ -203,23 +203,30 @@ def test_determinism(gpu):
 
 
 def test_incremental_mesh_equals_full_extraction(gpu):
-    """Keeping one mesh per block and replacing the blocks each incremental
-    call re-meshes gives, after every call, the full extraction's mesh."""
+    """Keeping one mesh per (block, direction) and replacing the items each
+    incremental call re-meshes gives, after every call, the full
+    extraction's mesh; a call with nothing written lists nothing."""
     w = synth.tiny(n_frames=6, seed=12)
     frames = synth.render(w.scene, w.K, w.poses, w.noise)
     with vhm.Volume(_params(w)) as vol:
-        per_block = {}
+        per_item = {}
         for i in range(6):
             vol.integrate(frames[i], w.K, w.poses[i])
             if i % 2 == 1:
-                keys, tris, owner = vol.extract_mesh_incremental()
-                print(f"[incremental] frame {i}: {len(keys)} blocks re-meshed, {len(tris)} triangles", flush=True)
-                assert len(keys) > 0
+                keys, dirs, tris, items = vol.extract_mesh_incremental()
+                m = len(keys)
+                n_items = int(sum(bin(int(d)).count("1") for d in dirs))
+                print(f"[incremental] frame {i}: {m} blocks, {n_items} (block, direction) items re-meshed, "
+                      f"{len(tris)} triangles", flush=True)
+                assert m > 0 and n_items < 6 * m  # some directions of listed blocks were not re-meshed
+                assert np.all(items // m < 6) and np.all(dirs[items % m] & (1 << (items // m)).astype(np.uint8))
                 for j, k in enumerate(keys):
-                    per_block[tuple(k)] = tris[owner == j]
-                union = np.concatenate([t for t in per_block.values() if len(t)]) if per_block else np.zeros((0, 3, 3))
+                    for d in range(6):
+                        if dirs[j] & (1 << d):
+                            per_item[(tuple(k), d)] = tris[items == d * m + j]
+                union = np.concatenate([t for t in per_item.values() if len(t)]) if per_item else np.zeros((0, 3, 3))
                 assert np.array_equal(O.canonical_mesh(union), O.canonical_mesh(vol.extract_mesh()))
-        keys, tris, owner = vol.extract_mesh_incremental()
+        keys, dirs, tris, items = vol.extract_mesh_incremental()
         assert len(keys) == 0 and len(tris) == 0  # nothing written since the last call
 
 

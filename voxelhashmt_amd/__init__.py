@@ -93,9 +93,11 @@ _SIGS = {
     "vhm_download_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "vhm_observed_block_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vhm_extract_mesh": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
-    "vhm_extract_mesh_incremental": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t,
+    "vhm_extract_mesh_incremental": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_size_t,
                                                C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
                                                C.POINTER(C.c_size_t)]),
+    "vhm_incremental_mesh_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                            C.POINTER(C.c_size_t)]),
     "vhm_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "vhm_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "vhm_synth_render": (C.c_int, [C.POINTER(Scene), C.POINTER(Intrinsics), C.POINTER(Pose), C.c_int, C.c_uint32,
@@ -349,32 +351,36 @@ class Volume:
             self._check(lib().vhm_extract_mesh(self._h, iso_level, out.ctypes.data, n.value, C.byref(m)))
         return out[: n.value]
 
-    def extract_mesh_incremental(self, iso_level: float = 0.0, device_tris=None, device_owner=None):
-        """Re-meshes the blocks changed since the last incremental call.
-        Returns (block_keys (m,3), triangles (n,3,3), owner (n,) indices into
-        block_keys).  With device_tris (a torch CUDA float tensor of n_max*9
-        elements) and device_owner (int32, n_max) the triangles stay on the GPU
-        and (m, n) is returned."""
+    def extract_mesh_incremental(self, iso_level: float = 0.0, device_tris=None, device_items=None):
+        """Re-meshes the (block, direction) pairs changed since the last
+        incremental call.  Returns (block_keys (m,3), dirs (m,) direction bit
+        masks, triangles (n,3,3), items (n,): d * m + i for direction d of
+        listed block i).  With device_tris (a torch CUDA float tensor of
+        n_max*9 elements) and device_items (int32, n_max) the triangles stay
+        on the GPU and (m, n) is returned."""
         nb, nt = C.c_size_t(), C.c_size_t()
         if device_tris is not None:
             cap = device_tris.numel() // 9
             self._check(lib().vhm_extract_mesh_incremental(
-                self._h, iso_level, None, 0, C.byref(nb), device_tris.data_ptr(),
-                None if device_owner is None else device_owner.data_ptr(), cap, VHM_DEVICE, C.byref(nt)))
+                self._h, iso_level, None, None, 0, C.byref(nb), device_tris.data_ptr(),
+                None if device_items is None else device_items.data_ptr(), cap, VHM_DEVICE, C.byref(nt)))
             return nb.value, nt.value
         cap_b = max(self.block_count(), 1)
         keys = np.zeros((cap_b, 3), np.int32)
-        # triangle capacity: at most 5 per cube and cluster; grow and retry is not
-        # possible (the call consumes the dirty flags), so size generously
-        cap_t = cap_b * 6 * 512 * 2
+        dirs = np.zeros(cap_b, np.uint8)
+        cap_t = 1 << 20
         tris = np.zeros((cap_t, 3, 3), np.float32)
-        owner = np.zeros(cap_t, np.uint32)
-        self._check(lib().vhm_extract_mesh_incremental(self._h, iso_level, keys.ctypes.data, cap_b, C.byref(nb),
-                                                       tris.ctypes.data, owner.ctypes.data, cap_t, VHM_HOST,
-                                                       C.byref(nt)))
-        if nt.value > cap_t:
-            raise VhmError(6, "incremental mesh larger than the host buffer")
-        return keys[: nb.value].copy(), tris[: nt.value].copy(), owner[: nt.value].copy()
+        items = np.zeros(cap_t, np.uint32)
+        self._check(lib().vhm_extract_mesh_incremental(self._h, iso_level, keys.ctypes.data, dirs.ctypes.data, cap_b,
+                                                       C.byref(nb), tris.ctypes.data, items.ctypes.data, cap_t,
+                                                       VHM_HOST, C.byref(nt)))
+        if nt.value > cap_t:  # fetch everything from the library's copy
+            tris = np.zeros((nt.value, 3, 3), np.float32)
+            items = np.zeros(nt.value, np.uint32)
+            n = C.c_size_t()
+            self._check(lib().vhm_incremental_mesh_copy(self._h, tris.ctypes.data, items.ctypes.data, nt.value,
+                                                        C.byref(n)))
+        return keys[: nb.value].copy(), dirs[: nb.value].copy(), tris[: nt.value].copy(), items[: nt.value].copy()
 
     def set_timing(self, enabled: bool):
         self._check(lib().vhm_set_timing(self._h, int(enabled)))

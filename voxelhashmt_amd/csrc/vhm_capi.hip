@@ -67,9 +67,11 @@ struct vhm_ctx {
   uint32_t* remesh_flags = nullptr;  // max_blocks, kept zero between calls
   uint32_t* remesh_pos = nullptr;
   uint32_t* remesh_list = nullptr;
+  uint8_t* remesh_dirs = nullptr;  // directions to re-mesh per listed block
   uint32_t* remesh_partials = nullptr;
   uint32_t* tri_owner = nullptr;
   size_t tri_owner_cap = 0;
+  size_t inc_tris = 0;  // triangles of the last host-output incremental extraction
   // timing
   bool timing = false;
   struct Ev {
@@ -417,6 +419,7 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->remesh_flags, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_pos, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_list, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->remesh_dirs, size_t(p->max_blocks));
   if (e == hipSuccess) e = hipMalloc(&c->remesh_partials, scanPartialsSize(p->max_blocks) * 4u);
   if (e != hipSuccess) {
     vhm_destroy(c);
@@ -440,7 +443,7 @@ int vhm_destroy(vhm_ctx* c) {
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
-                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts};
+                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -569,6 +572,21 @@ int vhm_block_keys(vhm_ctx* c, int32_t* keys_out, size_t capacity, size_t* n_out
   return VHM_OK;
 }
 
+int vhm_incremental_mesh_copy(vhm_ctx* c, float* tris_out, uint32_t* tri_item_out, size_t capacity, size_t* n_out) {
+  if (!c || !n_out) return VHM_ERR_INVALID_ARGUMENT;
+  *n_out = c->inc_tris;
+  const size_t t = capacity < c->inc_tris ? capacity : c->inc_tris;
+  if (t == 0) return VHM_OK;
+  if (tris_out)
+    VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, t * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H triangles");
+  if (tri_item_out)
+    VHM_TRY(c, hipMemcpyAsync(tri_item_out, c->tri_owner, t * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H triangle owners");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return VHM_OK;
+}
+
 int vhm_observed_block_keys(vhm_ctx* c, int32_t* keys_out, size_t capacity, size_t* n_out) {
   if (!c || !n_out) return VHM_ERR_INVALID_ARGUMENT;
   DevHeader h{};
@@ -579,8 +597,9 @@ int vhm_observed_block_keys(vhm_ctx* c, int32_t* keys_out, size_t capacity, size
   if (nb == 0) return VHM_OK;
   // the remesh flags are all zero between calls; the compaction clears them again
   VHM_TRY(c, launchObservedFlags(view(c), nb, c->remesh_flags, c->stream), "observed flags kernel launch");
-  VHM_TRY(c, launchScan(c->remesh_flags, c->remesh_pos, nb, c->remesh_partials, c->stream), "observed scan launch");
-  VHM_TRY(c, launchCompactFlags(c->remesh_flags, c->remesh_pos, nb, c->remesh_list, c->remesh_flags, c->stream),
+  VHM_TRY(c, launchScan(c->remesh_flags, c->remesh_pos, nb, c->remesh_partials, c->stream, true),
+          "observed scan launch");
+  VHM_TRY(c, launchCompactFlags(c->remesh_flags, c->remesh_pos, nb, c->remesh_list, nullptr, c->stream),
           "observed compact launch");
   uint32_t m = 0;
   VHM_TRY(c, hipMemcpyAsync(&m, c->remesh_partials + (scanPartialsSize(nb) - 1), sizeof(uint32_t),
@@ -645,7 +664,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
   hipEvent_t ev{};
   beginTimed(c, 3, &ev);
-  VHM_TRY(c, launchMeshCount(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchMeshCount(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, nullptr, c->stream), "mesh count kernel launch");
   VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
           "mesh scan kernel launch");
   uint32_t total = 0;
@@ -668,23 +687,26 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   return VHM_OK;
 }
 
-int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out, size_t block_capacity,
-                                 size_t* n_blocks_out, float* tris_out, uint32_t* tri_block_out, size_t tri_capacity,
-                                 int out_memkind, size_t* n_tris_out) {
+int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out, uint8_t* block_dirs_out,
+                                 size_t block_capacity, size_t* n_blocks_out, float* tris_out, uint32_t* tri_item_out,
+                                 size_t tri_capacity, int out_memkind, size_t* n_tris_out) {
   if (!c || !n_blocks_out || !n_tris_out) return VHM_ERR_INVALID_ARGUMENT;
   DevHeader h{};
   int rc = readHeader(c, &h);
   if (rc) return rc;
   *n_blocks_out = 0;
   *n_tris_out = 0;
+  c->inc_tris = 0;
   const uint32_t nb = h.num_blocks < c->P.max_blocks ? h.num_blocks : c->P.max_blocks;
   if (nb == 0) return VHM_OK;
   hipEvent_t ev{};
   beginTimed(c, 3, &ev);
-  // dirty blocks and their -x/-y/-z neighbours -> flags -> ordered list
+  // dirty (block, direction) masks, spread to the -x/-y/-z neighbours ->
+  // ordered list of blocks with their direction masks
   VHM_TRY(c, launchMarkRemesh(view(c), nb, c->remesh_flags, c->stream), "remesh mark kernel launch");
-  VHM_TRY(c, launchScan(c->remesh_flags, c->remesh_pos, nb, c->remesh_partials, c->stream), "remesh scan launch");
-  VHM_TRY(c, launchCompactFlags(c->remesh_flags, c->remesh_pos, nb, c->remesh_list, c->remesh_flags, c->stream),
+  VHM_TRY(c, launchScan(c->remesh_flags, c->remesh_pos, nb, c->remesh_partials, c->stream, true),
+          "remesh scan launch");
+  VHM_TRY(c, launchCompactFlags(c->remesh_flags, c->remesh_pos, nb, c->remesh_list, c->remesh_dirs, c->stream),
           "remesh compact launch");
   uint32_t m = 0;
   VHM_TRY(c, hipMemcpyAsync(&m, c->remesh_partials + (scanPartialsSize(nb) - 1), sizeof(uint32_t),
@@ -708,7 +730,7 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
   ma.iso = iso;
   ma.extract_tol = c->P.voxel_size * c->P.layer_extract_scale;
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
-  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->remesh_dirs, c->stream), "mesh count kernel launch");
   VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
           "mesh scan kernel launch");
   uint32_t total = 0;
@@ -724,23 +746,32 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
     VHM_TRY(c, hipMemcpyAsync(block_keys_out, c->key_buf, 3 * kb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
             "hipMemcpy D2H remesh keys");
   }
+  if (block_dirs_out && block_capacity > 0) {
+    const size_t kb = block_capacity < m ? block_capacity : m;
+    VHM_TRY(c, hipMemcpyAsync(block_dirs_out, c->remesh_dirs, kb, hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H remesh directions");
+  }
   if (total > 0 && tris_out && tri_capacity > 0 && out_memkind == VHM_DEVICE) {
     // straight into the caller's device buffers; triangles past the capacity are dropped
     const uint32_t cap = static_cast<uint32_t>(tri_capacity < total ? tri_capacity : total);
-    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, tris_out, tri_block_out, cap,
+    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, tris_out, tri_item_out, cap,
                               c->stream),
             "mesh emit kernel launch");
-  } else if (total > 0 && tris_out && tri_capacity > 0) {
+  } else if (total > 0 && out_memkind != VHM_DEVICE) {
+    // host output: emitted into the context's buffers (kept for
+    // vhm_incremental_mesh_copy), then min(capacity, total) copied
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
     if ((rc = ensure(c, &c->tri_owner, &c->tri_owner_cap, size_t(total), "hipMalloc triangle owners"))) return rc;
     VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, c->tri_owner, total,
                               c->stream),
             "mesh emit kernel launch");
+    c->inc_tris = total;
     const size_t t = tri_capacity < total ? tri_capacity : total;
-    VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, t * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
-            "hipMemcpy D2H triangles");
-    if (tri_block_out)
-      VHM_TRY(c, hipMemcpyAsync(tri_block_out, c->tri_owner, t * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+    if (tris_out && t)
+      VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, t * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+              "hipMemcpy D2H triangles");
+    if (tri_item_out && t)
+      VHM_TRY(c, hipMemcpyAsync(tri_item_out, c->tri_owner, t * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
               "hipMemcpy D2H triangle owners");
   }
   endTimed(c, ev);

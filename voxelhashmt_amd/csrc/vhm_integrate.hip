@@ -74,6 +74,7 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
   // observed bit this frame (every block is in the active list once)
   const uint32_t obs_word = V.obs[static_cast<size_t>(p) * 16u + (vi >> 5)];
   const uint32_t dmask = V.dirmask[p];
+  const uint32_t dirty_word = V.dirty[p];
   const F3 pc{Q.pcx[i], Q.pcy[i], Q.pcz[i]};
   const VoxelObservation o = observeSurface(D, np, a.T_wc, pc, Q.u[i], Q.v[i], Q.dep[i], a.trunc, a.min_view);
   if (o.dir < 0) return;
@@ -98,14 +99,14 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
     const uint4 now = make_uint4(__float_as_uint(L.w[k]), __float_as_uint(L.sc[k]), __float_as_uint(L.po[k]), L.pn[k]);
     if (now.x != old[k].x || now.y != old[k].y || now.z != old[k].z || now.w != old[k].w) rec[k] = now;
   }
-  // observed bitmask (keeps observedVoxelCount O(1)), dirty flag and
-  // direction mask for incremental meshing
+  // observed bitmask (keeps observedVoxelCount O(1)), and the block's dirty
+  // and observed direction masks for incremental meshing
   const uint32_t bit = 1u << (vi & 31u);
   if (!(obs_word & bit)) {
     atomicOr(&V.obs[static_cast<size_t>(p) * 16u + (vi >> 5)], bit);
     ++t.new_obs;
   }
-  V.dirty[p] = 1u;
+  if (!(dirty_word & (1u << d))) atomicOr(&V.dirty[p], 1u << d);
   if (!(dmask & (1u << d))) atomicOr(&V.dirmask[p], 1u << d);
 }
 

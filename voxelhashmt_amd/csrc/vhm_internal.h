@@ -124,16 +124,20 @@ hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeVi
 // counts[dir * n + i] and per-cube counts cube_counts[512 * (dir * n + i) ..]
 // (6 * n * 512 bytes), then triangles at the scanned offsets.
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           uint32_t* counts, uint8_t* cube_counts, hipStream_t s);
-hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s);
+                           uint32_t* counts, uint8_t* cube_counts, const uint8_t* dirs, hipStream_t s);
+// out[i] = sum(in[0..i)) (with flags: the count of non-zero in[0..i));
+// partials[(n + 1023) / 1024] receives the total.
+hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s,
+                      bool flags = false);
 size_t scanPartialsSize(uint32_t n);
 hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                           const uint32_t* counts, const uint8_t* cube_counts, const uint32_t* offsets, float* tris,
                           uint32_t* tri_owner, uint32_t capacity, hipStream_t s);
-// Incremental meshing: dirty blocks -> flags of blocks to re-mesh -> list.
+// Incremental meshing: dirty (block, direction) masks -> per-block masks of
+// directions to re-mesh -> list of blocks with their masks.
 hipError_t launchMarkRemesh(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s);
-hipError_t launchCompactFlags(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* list,
-                              uint32_t* flags_out, hipStream_t s);
+hipError_t launchCompactFlags(uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* list, uint8_t* dirs,
+                              hipStream_t s);
 hipError_t launchListKeys(const VolumeView& V, const uint32_t* list, uint32_t n, int32_t* out, hipStream_t s);
 hipError_t launchObservedFlags(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s);
 
@@ -157,6 +161,7 @@ struct BatchScene {
   // incremental meshing of the step
   uint32_t* remesh_flags;     // [max_blocks]
   uint32_t* remesh_list;      // [max_blocks]
+  uint8_t* remesh_dirs;       // [max_blocks] directions to re-mesh per listed block
   uint32_t* mesh_counts;      // [6 * max_blocks]
   uint32_t* mesh_offsets;     // [6 * max_blocks]
   uint8_t* cube_counts;       // [6 * max_blocks][512]

@@ -481,18 +481,25 @@ __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir
 // Grid (n, 6): workgroup (i, dir) meshes block list[i] (or pool index i
 // when list is null) in direction dir.  Item dir * n + i: total in
 // counts[item], per-cube counts in cube_counts[512 * item ..].
+// dirs (optional): per listed block, the directions to mesh; the others
+// count zero triangles.
 __global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
                                                        MeshArgs a, uint32_t* __restrict__ counts,
-                                                       uint8_t* __restrict__ cube_counts) {
+                                                       uint8_t* __restrict__ cube_counts,
+                                                       const uint8_t* __restrict__ dirs) {
   __shared__ MeshShared S;
   const uint32_t i = blockIdx.x, dir = blockIdx.y;
   const size_t item = static_cast<size_t>(dir) * n + i;
+  if (dirs && !(dirs[i] & (1u << dir))) {
+    if (threadIdx.x == 0) counts[item] = 0u;
+    return;
+  }
   const uint32_t total = countItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, cube_counts + 512 * item);
   if (threadIdx.x == 0) counts[item] = total;
 }
 
 // Writes the triangles of workgroup (i, dir) at offsets[dir * n + i] in cube
-// order; tri_owner (optional) receives i for each triangle.
+// order; tri_owner (optional) receives the item dir * n + i for each triangle.
 __global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
                                                       MeshArgs a, const uint32_t* __restrict__ counts,
                                                       const uint8_t* __restrict__ cube_counts,
@@ -502,23 +509,31 @@ __global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, const uint32
   const uint32_t i = blockIdx.x, dir = blockIdx.y;
   const size_t item = static_cast<size_t>(dir) * n + i;
   emitItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, offsets[item], counts[item], cube_counts + 512 * item,
-           tris, tri_owner, i, capacity);
+           tris, tri_owner, static_cast<uint32_t>(item), capacity);
 }
 
-// Blocks whose cubes read a dirty block: the dirty block itself and its
-// -x/-y/-z neighbours (a cube at voxel v reads v + {0,1}^3).  Sets flags[q]
-// for each of those present in the table and clears the dirty flag.
-__global__ __launch_bounds__(256) void markRemeshKernel(VolumeView V, uint32_t num_blocks, uint32_t* __restrict__ flags) {
-  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < num_blocks; p += gridDim.x * 256u) {
-    if (V.dirty[p] == 0u) continue;
+// (Block, direction) pairs whose cubes read a written direction of a dirty
+// block: the dirty block itself and its -x/-y/-z neighbours (a cube at voxel
+// v reads v + {0,1}^3), in the directions written.  ORs the dirty block's
+// direction mask into flags[q] for each of those present in the table and
+// clears the dirty mask.
+__device__ void markRemeshRange(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, uint32_t start,
+                                uint32_t stride) {
+  for (uint32_t p = start; p < num_blocks; p += stride) {
+    const uint32_t m = V.dirty[p];
+    if (m == 0u) continue;
     V.dirty[p] = 0u;
     const uint64_t key = V.pool_keys[p];
     for (int o = 0; o < 8; ++o) {
       const uint32_t q = o == 0 ? p : lookupPool(V, keyX(key) - (o & 1), keyY(key) - ((o >> 1) & 1),
                                                  keyZ(key) - (o >> 2));
-      if (q < V.max_blocks) flags[q] = 1u;
+      if (q < V.max_blocks) atomicOr(&flags[q], m);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void markRemeshKernel(VolumeView V, uint32_t num_blocks, uint32_t* __restrict__ flags) {
+  markRemeshRange(V, num_blocks, flags, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
 // flags[p] = 1 for blocks with any observed voxel (non-zero observed bitmask).
@@ -533,14 +548,20 @@ __global__ __launch_bounds__(256) void observedFlagsKernel(VolumeView V, uint32_
   }
 }
 
-__global__ __launch_bounds__(256) void compactFlagsKernel(const uint32_t* __restrict__ flags,
+// list[pos[p]] = p for every p with flags[p] != 0 (pos: exclusive scan of
+// flags != 0); dirs (optional) receives flags[p] at the same index; the
+// flags are cleared.
+__global__ __launch_bounds__(256) void compactFlagsKernel(uint32_t* __restrict__ flags,
                                                           const uint32_t* __restrict__ pos, uint32_t n,
-                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ flags_out) {
-  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < n; p += gridDim.x * 256u)
-    if (flags[p]) {
+                                                          uint32_t* __restrict__ list, uint8_t* __restrict__ dirs) {
+  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < n; p += gridDim.x * 256u) {
+    const uint32_t f = flags[p];
+    if (f) {
       list[pos[p]] = p;
-      flags_out[p] = 0u;
+      if (dirs) dirs[pos[p]] = static_cast<uint8_t>(f);
+      flags[p] = 0u;
     }
+  }
 }
 
 __global__ __launch_bounds__(256) void listKeysKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
@@ -557,12 +578,17 @@ __global__ __launch_bounds__(256) void listKeysKernel(VolumeView V, const uint32
 
 constexpr int kScanTile = 1024;
 
+// kFlags: scan (in != 0) instead of in.
+template <bool kFlags>
 __global__ __launch_bounds__(256) void scanTilesKernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                        uint32_t n, uint32_t* __restrict__ partials) {
   __shared__ uint32_t s[kScanTile];
   const uint32_t base = blockIdx.x * kScanTile;
   const int tid = threadIdx.x;
-  for (int i = tid; i < kScanTile; i += 256) s[i] = (base + i < n) ? in[base + i] : 0u;
+  for (int i = tid; i < kScanTile; i += 256) {
+    const uint32_t x = (base + i < n) ? in[base + i] : 0u;
+    s[i] = kFlags ? (x != 0u ? 1u : 0u) : x;
+  }
   __syncthreads();
   // each lane scans 4 consecutive elements, then lanes are scanned
   uint32_t v[4], acc = 0;
@@ -625,16 +651,7 @@ __global__ __launch_bounds__(256) void scanAddKernel(uint32_t* __restrict__ out,
 
 __device__ void markRemeshScene(const VolumeView& V, uint32_t* flags, uint32_t start, uint32_t stride) {
   const uint32_t nb = V.hdr->num_blocks < V.max_blocks ? V.hdr->num_blocks : V.max_blocks;
-  for (uint32_t p = start; p < nb; p += stride) {
-    if (V.dirty[p] == 0u) continue;
-    V.dirty[p] = 0u;
-    const uint64_t key = V.pool_keys[p];
-    for (int o = 0; o < 8; ++o) {
-      const uint32_t q = o == 0 ? p : lookupPool(V, keyX(key) - (o & 1), keyY(key) - ((o >> 1) & 1),
-                                                 keyZ(key) - (o >> 2));
-      if (q < V.max_blocks) flags[q] = 1u;
-    }
-  }
+  markRemeshRange(V, nb, flags, start, stride);
 }
 
 __global__ __launch_bounds__(256) void markRemeshBatchKernel(const BatchScene* __restrict__ scenes) {
@@ -645,7 +662,7 @@ __global__ __launch_bounds__(256) void markRemeshBatchKernel(const BatchScene* _
 // Sequential-chunk exclusive scan by one workgroup: out[i] = sum(in[0..i)),
 // returns the total (all lanes).  With `list`, instead compacts the indices of
 // non-zero entries of `in` into list (in order) and clears them.
-__device__ uint32_t workgroupScan(uint32_t* in, uint32_t n, uint32_t* out, uint32_t* list) {
+__device__ uint32_t workgroupScan(uint32_t* in, uint32_t n, uint32_t* out, uint32_t* list, uint8_t* dirs) {
   __shared__ uint32_t s[256];
   __shared__ uint32_t carry;
   const int tid = threadIdx.x;
@@ -653,8 +670,8 @@ __device__ uint32_t workgroupScan(uint32_t* in, uint32_t n, uint32_t* out, uint3
   __syncthreads();
   for (uint32_t base = 0; base < n; base += 256) {
     const uint32_t i = base + tid;
-    uint32_t x = i < n ? in[i] : 0u;
-    if (list) x = x ? 1u : 0u;
+    const uint32_t raw = i < n ? in[i] : 0u;
+    const uint32_t x = list ? (raw ? 1u : 0u) : raw;
     s[tid] = x;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
@@ -668,6 +685,7 @@ __device__ uint32_t workgroupScan(uint32_t* in, uint32_t n, uint32_t* out, uint3
       if (list) {
         if (x) {
           list[excl] = i;
+          dirs[excl] = static_cast<uint8_t>(raw);
           in[i] = 0u;
         }
       } else {
@@ -685,7 +703,7 @@ __global__ __launch_bounds__(256) void compactRemeshBatchKernel(const BatchScene
   const BatchScene& sc = scenes[blockIdx.x];
   const VolumeView& V = sc.V;
   const uint32_t nb = V.hdr->num_blocks < V.max_blocks ? V.hdr->num_blocks : V.max_blocks;
-  const uint32_t m = workgroupScan(sc.remesh_flags, nb, nullptr, sc.remesh_list);
+  const uint32_t m = workgroupScan(sc.remesh_flags, nb, nullptr, sc.remesh_list, sc.remesh_dirs);
   if (threadIdx.x == 0) sc.out[0] = m;
 }
 
@@ -708,15 +726,17 @@ __global__ __launch_bounds__(256) void meshCountBatchKernel(const BatchScene* __
     const BatchScene& sc = scenes[s];
     const uint32_t m = sc.out[0], j = i - prefix[s];
     const uint32_t dir = j / m, idx = j % m;
-    const uint32_t c = countItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma,
-                                 sc.cube_counts + 512ull * (dir * m + idx));
+    uint32_t c = 0;
+    if (sc.remesh_dirs[idx] & (1u << dir))  // workgroup-uniform
+      c = countItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma,
+                    sc.cube_counts + 512ull * (dir * m + idx));
     if (threadIdx.x == 0) sc.mesh_counts[dir * m + idx] = c;
   }
 }
 
 __global__ __launch_bounds__(256) void meshScanBatchKernel(const BatchScene* __restrict__ scenes) {
   const BatchScene& sc = scenes[blockIdx.x];
-  const uint32_t t = workgroupScan(sc.mesh_counts, 6u * sc.out[0], sc.mesh_offsets, nullptr);
+  const uint32_t t = workgroupScan(sc.mesh_counts, 6u * sc.out[0], sc.mesh_offsets, nullptr, nullptr);
   if (threadIdx.x == 0) sc.out[1] = t;
 }
 
@@ -737,18 +757,24 @@ __global__ __launch_bounds__(256) void meshEmitBatchKernel(const BatchScene* __r
 }  // namespace
 
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           uint32_t* counts, uint8_t* cube_counts, hipStream_t s) {
+                           uint32_t* counts, uint8_t* cube_counts, const uint8_t* dirs, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  meshCountKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts, cube_counts);
+  meshCountKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts, cube_counts, dirs);
   return hipGetLastError();
 }
 
 size_t scanPartialsSize(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 // out[i] = sum(in[0..i)); partials[(n + 1023) / 1024] receives the total.
-hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s) {
+hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s,
+                      bool flags) {
   const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
-  if (tiles > 0) scanTilesKernel<<<tiles, 256, 0, s>>>(in, out, n, partials);
+  if (tiles > 0) {
+    if (flags)
+      scanTilesKernel<true><<<tiles, 256, 0, s>>>(in, out, n, partials);
+    else
+      scanTilesKernel<false><<<tiles, 256, 0, s>>>(in, out, n, partials);
+  }
   scanPartialsKernel<<<1, 256, 0, s>>>(partials, tiles);
   if (tiles > 0) scanAddKernel<<<tiles, 256, 0, s>>>(out, n, partials);
   return hipGetLastError();
@@ -780,10 +806,10 @@ hipError_t launchObservedFlags(const VolumeView& V, uint32_t num_blocks, uint32_
   return hipGetLastError();
 }
 
-hipError_t launchCompactFlags(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* list,
-                              uint32_t* flags_out, hipStream_t s) {
+hipError_t launchCompactFlags(uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t* list, uint8_t* dirs,
+                              hipStream_t s) {
   if (n == 0) return hipSuccess;
-  compactFlagsKernel<<<gridFor(n), 256, 0, s>>>(flags, pos, n, list, flags_out);
+  compactFlagsKernel<<<gridFor(n), 256, 0, s>>>(flags, pos, n, list, dirs);
   return hipGetLastError();
 }
 
