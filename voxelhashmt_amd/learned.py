@@ -77,8 +77,11 @@ def fuse(vol: Volume, w, frames, idx) -> float:
     return time.perf_counter() - t0
 
 
-def build_training_set(dn: Denoiser, seeds, frames_in=10, frames_target=300, blocks_per_scene=1000):
-    """Appends training samples from each seed's small room to dn."""
+def build_training_set(dn: Denoiser, seeds, frames_in=10, frames_target=300, blocks_per_scene=1000,
+                       target_noise="halved"):
+    """Appends training samples from each seed's small room to dn.  The
+    target frames carry the noise sigma halved ("halved", the training
+    configuration) or no noise at all ("none")."""
     vin = vt = None
     for s in seeds:
         w = synth.plain_small(n_frames=frames_target, seed=s)
@@ -91,7 +94,7 @@ def build_training_set(dn: Denoiser, seeds, frames_in=10, frames_target=300, blo
         frames = render_gpu(w, w.noise)
         fuse(vin, w, frames, spread(frames_target, frames_in))
         del frames
-        frames = render_gpu(w, halved_noise(s))
+        frames = render_gpu(w, halved_noise(s) if target_noise == "halved" else None)
         fuse(vt, w, frames, range(frames_target))
         del frames
         k_in = vin.observed_block_keys()
@@ -232,6 +235,8 @@ def main(argv=None):
     ap.add_argument("--blocks-per-scene", type=int, default=1000)
     ap.add_argument("--frames-in", type=int, default=10)
     ap.add_argument("--frames-target", type=int, default=300)
+    ap.add_argument("--target-noise", choices=["halved", "none"], default="halved",
+                    help="noise of the many-frame target fusion: sigma halved (the training configuration) or none")
     ap.add_argument("--holdout-scenes", type=int, default=8, help="of the training scenes, kept out of training")
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=4096)
@@ -240,6 +245,9 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=2000)
     ap.add_argument("--eval-frames-plain", type=int, default=300)
     ap.add_argument("--eval-frames-learned", type=int, default=30)
+    ap.add_argument("--apply-max-delta", type=float, default=0.5, help="per-cell confidence bound (tsdf units)")
+    ap.add_argument("--apply-block-max-delta", type=float, default=0.25,
+                    help="per-block fallback to plain fusion above this mean |change|")
     ap.add_argument("--params-in", default=None, help="skip training: load parameters (.npy float32)")
     ap.add_argument("--params-out", default=None)
     ap.add_argument("--out", default=None, help="results JSON")
@@ -254,7 +262,7 @@ def main(argv=None):
         dn.set_params(init_params(0))
         t0 = time.perf_counter()
         seeds = list(range(args.train_seed, args.train_seed + args.train_scenes))
-        build_training_set(dn, seeds, args.frames_in, args.frames_target, args.blocks_per_scene)
+        build_training_set(dn, seeds, args.frames_in, args.frames_target, args.blocks_per_scene, args.target_noise)
         res["train_set_blocks"] = dn.count
         res["train_set_seconds"] = time.perf_counter() - t0
         # the last scenes' samples are held out of training
@@ -276,7 +284,9 @@ def main(argv=None):
     scenes = []
     for i in range(args.eval_scenes):
         w = synth.plain_scale(n_frames=args.eval_frames_plain, seed=args.eval_seed + i)
-        scenes.append(asdict(compare_on_scene(dn, w, args.eval_frames_plain, args.eval_frames_learned)))
+        scenes.append(asdict(compare_on_scene(dn, w, args.eval_frames_plain, args.eval_frames_learned,
+                                              apply_kwargs={"max_delta": args.apply_max_delta,
+                                                            "block_max_delta": args.apply_block_max_delta})))
         if args.out:
             res["scenes"] = scenes
             with open(args.out, "w") as f:
