@@ -249,3 +249,41 @@ def test_device_renderer_matches_host(gpu):
         clean_h = synth.render(w.scene, w.K, w.poses, None)
         synth.render(w.scene, w.K, w.poses, None, 0, dev)
         assert np.array_equal(dev.cpu().numpy().view(np.uint16), clean_h)  # no noise: exact
+
+
+def test_incremental_mesh_copy_after_small_buffer(gpu):
+    """A host buffer smaller than the incremental mesh gets the first
+    triangles; vhm_incremental_mesh_copy then returns all of them, equal to
+    what a large enough buffer receives."""
+    import ctypes as C
+    w = synth.tiny(n_frames=4, seed=14)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    L = vhm.lib()
+    results = []
+    for cap in (16, 1 << 20):
+        with vhm.Volume(_params(w)) as vol:
+            for i in range(4):
+                vol.integrate(frames[i], w.K, w.poses[i])
+            nb_cap = vol.block_count()
+            keys = np.zeros((nb_cap, 3), np.int32)
+            dirs = np.zeros(nb_cap, np.uint8)
+            tris = np.zeros((cap, 3, 3), np.float32)
+            items = np.zeros(cap, np.uint32)
+            nb, nt = C.c_size_t(), C.c_size_t()
+            assert L.vhm_extract_mesh_incremental(vol._h, 0.0, keys.ctypes.data, dirs.ctypes.data, nb_cap,
+                                                  C.byref(nb), tris.ctypes.data, items.ctypes.data, cap,
+                                                  vhm.VHM_HOST, C.byref(nt)) == 0
+            n = nt.value
+            assert n > 16
+            full = np.zeros((n, 3, 3), np.float32)
+            full_items = np.zeros(n, np.uint32)
+            got = C.c_size_t()
+            assert L.vhm_incremental_mesh_copy(vol._h, full.ctypes.data, full_items.ctypes.data, n, C.byref(got)) == 0
+            assert got.value == n
+            m = min(cap, n)
+            assert np.array_equal(tris[:m].view(np.uint32), full[:m].view(np.uint32))
+            results.append((full, full_items, keys[:nb.value].copy(), dirs[:nb.value].copy()))
+    # the two volumes saw the same frames: same items and triangles (pool order may differ)
+    (t0, i0, k0, d0), (t1, i1, k1, d1) = results
+    assert np.array_equal(O.canonical_mesh(t0), O.canonical_mesh(t1))
+    assert sorted(map(tuple, k0)) == sorted(map(tuple, k1))

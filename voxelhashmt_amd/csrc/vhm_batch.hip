@@ -277,7 +277,7 @@ int vhm_batch_prepare(vhm_batch* b, int integrate_grid, int mesh_grid, int use_g
     b->graph = nullptr;
   }
   if (use_graph) {
-    B_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal), "hipStreamBeginCapture");
+    B_TRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     const int rc = enqueueStep(b);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
