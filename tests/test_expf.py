@@ -76,3 +76,14 @@ def test_dequant_equals_division_for_every_code(chk):
     ref = (np.arange(65536, dtype=np.float32) / np.float32(65535.0)) * np.float32(2.0) - np.float32(1.0)
     assert np.array_equal(div.view(np.uint32), ref.view(np.uint32))  # the plain division, as numpy rounds it
     assert np.array_equal(prod.view(np.uint32), div.view(np.uint32))
+
+
+def test_norm_threshold_is_the_sqrt_boundary():
+    """norm3(v) <= 1e-6f  <=>  dot(v, v) <= 0x1.19799ap-40 (the constant in
+    vhm_math.h): sqrtf is correctly rounded and monotone, so the boundary
+    and its neighbourhood decide it."""
+    T = np.float32(float.fromhex("0x1.19799ap-40"))
+    lim = np.float32(1e-6)
+    bits = T.view(np.uint32).astype(np.int64)
+    s = (np.arange(bits - 100000, bits + 100000)).astype(np.uint32).view(np.float32)
+    assert np.array_equal(np.sqrt(s) <= lim, s <= T)

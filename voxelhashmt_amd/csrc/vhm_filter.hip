@@ -80,10 +80,10 @@ __device__ void filterTile(const In* __restrict__ in, float* __restrict__ out, c
           const F3 pu = backProjectCont(static_cast<float>(u), static_cast<float>(v - 1), up, a.K);
           const F3 pd = backProjectCont(static_cast<float>(u), static_cast<float>(v + 1), dn, a.K);
           n = normalized3(cross3(sub3(pd, pu), sub3(pr, pl)));
-          if (!(norm3(n) <= 1e-6f)) {
+          if (!normAtMost1em6(n)) {
             const F3 cp = backProjectCont(static_cast<float>(u), static_cast<float>(v), c, a.K);
             n = orientTowardCamera(n, cp);
-            ok = norm3(n) > 1e-6f;
+            ok = !normAtMost1em6(n);
           }
         }
       }

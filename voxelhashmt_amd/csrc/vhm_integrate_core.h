@@ -71,8 +71,10 @@ VHM_HD bool bilinearFromTaps(float d00, float d10, float d01, float d11, float t
   if (d00 > 0.0f && d10 > 0.0f && d01 > 0.0f && d11 > 0.0f) {
     const float ref = (nearest_in_range && nearest > 0.0f) ? nearest : d00;
     const float thr = discontinuityThreshold(ref, disc_base, disc_rel);
-    const float mn = smin(smin(d00, d10), smin(d01, d11));
-    const float mx = smax(smax(d00, d10), smax(d01, d11));
+    // all four taps are > 0 here (no NaN, no signed zero), where the IEEE
+    // min/max instructions agree with std::min/std::max
+    const float mn = fminf(fminf(d00, d10), fminf(d01, d11));
+    const float mx = fmaxf(fmaxf(d00, d10), fmaxf(d01, d11));
     const float top = d00 + (d10 - d00) * tx;
     const float bottom = d01 + (d11 - d01) * tx;
     if (mx - mn <= thr) {
@@ -121,7 +123,7 @@ VHM_HD bool baseNormal(const D& dv, const NormalParams& a, float u, float v, F3&
   const F3 pl = backProjectN(u - 1.0f, v, dl, a), pr = backProjectN(u + 1.0f, v, dr, a);
   const F3 pu = backProjectN(u, v - 1.0f, du, a), pd = backProjectN(u, v + 1.0f, dd, a);
   n = normalized3(cross3(sub3(pd, pu), sub3(pr, pl)));
-  return norm3(n) > 1e-6f;
+  return !normAtMost1em6(n);
 }
 
 // voxel_hash_tsdf.h:773-840.  `cd` = sampleDepthBilinear(u, v), already
@@ -160,8 +162,8 @@ VHM_HD bool surfaceNormalReference(const D& dv, const NormalParams& a, float u, 
   }
   n = normalized3(mul3(acc, 1.0f / ws));
   n = orientTowardCamera(n, cp);
-  if (norm3(n) <= 1e-6f) n = cn;
-  return norm3(n) > 1e-6f;
+  if (normAtMost1em6(n)) n = cn;
+  return !normAtMost1em6(n);
 }
 
 // --------------------------------------------------------- fast path ----
@@ -246,7 +248,7 @@ VHM_HD bool surfaceNormalFast(const D& dv, const NormalParams& a, float u, float
       F3 nn{0.0f, 0.0f, 0.0f};
       if (good) {
         nn = normalized3(cross3(sub3(Pt[td], Pt[tu]), sub3(Pt[tr], Pt[tl])));
-        good = norm3(nn) > 1e-6f;
+        good = !normAtMost1em6(nn);
       }
       bn[j * 3 + i] = nn;
       bok[j * 3 + i] = good;
@@ -278,8 +280,8 @@ VHM_HD bool surfaceNormalFast(const D& dv, const NormalParams& a, float u, float
   }
   n = normalized3(mul3(acc, 1.0f / ws));
   n = orientTowardCamera(n, cp);
-  if (norm3(n) <= 1e-6f) n = cn;
-  return norm3(n) > 1e-6f;
+  if (normAtMost1em6(n)) n = cn;
+  return !normAtMost1em6(n);
 }
 
 template <class D>
@@ -337,7 +339,7 @@ VHM_HD bool integrateSample(Slots& L, float coord, F3 pw, F3 nw, float w, float 
                             float max_weight) {
   if (w <= 0.0f) return false;
   const F3 n = normalized3(nw);
-  if (norm3(n) <= 1e-6f) return false;
+  if (normAtMost1em6(n)) return false;
   const float offset = dot3(n, pw);
   // findLayerSlotForObservation (357-392)
   int empty = -1, best = -1;
@@ -373,7 +375,7 @@ VHM_HD bool integrateSample(Slots& L, float coord, F3 pw, F3 nw, float w, float 
   if (pw0 > 0.0f) {
     const F3 prev = unpackNormalOct(pn0);
     blended = normalized3(div3(add3(mul3(prev, pw0), mul3(n, ew)), nw1));
-    if (norm3(blended) <= 1e-6f) blended = n;
+    if (normAtMost1em6(blended)) blended = n;
   }
   const uint32_t pn1 = packNormalOct(blended);
 #pragma unroll

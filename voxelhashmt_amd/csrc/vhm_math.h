@@ -89,6 +89,13 @@ VHM_HD F3 div3(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
 VHM_HD float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 VHM_HD F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 VHM_HD float norm3(F3 v) { return sqrtf(dot3(v, v)); }
+
+// norm3(v) <= 1e-6f exactly when dot3(v, v) <= kNormSq1em6: sqrtf is
+// correctly rounded and monotone, and this is the largest float whose
+// rounded square root does not exceed 1e-6f (found by bisection over the
+// float bit patterns; checked at the boundary in the CPU tests).
+constexpr float kNormSq1em6 = 0x1.19799ap-40f;
+VHM_HD bool normAtMost1em6(F3 v) { return dot3(v, v) <= kNormSq1em6; }
 VHM_HD F3 normalized3(F3 v) {
   const float len = norm3(v);
   if (len <= 1e-8f) return {0.0f, 0.0f, 0.0f};
@@ -158,7 +165,7 @@ VHM_HD float discontinuityThreshold(float d, float base, float rel) { return sma
 // bound plus its own rounding -- the signs agree and the divisions are
 // skipped; otherwise the reference sequence runs as written.
 VHM_HD F3 orientTowardCamera(F3 n, F3 p) {
-  if (norm3(n) <= 1e-6f || norm3(p) <= 1e-6f) return n;
+  if (normAtMost1em6(n) || normAtMost1em6(p)) return n;
   const float d1 = dot3(n, p);
   const float t = fabsf(n.x * p.x) + fabsf(n.y * p.y) + fabsf(n.z * p.z);
   if (t > 1e-30f && fabsf(d1) > 1e-5f * t) return d1 > 0.0f ? mul3(n, -1.0f) : n;
