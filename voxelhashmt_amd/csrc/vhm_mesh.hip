@@ -256,6 +256,8 @@ __device__ __forceinline__ bool cubeMayCross(const MeshShared& S, int cube) {
 // 24..31 (those would hold +inf and never move).  Checked exhaustively by the
 // 0-1 principle on the host when the table was generated.
 constexpr int kSortPairs = 132;
+constexpr int kSortPairs16 = 63;  // the prefix that sorts entries 0..15
+constexpr int kSortPairs8 = 19;   // the prefix that sorts entries 0..7
 constexpr uint8_t kSortNet[kSortPairs][2] = {
     {0,1}, {2,3}, {0,2}, {1,3}, {1,2}, {4,5}, {6,7}, {4,6}, {5,7}, {5,6}, {0,4}, {2,6}, {2,4}, {1,5}, {3,7}, {3,5},
     {1,2}, {3,4}, {5,6}, {8,9}, {10,11}, {8,10}, {9,11}, {9,10}, {12,13}, {14,15}, {12,14}, {13,15}, {13,14},
@@ -293,11 +295,12 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
   for (int c = 0; c < 8; ++c)
     rc[c] = (lx + kMcCornerOffset[c][0]) + kR * ((ly + kMcCornerOffset[c][1]) + kR * (lz + kMcCornerOffset[c][2]));
 
-  // (1) candidates
+  // (1) candidates, slot-major: srt[8 k + c]
   constexpr float kNone = __builtin_huge_valf();
   float sc[8][kSlots];
   float srt[8 * kSlots];
   int nc = 0;
+  bool slot1 = false, slot2 = false;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const uint32_t b = S.bits[rc[c]];
@@ -307,14 +310,24 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
       if ((b & kBitAllocated) && (b & (1u << k))) {
         s = planeSD(S, k, rc[c], cc);
         ++nc;
+        slot1 |= k == 1;
+        slot2 |= k == 2;
       }
       sc[c][k] = s;
-      srt[c * kSlots + k] = s;
+      srt[8 * k + c] = s;
     }
   }
   if (nc == 0) return 0;
+  // Observed slots are contiguous from slot 0 (sortDirectionalLayers), so
+  // when no lane of the wavefront has a slot-2 (or slot-1) candidate the
+  // entries 16..23 (8..23) are +inf and the network's prefix that sorts
+  // 0..15 (0..7) -- Batcher's construction sorts the halves first -- sorts
+  // everything.
+  const bool any1 = __any(slot1 || slot2), any2 = __any(slot2);
+  const int pairs = any2 ? kSortPairs : (any1 ? kSortPairs16 : kSortPairs8);
 #pragma clang loop unroll(full)
   for (int q = 0; q < kSortPairs; ++q) {
+    if (q >= pairs) break;
     const int i = kSortNet[q][0], j = kSortNet[q][1];
     const float x = srt[i], y = srt[j];
     const bool sw = y < x;
@@ -343,10 +356,19 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
   }
   S.cand[ncl++][tid] = centre;
 
-  // (3) per cluster
+  // (3) per cluster.  A selected slot lies within extract_tol of the
+  // centre at the cell centre, and its value at a corner centre differs from
+  // that by at most |n . (corner - cell centre)| <= 0.87 voxel plus the
+  // rounding of two dot products of magnitude <= R (the cube's coordinate
+  // extent); a cluster whose centre is farther than extract_tol + voxel +
+  // 2^-20 R from iso has every corner value (and every completion mean of
+  // them) on one side of iso and yields no triangle.
+  const float reach = a.extract_tol + a.voxel +
+                      (fabsf(cc.x) + fabsf(cc.y) + fabsf(cc.z) + 2.0f * a.voxel) * 0x1.0p-20f;
   uint32_t ntri = 0;
   for (int j = 0; j < ncl; ++j) {
     const float cen = S.cand[j][tid];
+    if (cen - reach > a.iso || cen + reach < a.iso) continue;
     float val[8];
     uint32_t valid = 0;
 #pragma unroll
