@@ -270,6 +270,19 @@ constexpr uint8_t kSortNet[kSortPairs][2] = {
     {13,21}, {5,9}, {13,17}, {3,19}, {11,19}, {7,23}, {15,23}, {7,11}, {15,19}, {3,5}, {7,9}, {11,13}, {15,17},
     {19,21}, {1,2}, {3,4}, {5,6}, {7,8}, {9,10}, {11,12}, {13,14}, {15,16}, {17,18}, {19,20}, {21,22}};
 
+// Compare-exchange pairs [kFrom, kTo) of kSortNet on v.
+template <int kFrom, int kTo>
+__device__ __forceinline__ void sortPairs(float* v) {
+#pragma clang loop unroll(full)
+  for (int q = kFrom; q < kTo; ++q) {
+    const int i = kSortNet[q][0], j = kSortNet[q][1];
+    const float x = v[i], y = v[j];
+    const bool sw = y < x;
+    v[i] = sw ? y : x;
+    v[j] = sw ? x : y;
+  }
+}
+
 // One cube of extractDirectionalMultiLayerMesh (voxel_hash_tsdf.h:853-935)
 // that passed cubeMayCross.  Returns the triangle count; with kEmit also
 // writes them at `out`.
@@ -323,16 +336,12 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
   // entries 16..23 (8..23) are +inf and the network's prefix that sorts
   // 0..15 (0..7) -- Batcher's construction sorts the halves first -- sorts
   // everything.
-  const bool any1 = __any(slot1 || slot2), any2 = __any(slot2);
-  const int pairs = any2 ? kSortPairs : (any1 ? kSortPairs16 : kSortPairs8);
-#pragma clang loop unroll(full)
-  for (int q = 0; q < kSortPairs; ++q) {
-    if (q >= pairs) break;
-    const int i = kSortNet[q][0], j = kSortNet[q][1];
-    const float x = srt[i], y = srt[j];
-    const bool sw = y < x;
-    srt[i] = sw ? y : x;
-    srt[j] = sw ? x : y;
+  // Each range is unrolled on its own so every index stays a constant and
+  // srt stays in registers; the two branches are uniform.
+  sortPairs<0, kSortPairs8>(srt);
+  if (__any(slot1 || slot2)) {
+    sortPairs<kSortPairs8, kSortPairs16>(srt);
+    if (__any(slot2)) sortPairs<kSortPairs16, kSortPairs>(srt);
   }
 
   // (2) clusters: a candidate joins the open cluster unless it lies more than
