@@ -92,6 +92,10 @@ void vhm_default_params(vhm_params* p, int preset, float voxel_size, float trunc
  * usable, 0 otherwise; msg receives the reason either way. */
 int vhm_query_device(char* msg, size_t msg_capacity);
 
+/* Free and total memory of the current device (hipMemGetInfo), e.g. to
+ * size max_blocks (147,456 B of pool per block). */
+int vhm_device_memory(size_t* free_bytes, size_t* total_bytes);
+
 /* Creates a volume on the current device: reserves the block pool and hash
  * table (replaces GpuVolumeCache, cuda_tsdf_integrator.cu:114-167). */
 int vhm_create(const vhm_params* p, vhm_ctx** out);

@@ -5,6 +5,9 @@
   * _ref/libvhm_ref.so -- the reference's own CPU path compiled from its
                      sources (oracle/Makefile `ref`); present only where the
                      reference tree was available at build time.
+  * _ref/libdemo_ref.so -- the reference demo's and dataset loader's own
+                     functions (oracle/demo_harness.cpp), likewise.
+  * _ref/tsdf_voxel_hash_demo -- the reference's CPU-only demo executable.
 """
 from __future__ import annotations
 
@@ -16,6 +19,8 @@ import numpy as np
 ORACLE_DIR = Path(__file__).resolve().parents[1] / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 REF_LIB = ORACLE_DIR / "_ref" / "libvhm_ref.so"
+DEMO_REF_LIB = ORACLE_DIR / "_ref" / "libdemo_ref.so"
+REF_DEMO_EXE = ORACLE_DIR / "_ref" / "tsdf_voxel_hash_demo"
 
 
 class OracleParams(C.Structure):
@@ -31,6 +36,7 @@ class OracleParams(C.Structure):
 
 _oracle = None
 _ref = None
+_demo_ref = None
 
 VP = C.c_void_p
 FP = C.POINTER(C.c_float)
@@ -122,6 +128,39 @@ def ref():
             f.restype, f.argtypes = r, a
         _ref = L
     return _ref
+
+
+def demo_ref_available() -> bool:
+    return DEMO_REF_LIB.exists()
+
+
+def demo_ref():
+    """The reference demo / dataset-loader functions (oracle/demo_harness.cpp)."""
+    global _demo_ref
+    if _demo_ref is None:
+        if not DEMO_REF_LIB.exists():
+            raise RuntimeError(f"{DEMO_REF_LIB} missing: run `make -C oracle ref` where the reference tree exists")
+        L = C.CDLL(str(DEMO_REF_LIB))
+        CP = C.c_char_p
+        sig = {
+            "ref_demo_look_at": (None, [FP, FP, FP, FP]),
+            "ref_demo_trace": (C.c_int, [FP, FP, FP]),
+            "ref_demo_rng_create": (VP, [C.c_uint32]),
+            "ref_demo_rng_destroy": (None, [VP]),
+            "ref_demo_render": (None, [VP, C.c_int, C.c_int, FP, FP, FP, C.c_int, VP]),
+            "ref_write_ply": (C.c_int, [CP, VP, C.c_size_t]),
+            "ref_parse_intrinsics_file": (C.c_int, [CP, FP, C.POINTER(C.c_int), FP, C.POINTER(C.c_int), CP,
+                                                    C.c_size_t]),
+            "ref_load_pose": (C.c_int, [CP, FP, FP, CP, C.c_size_t]),
+            "ref_quaternion_to_matrix": (None, [C.c_float, C.c_float, C.c_float, C.c_float, FP]),
+            "ref_list_frame_ids": (C.c_int, [CP, VP, C.c_size_t, C.POINTER(C.c_size_t), CP, C.c_size_t]),
+            "ref_find_intrinsics_path": (C.c_int, [CP, CP, C.c_size_t, CP, C.c_size_t]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _demo_ref = L
+    return _demo_ref
 
 
 def p(a: np.ndarray) -> int:
@@ -273,6 +312,12 @@ def oracle_filter(params: OracleParams, depth_m, K, fp64=False) -> np.ndarray:
 def sorted_keys(keys: np.ndarray) -> np.ndarray:
     keys = np.asarray(keys, np.int32).reshape(-1, 3)
     return keys[np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))]
+
+
+def mesh_digest(tris: np.ndarray) -> str:
+    """sha256 of the canonical (sorted) triangle multiset."""
+    import hashlib
+    return hashlib.sha256(canonical_mesh(tris).tobytes()).hexdigest()
 
 
 def canonical_mesh(tris: np.ndarray) -> np.ndarray:

@@ -1,5 +1,5 @@
 """The C-ABI library loads without a GPU and exports every function that
-include/vhm.h declares; host-only entry points work without a device."""
+include/*.h declares; host-only entry points work without a device."""
 import ctypes as C
 import re
 import subprocess
@@ -13,7 +13,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def _declared():
-    text = (ROOT / "include" / "vhm.h").read_text()
+    text = "".join(h.read_text() for h in sorted((ROOT / "include").glob("*.h")))
     return sorted(set(re.findall(r"\b(vhm_[a-z0-9_]+)\s*\(", text)))
 
 

@@ -20,7 +20,7 @@ VHM_HOST, VHM_DEVICE = 0, 1
 PRESET_REFERENCE_CPU, PRESET_REFERENCE_CUDA = 0, 1
 FLAG_PREFILTERED = 1
 STATUS = {0: "VHM_OK", 1: "VHM_ERR_INVALID_ARGUMENT", 2: "VHM_ERR_HIP", 3: "VHM_ERR_CAPACITY",
-          4: "VHM_ERR_KEY_RANGE", 5: "VHM_ERR_NO_DEVICE", 6: "VHM_ERR_BUFFER_TOO_SMALL"}
+          4: "VHM_ERR_KEY_RANGE", 5: "VHM_ERR_NO_DEVICE", 6: "VHM_ERR_BUFFER_TOO_SMALL", 7: "VHM_ERR_IO"}
 BLOCK_BYTES = 147456  # 512 voxels x 6 directions x 3 slots x 16 B
 VOXEL_DTYPE = np.dtype([("weight", "<f4"), ("surface_coord", "<f4"), ("plane_offset", "<f4"),
                         ("packed_normal", "<u4")])
@@ -136,11 +136,33 @@ _SIGS = {
                                         C.c_void_p, C.c_size_t]),
     "vhm_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
+    "vhm_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    # include/vhm_io.h (host only)
+    "vhm_io_last_error": (C.c_char_p, []),
+    "vhm_demo_intrinsics": (None, [C.c_int32, C.c_int32, C.POINTER(Intrinsics)]),
+    "vhm_demo_pose": (None, [C.c_int32, C.c_int32, C.POINTER(Pose)]),
+    "vhm_demo_trace": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "vhm_demo_renderer_create": (C.c_int, [C.c_uint32, C.POINTER(C.c_void_p)]),
+    "vhm_demo_renderer_destroy": (None, [C.c_void_p]),
+    "vhm_demo_render": (C.c_int, [C.c_void_p, C.POINTER(Intrinsics), C.POINTER(Pose), C.c_int32, C.c_uint32,
+                                  C.c_void_p]),
+    "vhm_write_ply": (C.c_int, [C.c_char_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vhm_dataset_frame_ids": (C.c_int, [C.c_char_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vhm_dataset_intrinsics": (C.c_int, [C.c_char_p, C.POINTER(Intrinsics), C.POINTER(C.c_float)]),
+    "vhm_load_pose": (C.c_int, [C.c_char_p, C.POINTER(Pose)]),
+    "vhm_quaternion_to_matrix": (None, [C.c_float, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)]),
+    "vhm_png_size": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "vhm_load_depth_png": (C.c_int, [C.c_char_p, C.POINTER(Intrinsics), C.c_float, C.c_void_p,
+                                     C.POINTER(C.c_size_t)]),
+    "vhm_load_depth_png_u16": (C.c_int, [C.c_char_p, C.POINTER(Intrinsics), C.c_float, C.c_void_p,
+                                         C.POINTER(C.c_size_t)]),
+    "vhm_load_dataset_frame": (C.c_int, [C.c_char_p, C.c_int32, C.c_char_p, C.POINTER(Intrinsics), C.c_float,
+                                         C.POINTER(Pose), C.c_void_p, C.POINTER(C.c_size_t)]),
 }
 
 
 def exported_symbols():
-    """Names the C-ABI must export (every function declared in include/vhm.h)."""
+    """Names the C-ABI must export (every function declared in include/*.h)."""
     return sorted(_SIGS)
 
 

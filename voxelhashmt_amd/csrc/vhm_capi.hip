@@ -379,6 +379,13 @@ int vhm_query_device(char* msg, size_t cap) {
   return ok;
 }
 
+int vhm_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return VHM_ERR_INVALID_ARGUMENT;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return VHM_ERR_NO_DEVICE;
+  return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? VHM_OK : VHM_ERR_HIP;
+}
+
 int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (!p || !out) return VHM_ERR_INVALID_ARGUMENT;
   *out = nullptr;
