@@ -41,6 +41,7 @@ struct vhm_batch {
   uint32_t* alloc_prefix = nullptr;
   uint32_t* active_prefix = nullptr;
   uint32_t* mesh_prefix = nullptr;
+  float* mesh_cand = nullptr;  // marching cubes' cluster-centre scratch (kMeshCandBytes)
   uint32_t filter_total = 0, alloc_total = 0;
   int integrate_grid = 2048, mesh_grid = 2048, num_cus = 256;
   hipGraph_t graph = nullptr;
@@ -91,7 +92,7 @@ int enqueueStep(vhm_batch* b) {
   B_TRY(b, launchActivePrefix(b->scenes, S, b->state, b->active_prefix, b->stream), "batch prefix launch");
   B_TRY(b, launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, b->integrate_grid, b->stream),
         "batch integrate launch");
-  B_TRY(b, launchMeshBatch(b->scenes, S, b->mesh_prefix, b->mesh_grid, b->stream), "batch mesh launch");
+  B_TRY(b, launchMeshBatch(b->scenes, S, b->mesh_prefix, b->mesh_grid, b->mesh_cand, b->stream), "batch mesh launch");
   B_TRY(b, launchStepIncrement(b->state, b->stream), "batch step launch");
   return VHM_OK;
 }
@@ -130,6 +131,7 @@ int vhm_batch_create(int num_scenes, const vhm_params* params, vhm_batch** out) 
   if (e == hipSuccess) e = bmalloc(b, &b->alloc_prefix, 4 * (num_scenes + 1));
   if (e == hipSuccess) e = bmalloc(b, &b->active_prefix, 4 * (num_scenes + 1));
   if (e == hipSuccess) e = bmalloc(b, &b->mesh_prefix, 4 * (num_scenes + 1));
+  if (e == hipSuccess) e = bmalloc(b, &b->mesh_cand, kMeshCandBytes);
   if (e == hipSuccess) e = hipMemset(b->state, 0, sizeof(BatchState));
   if (e != hipSuccess) {
     vhm_batch_destroy(b);
@@ -365,7 +367,7 @@ int timedStep(vhm_batch* b, int gi, int gm, float ms[3], uint64_t* updates, uint
   if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
   if (e == hipSuccess) e = launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, gi, b->stream);
   if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);
-  if (e == hipSuccess) e = launchMeshBatch(b->scenes, S, b->mesh_prefix, gm, b->stream);
+  if (e == hipSuccess) e = launchMeshBatch(b->scenes, S, b->mesh_prefix, gm, b->mesh_cand, b->stream);
   if (e == hipSuccess) e = hipEventRecord(ev[3], b->stream);
   if (e == hipSuccess) e = launchStepIncrement(b->state, b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);

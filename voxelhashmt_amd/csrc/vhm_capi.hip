@@ -57,6 +57,8 @@ struct vhm_ctx {
   size_t mesh_counts_cap = 0, mesh_offsets_cap = 0, mesh_partials_cap = 0;
   uint8_t* cube_counts = nullptr;  // per-cube triangle counts of the count pass
   size_t cube_counts_cap = 0;
+  float* mesh_cand = nullptr;  // marching cubes' cluster-centre scratch (kMeshCandBytes)
+  size_t mesh_cand_cap = 0;
   float* tris = nullptr;
   size_t tris_cap = 0;
   int32_t* key_buf = nullptr;
@@ -450,7 +452,7 @@ int vhm_destroy(vhm_ctx* c) {
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
-                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs};
+                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -660,6 +662,8 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   if (entries > 0xffffffffull) return fail(c, VHM_ERR_CAPACITY, "mesh extraction failed: too many blocks");
   if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
   if ((rc = ensure(c, &c->cube_counts, &c->cube_counts_cap, entries * 512, "hipMalloc cube counts"))) return rc;
+  if ((rc = ensure(c, &c->mesh_cand, &c->mesh_cand_cap, kMeshCandBytes / sizeof(float), "hipMalloc mesh scratch")))
+    return rc;
   if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
   if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
                    "hipMalloc mesh partials")))
@@ -671,7 +675,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
   hipEvent_t ev{};
   beginTimed(c, 3, &ev);
-  VHM_TRY(c, launchMeshCount(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, nullptr, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchMeshCount(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, nullptr, c->mesh_cand, c->stream), "mesh count kernel launch");
   VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
           "mesh scan kernel launch");
   uint32_t total = 0;
@@ -682,7 +686,7 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   *n_out = total;
   if (tris_out && capacity > 0 && total > 0) {
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
-    VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, nullptr, total, c->stream),
+    VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, nullptr, total, c->mesh_cand, c->stream),
             "mesh emit kernel launch");
     const size_t m = capacity < total ? capacity : total;
     VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, m * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
@@ -728,6 +732,8 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
   const size_t entries = size_t(m) * kDirs;
   if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
   if ((rc = ensure(c, &c->cube_counts, &c->cube_counts_cap, entries * 512, "hipMalloc cube counts"))) return rc;
+  if ((rc = ensure(c, &c->mesh_cand, &c->mesh_cand_cap, kMeshCandBytes / sizeof(float), "hipMalloc mesh scratch")))
+    return rc;
   if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
   if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
                    "hipMalloc mesh partials")))
@@ -737,7 +743,7 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
   ma.iso = iso;
   ma.extract_tol = c->P.voxel_size * c->P.layer_extract_scale;
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
-  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->remesh_dirs, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->stream), "mesh count kernel launch");
   VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
           "mesh scan kernel launch");
   uint32_t total = 0;
@@ -762,7 +768,7 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
     // straight into the caller's device buffers; triangles past the capacity are dropped
     const uint32_t cap = static_cast<uint32_t>(tri_capacity < total ? tri_capacity : total);
     VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, tris_out, tri_item_out, cap,
-                              c->stream),
+                              c->mesh_cand, c->stream),
             "mesh emit kernel launch");
   } else if (total > 0 && out_memkind != VHM_DEVICE) {
     // host output: emitted into the context's buffers (kept for
@@ -770,7 +776,7 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
     if ((rc = ensure(c, &c->tri_owner, &c->tri_owner_cap, size_t(total), "hipMalloc triangle owners"))) return rc;
     VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, c->tri_owner, total,
-                              c->stream),
+                              c->mesh_cand, c->stream),
             "mesh emit kernel launch");
     c->inc_tris = total;
     const size_t t = tri_capacity < total ? tri_capacity : total;

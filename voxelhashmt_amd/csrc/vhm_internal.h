@@ -111,6 +111,14 @@ struct MeshArgs {
   float voxel, iso, extract_tol;
 };
 
+// Marching cubes keeps each lane's cluster centres (at most 8 corners x 3
+// slots) in a per-workgroup global scratch column rather than in LDS, which
+// lets 4 workgroups share a CU.  The mesh kernels run persistent grids of at
+// most kMeshGridMax workgroups, each owning kMeshCandWords floats.
+constexpr uint32_t kMeshGridMax = 2048;
+constexpr size_t kMeshCandWords = 24 * 256;
+constexpr size_t kMeshCandBytes = kMeshGridMax * kMeshCandWords * sizeof(float);
+
 // Launchers (each defined next to its kernel).  All are asynchronous on `s`.
 hipError_t launchFilterF32(const float* in, float* out, const FilterArgs& a, hipStream_t s);
 hipError_t launchFilterU16(const uint16_t* in, float* out, const FilterArgs& a, hipStream_t s);
@@ -124,7 +132,7 @@ hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeVi
 // counts[dir * n + i] and per-cube counts cube_counts[512 * (dir * n + i) ..]
 // (6 * n * 512 bytes), then triangles at the scanned offsets.
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           uint32_t* counts, uint8_t* cube_counts, const uint8_t* dirs, hipStream_t s);
+                           uint32_t* counts, uint8_t* cube_counts, const uint8_t* dirs, float* cand, hipStream_t s);
 // out[i] = sum(in[0..i)) (with flags: the count of non-zero in[0..i));
 // partials[(n + 1023) / 1024] receives the total.
 hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s,
@@ -132,7 +140,7 @@ hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* p
 size_t scanPartialsSize(uint32_t n);
 hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                           const uint32_t* counts, const uint8_t* cube_counts, const uint32_t* offsets, float* tris,
-                          uint32_t* tri_owner, uint32_t capacity, hipStream_t s);
+                          uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s);
 // Incremental meshing: dirty (block, direction) masks -> per-block masks of
 // directions to re-mesh -> list of blocks with their masks.
 hipError_t launchMarkRemesh(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s);
@@ -194,7 +202,8 @@ hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t*
                                 int grid, hipStream_t s);
 hipError_t launchStepIncrement(BatchState* st, hipStream_t s);
 // Incremental meshing of every scene's dirty blocks (prefix: S + 1 scratch).
-hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, hipStream_t s);
+hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, float* cand,
+                           hipStream_t s);
 
 struct SynthDeviceScene {
   float room_min[3], room_max[3];

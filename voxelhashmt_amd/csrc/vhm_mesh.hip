@@ -29,7 +29,7 @@ __device__ __constant__ McTables c_mc = makeMcTables();
 
 constexpr int kR = 9;  // corner region edge (8 voxels + 1 from the + neighbours)
 constexpr int kRegion = kR * kR * kR;
-constexpr int kMaxCand = 24;
+static_assert(kMeshCandWords == 8 * kSlots * 256, "one centre per (corner, slot) for each of 256 lanes");
 
 // Per-voxel bits of the staged region.
 constexpr uint8_t kBitAllocated = 8;   // the voxel's block is allocated
@@ -37,12 +37,10 @@ constexpr uint8_t kBitBelow = 16;      // some observed slot's corner value may 
 constexpr uint8_t kBitAbove = 32;      // some observed slot's corner value may be >= iso
 
 struct MeshShared {
-  float nx[kSlots][kRegion], ny[kSlots][kRegion], nz[kSlots][kRegion], po[kSlots][kRegion];
-  float sdv[kSlots][kRegion];  // plane signed distance of slot k at the voxel's own centre
+  float4 plane[kSlots][kRegion];  // slot k's unit normal (x, y, z) and plane offset (w): one 16-B read
   uint8_t bits[kRegion];       // bit k: slot k observed; kBit* above
   uint32_t src[8];        // pool index of self + 7 positive neighbours (kInvalid if absent)
   uint32_t any_dir;       // OR over the 8 source blocks' direction masks
-  float cand[kMaxCand][256];
   uint32_t scan[256];
   uint32_t total;
   // cubes that may produce triangles, compacted so that busy lanes are dense
@@ -117,11 +115,7 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
           const F3 n = unpackNormalOct(slot.w);
           const float po = __uint_as_float(slot.z);
           const float v = dot3(n, centre) - po;
-          S.nx[k][r] = n.x;
-          S.ny[k][r] = n.y;
-          S.nz[k][r] = n.z;
-          S.po[k][r] = po;
-          S.sdv[k][r] = v;
+          S.plane[k][r] = make_float4(n.x, n.y, n.z, po);
           if (!(v >= below)) b |= kBitBelow;
           if (!(v <= above)) b |= kBitAbove;
         }
@@ -134,7 +128,8 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
 }
 
 __device__ __forceinline__ float planeSD(const MeshShared& S, int k, int r, F3 p) {
-  return dot3(F3{S.nx[k][r], S.ny[k][r], S.nz[k][r]}, p) - S.po[k][r];
+  const float4 q = S.plane[k][r];
+  return dot3(F3{q.x, q.y, q.z}, p) - q.w;
 }
 
 // completeCubeValuesWithFallback (voxel_hash_tsdf.h:423-477).
@@ -296,13 +291,17 @@ __device__ __forceinline__ void sortPairs(float* v) {
 // values, completion and the triangles (909-934).
 template <bool kEmit>
 __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
-                                                float* out) {
+                                                float* __restrict__ cand, float* out) {
   const int tid = threadIdx.x;
   const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
   const int ox = keyX(key) * kBlockSide + lx, oy = keyY(key) * kBlockSide + ly, oz = keyZ(key) * kBlockSide + lz;
   const float hv = 0.5f * a.voxel;
   const F3 cc{(static_cast<float>(ox) + 0.5f) * a.voxel + hv, (static_cast<float>(oy) + 0.5f) * a.voxel + hv,
               (static_cast<float>(oz) + 0.5f) * a.voxel + hv};
+  // centres of the corner voxels along each axis (offset 0 and 1)
+  const float cx[2] = {(static_cast<float>(ox) + 0.5f) * a.voxel, (static_cast<float>(ox + 1) + 0.5f) * a.voxel};
+  const float cy[2] = {(static_cast<float>(oy) + 0.5f) * a.voxel, (static_cast<float>(oy + 1) + 0.5f) * a.voxel};
+  const float cz[2] = {(static_cast<float>(oz) + 0.5f) * a.voxel, (static_cast<float>(oz + 1) + 0.5f) * a.voxel};
   int rc[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c)
@@ -357,13 +356,13 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
         centre = (centre * static_cast<float>(count) + sk) / static_cast<float>(count + 1);
         ++count;
       } else {
-        S.cand[ncl++][tid] = centre;
+        cand[256 * ncl++ + tid] = centre;
         centre = sk;
         count = 1;
       }
     }
   }
-  S.cand[ncl++][tid] = centre;
+  cand[256 * ncl++ + tid] = centre;
 
   // (3) per cluster.  A selected slot lies within extract_tol of the
   // centre at the cell centre, and its value at a corner centre differs from
@@ -376,7 +375,7 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
                       (fabsf(cc.x) + fabsf(cc.y) + fabsf(cc.z) + 2.0f * a.voxel) * 0x1.0p-20f;
   uint32_t ntri = 0;
   for (int j = 0; j < ncl; ++j) {
-    const float cen = S.cand[j][tid];
+    const float cen = cand[256 * j + tid];
     if (cen - reach > a.iso || cen + reach < a.iso) continue;
     float val[8];
     uint32_t valid = 0;
@@ -394,7 +393,8 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
         }
       }
       if (best < 0) continue;
-      val[c] = S.sdv[best][rc[c]];  // planeSD at the corner's centre, staged
+      // planeSD at the corner voxel's own centre, formed as the staging forms it
+      val[c] = planeSD(S, best, rc[c], F3{cx[kMcCornerOffset[c][0]], cy[kMcCornerOffset[c][1]], cz[kMcCornerOffset[c][2]]});
       valid |= 1u << c;
     }
     if (completeCube(val, valid)) {
@@ -442,7 +442,8 @@ __device__ __forceinline__ void listAppend(MeshShared& S, bool take, int cube, u
 // Per-cube counts go to cube_counts[0..511] (one byte per cube: at most
 // 24 clusters x 5 triangles) for the emit pass.  The cubes passing
 // cubeMayCross are compacted first and then spread over the lanes.
-__device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a,
+__device__ uint32_t countItem(MeshShared& S, float* __restrict__ cand, const VolumeView& V, uint32_t b, int dir,
+                              const MeshArgs& a,
                               uint8_t* __restrict__ cube_counts) {
   const uint64_t key = V.pool_keys[b];
   if (!stageRegion(S, V, key, b, dir, a)) {
@@ -463,7 +464,7 @@ __device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, in
   uint32_t mine = 0;
   for (uint32_t j = threadIdx.x; j < n; j += 256) {
     const int cube = S.list_cube[j];
-    const uint32_t c = processCube<false>(S, key, cube, a, nullptr);
+    const uint32_t c = processCube<false>(S, key, cube, a, cand, nullptr);
     cube_counts[cube] = static_cast<uint8_t>(c);
     mine += c;
   }
@@ -477,7 +478,8 @@ __device__ uint32_t countItem(MeshShared& S, const VolumeView& V, uint32_t b, in
 // cube order, running only the cubes the count pass found triangles in
 // (compacted over the lanes); tri_owner (optional) receives owner for each
 // triangle.
-__device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir, const MeshArgs& a, uint32_t base,
+__device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeView& V, uint32_t b, int dir,
+                         const MeshArgs& a, uint32_t base,
                          uint32_t total, const uint8_t* __restrict__ cube_counts, float* tris, uint32_t* tri_owner,
                          uint32_t owner, uint32_t capacity) {
   if (total == 0) return;  // workgroup-uniform
@@ -501,7 +503,7 @@ __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir
     const uint32_t start = S.list_start[j];
     const uint32_t c = cube_counts[cube];
     if (start + c <= capacity) {
-      processCube<true>(S, key, cube, a, tris + 9ull * start);
+      processCube<true>(S, key, cube, a, cand, tris + 9ull * start);
       if (tri_owner)
         for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = owner;
     }
@@ -509,38 +511,43 @@ __device__ void emitItem(MeshShared& S, const VolumeView& V, uint32_t b, int dir
   __syncthreads();
 }
 
-// Grid (n, 6): workgroup (i, dir) meshes block list[i] (or pool index i
-// when list is null) in direction dir.  Item dir * n + i: total in
-// counts[item], per-cube counts in cube_counts[512 * item ..].
-// dirs (optional): per listed block, the directions to mesh; the others
-// count zero triangles.
+// Persistent over the 6 n items: item dir * n + i meshes block list[i] (or
+// pool index i when list is null) in direction dir; total in counts[item],
+// per-cube counts in cube_counts[512 * item ..].  dirs (optional): per
+// listed block, the directions to mesh; the others count zero triangles.
+// cand: kMeshCandWords floats of cluster centres per workgroup.
 __global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
                                                        MeshArgs a, uint32_t* __restrict__ counts,
                                                        uint8_t* __restrict__ cube_counts,
-                                                       const uint8_t* __restrict__ dirs) {
+                                                       const uint8_t* __restrict__ dirs, float* __restrict__ cand) {
   __shared__ MeshShared S;
-  const uint32_t i = blockIdx.x, dir = blockIdx.y;
-  const size_t item = static_cast<size_t>(dir) * n + i;
-  if (dirs && !(dirs[i] & (1u << dir))) {
-    if (threadIdx.x == 0) counts[item] = 0u;
-    return;
+  float* const mine = cand + kMeshCandWords * blockIdx.x;
+  const uint32_t items = kDirs * n;
+  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    const uint32_t dir = item / n, i = item % n;
+    uint32_t total = 0;
+    if (!dirs || (dirs[i] & (1u << dir)))  // workgroup-uniform
+      total = countItem(S, mine, V, list ? list[i] : i, static_cast<int>(dir), a, cube_counts + 512ull * item);
+    if (threadIdx.x == 0) counts[item] = total;
   }
-  const uint32_t total = countItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, cube_counts + 512 * item);
-  if (threadIdx.x == 0) counts[item] = total;
 }
 
-// Writes the triangles of workgroup (i, dir) at offsets[dir * n + i] in cube
-// order; tri_owner (optional) receives the item dir * n + i for each triangle.
+// Writes the triangles of item dir * n + i at offsets[item] in cube order;
+// tri_owner (optional) receives the item for each triangle.
 __global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
                                                       MeshArgs a, const uint32_t* __restrict__ counts,
                                                       const uint8_t* __restrict__ cube_counts,
                                                       const uint32_t* __restrict__ offsets, float* __restrict__ tris,
-                                                      uint32_t* __restrict__ tri_owner, uint32_t capacity) {
+                                                      uint32_t* __restrict__ tri_owner, uint32_t capacity,
+                                                      float* __restrict__ cand) {
   __shared__ MeshShared S;
-  const uint32_t i = blockIdx.x, dir = blockIdx.y;
-  const size_t item = static_cast<size_t>(dir) * n + i;
-  emitItem(S, V, list ? list[i] : i, static_cast<int>(dir), a, offsets[item], counts[item], cube_counts + 512 * item,
-           tris, tri_owner, static_cast<uint32_t>(item), capacity);
+  float* const mine = cand + kMeshCandWords * blockIdx.x;
+  const uint32_t items = kDirs * n;
+  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    const uint32_t dir = item / n, i = item % n;
+    emitItem(S, mine, V, list ? list[i] : i, static_cast<int>(dir), a, offsets[item], counts[item],
+             cube_counts + 512ull * item, tris, tri_owner, item, capacity);
+  }
 }
 
 // (Block, direction) pairs whose cubes read a written direction of a dirty
@@ -749,8 +756,10 @@ __global__ void meshPrefixKernel(const BatchScene* __restrict__ scenes, int S, u
 }
 
 __global__ __launch_bounds__(256) void meshCountBatchKernel(const BatchScene* __restrict__ scenes, int S,
-                                                            const uint32_t* __restrict__ prefix) {
+                                                            const uint32_t* __restrict__ prefix,
+                                                            float* __restrict__ cand) {
   __shared__ MeshShared M;
+  float* const mine = cand + kMeshCandWords * blockIdx.x;
   const uint32_t total = prefix[S];
   for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
     const int s = findSegment(prefix, S, i);
@@ -759,7 +768,7 @@ __global__ __launch_bounds__(256) void meshCountBatchKernel(const BatchScene* __
     const uint32_t dir = j / m, idx = j % m;
     uint32_t c = 0;
     if (sc.remesh_dirs[idx] & (1u << dir))  // workgroup-uniform
-      c = countItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma,
+      c = countItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma,
                     sc.cube_counts + 512ull * (dir * m + idx));
     if (threadIdx.x == 0) sc.mesh_counts[dir * m + idx] = c;
   }
@@ -772,25 +781,29 @@ __global__ __launch_bounds__(256) void meshScanBatchKernel(const BatchScene* __r
 }
 
 __global__ __launch_bounds__(256) void meshEmitBatchKernel(const BatchScene* __restrict__ scenes, int S,
-                                                           const uint32_t* __restrict__ prefix) {
+                                                           const uint32_t* __restrict__ prefix,
+                                                           float* __restrict__ cand) {
   __shared__ MeshShared M;
+  float* const mine = cand + kMeshCandWords * blockIdx.x;
   const uint32_t total = prefix[S];
   for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
     const int s = findSegment(prefix, S, i);
     const BatchScene& sc = scenes[s];
     const uint32_t m = sc.out[0], j = i - prefix[s];
     const uint32_t dir = j / m, idx = j % m, item = dir * m + idx;
-    emitItem(M, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, sc.mesh_offsets[item],
+    emitItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, sc.mesh_offsets[item],
              sc.mesh_counts[item], sc.cube_counts + 512ull * item, sc.tris, nullptr, 0u, sc.tri_capacity);
   }
 }
 
 }  // namespace
 
+static uint32_t meshGrid(uint32_t items) { return items < kMeshGridMax ? items : kMeshGridMax; }
+
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           uint32_t* counts, uint8_t* cube_counts, const uint8_t* dirs, hipStream_t s) {
+                           uint32_t* counts, uint8_t* cube_counts, const uint8_t* dirs, float* cand, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  meshCountKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts, cube_counts, dirs);
+  meshCountKernel<<<meshGrid(kDirs * n), 256, 0, s>>>(V, list, n, a, counts, cube_counts, dirs, cand);
   return hipGetLastError();
 }
 
@@ -813,10 +826,10 @@ hipError_t launchScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* p
 
 hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                           const uint32_t* counts, const uint8_t* cube_counts, const uint32_t* offsets, float* tris,
-                          uint32_t* tri_owner, uint32_t capacity, hipStream_t s) {
+                          uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  meshEmitKernel<<<dim3(n, kDirs), 256, 0, s>>>(V, list, n, a, counts, cube_counts, offsets, tris, tri_owner,
-                                                 capacity);
+  meshEmitKernel<<<meshGrid(kDirs * n), 256, 0, s>>>(V, list, n, a, counts, cube_counts, offsets, tris, tri_owner,
+                                                      capacity, cand);
   return hipGetLastError();
 }
 
@@ -844,13 +857,15 @@ hipError_t launchCompactFlags(uint32_t* flags, const uint32_t* pos, uint32_t n, 
   return hipGetLastError();
 }
 
-hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, hipStream_t s) {
+hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, float* cand,
+                           hipStream_t s) {
+  const uint32_t g = meshGrid(grid > 0 ? static_cast<uint32_t>(grid) : 1u);
   markRemeshBatchKernel<<<dim3(64, S), 256, 0, s>>>(scenes);
   compactRemeshBatchKernel<<<S, 256, 0, s>>>(scenes);
   meshPrefixKernel<<<1, 64, 0, s>>>(scenes, S, prefix);
-  meshCountBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix);
+  meshCountBatchKernel<<<g, 256, 0, s>>>(scenes, S, prefix, cand);
   meshScanBatchKernel<<<S, 256, 0, s>>>(scenes);
-  meshEmitBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix);
+  meshEmitBatchKernel<<<g, 256, 0, s>>>(scenes, S, prefix, cand);
   return hipGetLastError();
 }
 
