@@ -167,6 +167,7 @@ __global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ 
 
 __global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) {
   hdr->active_count = 0u;
+  hdr->defer_count = 0u;
   hdr->frame = frame;
 }
 

@@ -70,6 +70,9 @@ struct vhm_ctx {
   uint32_t* remesh_pos = nullptr;
   uint32_t* remesh_list = nullptr;
   uint8_t* remesh_dirs = nullptr;  // directions to re-mesh per listed block
+  // integrator's deferred border blocks (max_blocks each; flags kept zero between frames)
+  uint32_t* defer_flags = nullptr;
+  uint32_t* defer_list = nullptr;
   uint32_t* remesh_partials = nullptr;
   uint32_t* tri_owner = nullptr;
   size_t tri_owner_cap = 0;
@@ -118,6 +121,8 @@ VolumeView view(vhm_ctx* c) {
   V.max_blocks = c->P.max_blocks;
   V.active = c->active;
   V.hdr = c->hdr;
+  V.defer_flags = c->defer_flags;
+  V.defer_list = c->defer_list;
   return V;
 }
 
@@ -315,6 +320,7 @@ int initVolume(vhm_ctx* c) {
   VHM_TRY(c, hipMemsetAsync(c->dirty, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset dirty");
   VHM_TRY(c, hipMemsetAsync(c->dirmask, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset direction masks");
   VHM_TRY(c, hipMemsetAsync(c->remesh_flags, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset remesh flags");
+  VHM_TRY(c, hipMemsetAsync(c->defer_flags, 0, size_t(c->P.max_blocks) * 4u, s), "hipMemset deferred flags");
   VHM_TRY(c, hipMemsetAsync(c->hdr, 0, sizeof(DevHeader), s), "hipMemset header");
   VHM_TRY(c, hipStreamSynchronize(s), "hipStreamSynchronize");
   c->frame = 0;
@@ -429,6 +435,8 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->remesh_pos, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_list, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_dirs, size_t(p->max_blocks));
+  if (e == hipSuccess) e = hipMalloc(&c->defer_flags, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->defer_list, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_partials, scanPartialsSize(p->max_blocks) * 4u);
   if (e != hipSuccess) {
     vhm_destroy(c);
@@ -452,7 +460,8 @@ int vhm_destroy(vhm_ctx* c) {
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
-                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand};
+                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand,
+                  c->defer_flags, c->defer_list};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->stream) hipStreamDestroy(c->stream);

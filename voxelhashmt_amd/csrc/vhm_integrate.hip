@@ -65,7 +65,20 @@ struct IntegrateTally {
   unsigned long long updates = 0, new_obs = 0, visited = 0;
 };
 
+// Which voxels of a block a pass integrates.  The surface normal of a voxel
+// projecting within 2 px of the top or left image border (or so far out
+// that u + 2 is inexact) needs estimateSurfaceNormalCamera's general path,
+// 45 separate bilinear samples instead of 21 shared ones
+// (fastNormalApplies).  Inline, that path would run for a whole wavefront
+// whenever one lane needs it and would bloat the main kernel's code; so the
+// main pass (kFast) integrates every other voxel and lists such blocks, and
+// a second kernel (kDeferred) integrates just their border voxels.  kAll
+// does both in one pass (the batched step).
+enum class Pass { kAll, kFast, kDeferred };
+constexpr int kDeferredGrid = 512;  // the deferred list is a few percent of the active blocks
+
 // Expensive part for queue entry i (one lane).
+template <Pass P>
 __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, const DepthView<float>& D,
                                                 const NormalParams& np, const VolumeView& V, const IntegrateArgs& a,
                                                 IntegrateTally& t) {
@@ -76,7 +89,8 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
   const uint32_t dmask = V.dirmask[p];
   const uint32_t dirty_word = V.dirty[p];
   const F3 pc{Q.pcx[i], Q.pcy[i], Q.pcz[i]};
-  const VoxelObservation o = observeSurface(D, np, a.T_wc, pc, Q.u[i], Q.v[i], Q.dep[i], a.trunc, a.min_view);
+  const VoxelObservation o = observeSurface<DepthView<float>, P == Pass::kFast>(D, np, a.T_wc, pc, Q.u[i], Q.v[i],
+                                                                                Q.dep[i], a.trunc, a.min_view);
   if (o.dir < 0) return;
   ++t.updates;
   const int d = o.dir;
@@ -112,18 +126,21 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
 
 // Runs the expensive part on queue entries [Q.n - m, Q.n) with m = min(Q.n, 256)
 // and removes them.  Workgroup-wide.
+template <Pass P>
 __device__ __forceinline__ void drainQueue(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
                                            const VolumeView& V, const IntegrateArgs& a, IntegrateTally& t) {
   const uint32_t n = Q.n;
   const uint32_t m = n < 256u ? n : 256u;
-  if (threadIdx.x < m) integrateQueued(Q, static_cast<int>(n - m + threadIdx.x), D, np, V, a, t);
+  if (threadIdx.x < m) integrateQueued<P>(Q, static_cast<int>(n - m + threadIdx.x), D, np, V, a, t);
   __syncthreads();
   if (threadIdx.x == 0) Q.n = n - m;
   __syncthreads();
 }
 
 // Cheap part for all 512 voxels of pool block p: append the voxels that pass
-// observeDepth to the queue.  Requires Q.n < 256 on entry.  Workgroup-wide.
+// observeDepth (and belong to pass P) to the queue.  Requires Q.n < 256 on
+// entry.  Workgroup-wide.
+template <Pass P>
 __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
                                              const VolumeView& V, const IntegrateArgs& a, uint32_t p) {
   const int tid = threadIdx.x, lane = tid & 63;
@@ -138,7 +155,18 @@ __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<
                 (static_cast<float>(bz + lz) + 0.5f) * a.voxel};
     F3 pc;
     float u, v, dep;
-    const bool pass = observeDepth(D, np, a.T_cw, pw, a.trunc, pc, u, v, dep);
+    bool pass = observeDepth(D, np, a.T_cw, pw, a.trunc, pc, u, v, dep);
+    if constexpr (P != Pass::kAll) {
+      const bool border = pass && !fastNormalApplies(u, v);
+      if constexpr (P == Pass::kFast) {
+        // list the block once (the flag is cleared by the deferred pass)
+        if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
+          V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
+        pass = pass && !border;
+      } else {
+        pass = border;
+      }
+    }
     const unsigned long long mask = __ballot(pass);
     uint32_t base = 0;
     if (lane == 0 && mask != 0ull) base = atomicAdd(&Q.n, static_cast<uint32_t>(__popcll(mask)));
@@ -159,18 +187,28 @@ __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<
 }
 
 // Integrates active-list entries [begin, end) of one volume (then drains
-// the queue completely).  Workgroup-wide; Q.n == 0 on entry and exit.
+// the queue completely); for kDeferred, deferred-list entries.
+// Workgroup-wide; Q.n == 0 on entry and exit.
+template <Pass P>
 __device__ __forceinline__ void integrateRange(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
                                const VolumeView& V, const IntegrateArgs& a, uint32_t begin, uint32_t end,
                                IntegrateTally& t) {
   for (uint32_t i = begin; i < end; ++i) {
-    const uint32_t p = V.vals[V.active[i]];
-    if (p >= V.max_blocks) continue;  // pool exhausted for this key (error already flagged)
-    if (threadIdx.x == 0) t.visited += kBlockVoxels;
-    enqueueBlock(Q, D, np, V, a, p);
-    while (Q.n >= 256u) drainQueue(Q, D, np, V, a, t);
+    uint32_t p;
+    if constexpr (P == Pass::kDeferred) {
+      p = V.defer_list[i];
+    } else {
+      p = V.vals[V.active[i]];
+      if (p >= V.max_blocks) continue;  // pool exhausted for this key (error already flagged)
+      if (threadIdx.x == 0) t.visited += kBlockVoxels;
+    }
+    enqueueBlock<P>(Q, D, np, V, a, p);
+    if constexpr (P == Pass::kDeferred) {
+      if (threadIdx.x == 0) V.defer_flags[p] = 0u;
+    }
+    while (Q.n >= 256u) drainQueue<P>(Q, D, np, V, a, t);
   }
-  while (Q.n > 0u) drainQueue(Q, D, np, V, a, t);
+  while (Q.n > 0u) drainQueue<P>(Q, D, np, V, a, t);
 }
 
 __device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& t) {
@@ -203,7 +241,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const uint32_t begin = blockIdx.x * chunk;
   const uint32_t end = begin + chunk < n_active ? begin + chunk : n_active;
   IntegrateTally t;
-  if (begin < end) integrateRange(Q, D, np, V, a, begin, end, t);
+  if (begin < end) integrateRange<Pass::kFast>(Q, D, np, V, a, begin, end, t);
+  flushTally(V, t);
+}
+
+// The deferred pass: the border voxels of the blocks the main pass listed.
+__global__ __launch_bounds__(256) void integrateDeferredKernel(const float* __restrict__ depth, VolumeView V,
+                                                               IntegrateArgs a) {
+  __shared__ IntegrateQueue Q;
+  if (threadIdx.x == 0) Q.n = 0;
+  __syncthreads();
+  const DepthView<float> D{depth, a.K.width, a.K.height};
+  const NormalParams np = normalParams(a);
+  const uint32_t n = V.hdr->defer_count;  // reset by the next frame's frameBeginKernel
+  const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t begin = blockIdx.x * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  IntegrateTally t;
+  if (begin < end) integrateRange<Pass::kDeferred>(Q, D, np, V, a, begin, end, t);
   flushTally(V, t);
 }
 
@@ -231,7 +286,7 @@ __global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __
     const DepthView<float> D{sc.filtered, a.K.width, a.K.height};
     const NormalParams np = normalParams(a);
     IntegrateTally t;
-    integrateRange(Q, D, np, sc.V, a, i - prefix[s], seg_end - prefix[s], t);
+    integrateRange<Pass::kAll>(Q, D, np, sc.V, a, i - prefix[s], seg_end - prefix[s], t);
     flushTally(sc.V, t);
     i = seg_end;
   }
@@ -274,6 +329,7 @@ __global__ __launch_bounds__(256) void gatherBlocksAoSKernel(const int32_t* __re
 hipError_t launchIntegrate(const float* depth, const VolumeView& V, const IntegrateArgs& a, int grid,
                            hipStream_t s) {
   integrateKernel<<<grid, 256, 0, s>>>(depth, V, a);
+  integrateDeferredKernel<<<kDeferredGrid, 256, 0, s>>>(depth, V, a);
   return hipGetLastError();
 }
 

@@ -420,7 +420,8 @@ VHM_HD bool observeDepth(const D& dv, const NormalParams& a, const Pose34& T_cw,
 
 // Second half (voxel_hash_tsdf.h:1023-1046): surface normal, weight,
 // surface point and direction of a voxel that passed observeDepth.
-template <class D>
+// kFastNormal: the caller guarantees fastNormalApplies(u, v).
+template <class D, bool kFastNormal = false>
 VHM_HD VoxelObservation observeSurface(const D& dv, const NormalParams& a, const Pose34& T_wc, F3 pc, float u,
                                        float v, float dep, float trunc, float min_view) {
   VoxelObservation o;
@@ -429,7 +430,12 @@ VHM_HD VoxelObservation observeSurface(const D& dv, const NormalParams& a, const
   const F3 ray = normalized3(pc);
   float vaw = 1.0f;
   F3 nc = mul3(ray, -1.0f);
-  if (surfaceNormal(dv, a, u, v, dep, nc)) vaw = fabsf(dot3(nc, ray));
+  bool have;
+  if constexpr (kFastNormal)
+    have = surfaceNormalFast(dv, a, u, v, dep, nc);
+  else
+    have = surfaceNormal(dv, a, u, v, dep, nc);
+  if (have) vaw = fabsf(dot3(nc, ray));
   const float w = observationWeight(dep, sd, trunc, vaw, min_view);
   if (w <= 0.0f) return o;
   const F3 spw = transformPoint(T_wc, backProjectN(u, v, dep, a));

@@ -48,6 +48,8 @@ struct DevHeader {
   unsigned long long updates;   // voxel updates (weight > 0, valid direction)
   unsigned long long visited;   // voxels visited (512 per integrated block)
   unsigned long long mesh_total;
+  uint32_t defer_count;  // blocks appended to the integrator's deferred list this frame
+  uint32_t pad;
 };
 
 struct VolumeView {
@@ -64,6 +66,12 @@ struct VolumeView {
   uint32_t max_blocks;
   uint32_t* active;
   DevHeader* hdr;
+  // integrator: blocks with voxels projecting within 2 px of the top/left
+  // image border (estimateSurfaceNormalCamera's general path), handled by a
+  // second kernel; defer_flags[p] is 1 while p is listed, 0 otherwise.
+  // Null in volumes driven only by the batched step.
+  uint32_t* defer_flags;
+  uint32_t* defer_list;
 };
 
 // Pool word of (direction, field, slot) of voxel vi within a block.  Layout
