@@ -177,6 +177,7 @@ __global__ void batchFrameBeginKernel(const BatchScene* __restrict__ scenes, int
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   scenes[s].V.hdr->active_count = 0u;
+  scenes[s].V.hdr->defer_count = 0u;
   scenes[s].V.hdr->frame = st->step + 1u;
 }
 

@@ -72,9 +72,8 @@ struct IntegrateTally {
 // (fastNormalApplies).  Inline, that path would run for a whole wavefront
 // whenever one lane needs it and would bloat the main kernel's code; so the
 // main pass (kFast) integrates every other voxel and lists such blocks, and
-// a second kernel (kDeferred) integrates just their border voxels.  kAll
-// does both in one pass (the batched step).
-enum class Pass { kAll, kFast, kDeferred };
+// a second kernel (kDeferred) integrates just their border voxels.
+enum class Pass { kFast, kDeferred };
 constexpr int kDeferredGrid = 512;  // the deferred list is a few percent of the active blocks
 
 // Expensive part for queue entry i (one lane).
@@ -156,16 +155,14 @@ __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<
     F3 pc;
     float u, v, dep;
     bool pass = observeDepth(D, np, a.T_cw, pw, a.trunc, pc, u, v, dep);
-    if constexpr (P != Pass::kAll) {
-      const bool border = pass && !fastNormalApplies(u, v);
-      if constexpr (P == Pass::kFast) {
-        // list the block once (the flag is cleared by the deferred pass)
-        if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
-          V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
-        pass = pass && !border;
-      } else {
-        pass = border;
-      }
+    const bool border = pass && !fastNormalApplies(u, v);
+    if constexpr (P == Pass::kFast) {
+      // list the block once (the flag is cleared by the deferred pass)
+      if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
+        V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
+      pass = pass && !border;
+    } else {
+      pass = border;
     }
     const unsigned long long mask = __ballot(pass);
     uint32_t base = 0;
@@ -277,7 +274,7 @@ __global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __
   uint32_t i = blockIdx.x * chunk;
   const uint32_t end = i + chunk < total ? i + chunk : total;
   while (i < end) {
-    const int s = findSegment(prefix, S, i);
+    const int s = __builtin_amdgcn_readfirstlane(findSegment(prefix, S, i));  // uniform: scene fields in SGPRs
     const BatchScene& sc = scenes[s];
     const uint32_t seg_end = prefix[s + 1] < end ? prefix[s + 1] : end;
     IntegrateArgs a = sc.ia;
@@ -286,10 +283,33 @@ __global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __
     const DepthView<float> D{sc.filtered, a.K.width, a.K.height};
     const NormalParams np = normalParams(a);
     IntegrateTally t;
-    integrateRange<Pass::kAll>(Q, D, np, sc.V, a, i - prefix[s], seg_end - prefix[s], t);
+    integrateRange<Pass::kFast>(Q, D, np, sc.V, a, i - prefix[s], seg_end - prefix[s], t);
     flushTally(sc.V, t);
     i = seg_end;
   }
+}
+
+// Batched deferred pass: workgroup column y takes scene y's deferred list.
+__global__ __launch_bounds__(256) void integrateDeferredBatchKernel(const BatchScene* __restrict__ scenes,
+                                                                    const BatchState* __restrict__ st) {
+  const BatchScene& sc = scenes[blockIdx.y];
+  const uint32_t n = sc.V.hdr->defer_count;  // reset by the next step's frame begin
+  const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t begin = blockIdx.x * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  if (begin >= end) return;  // workgroup-uniform
+  __shared__ IntegrateQueue Q;
+  if (threadIdx.x == 0) Q.n = 0;
+  __syncthreads();
+  const uint32_t step = st->step;
+  IntegrateArgs a = sc.ia;
+  a.T_wc = sc.poses[step];
+  a.T_cw = sc.poses_inv[step];
+  const DepthView<float> D{sc.filtered, a.K.width, a.K.height};
+  const NormalParams np = normalParams(a);
+  IntegrateTally t;
+  integrateRange<Pass::kDeferred>(Q, D, np, sc.V, a, begin, end, t);
+  flushTally(sc.V, t);
 }
 
 // Copies blocks into the reference's AoS Voxel layout (voxel_hash_tsdf.h:18-31):
@@ -336,6 +356,7 @@ hipError_t launchIntegrate(const float* depth, const VolumeView& V, const Integr
 hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t* prefix, const BatchState* st,
                                 int grid, hipStream_t s) {
   integrateBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix, st);
+  integrateDeferredBatchKernel<<<dim3(64, S), 256, 0, s>>>(scenes, st);
   return hipGetLastError();
 }
 
