@@ -116,6 +116,15 @@ int vhm_integrate_f32(vhm_ctx* ctx, const float* depth, int memkind, const vhm_i
 int vhm_integrate_u16(vhm_ctx* ctx, const uint16_t* depth, float depth_scale, int memkind,
                       const vhm_intrinsics* K, const vhm_pose* T_wc, int allocation_stride, uint32_t flags);
 
+/* integratePointCloud (voxel_hash_tsdf.h:75-97, 952-983): n camera-space
+ * points (x, y, z floats, host or device memory) seen from T_wc.  Each
+ * point's ray is sampled every max(voxel / 2, 2.5 mm) across the truncation
+ * band around the point; every sample's block is created and every sample
+ * with weight > 0 updates its voxel, in the reference's order (points in
+ * order, samples along each ray in order).  Non-finite points are skipped.
+ * The volume's frame counter is unchanged.  Synchronous. */
+int vhm_integrate_point_cloud(vhm_ctx* ctx, const float* points, size_t n, int memkind, const vhm_pose* T_wc);
+
 /* The depth bilateral filter alone (applyDepthBilateralFilter,
  * reconstruction_backend.cpp:96-189).  Synchronous. */
 int vhm_filter_depth_f32(vhm_ctx* ctx, const float* depth, int in_memkind, const vhm_intrinsics* K,

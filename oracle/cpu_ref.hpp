@@ -9,8 +9,10 @@
 // the library built from this file (oracle/liboracle.so), and only as the
 // checker or the timed CPU baseline.  The GPU product never links it.
 //
-// Every function cites the reference file:line it restates.  Pinning (what
-// checks each function beyond this restatement) is listed in oracle/README.md.
+// Every function cites the reference file:line it restates.  What pins each
+// function beyond this restatement (the reference's own CPU code compiled
+// from its sources, closed forms, brute force) is in the docstrings of
+// tests/test_oracle_units.py, tests/test_golden.py and tests/test_point_cloud.py.
 
 #pragma once
 
@@ -398,6 +400,12 @@ class Volume {
     return std::vector<Key>(active.begin(), active.end());
   }
 
+  std::vector<VoxelT<T>>& blockFor(const Key& k) {  // blocks_[key] (value-initialised voxels)
+    auto it = blocks.find(k);
+    if (it == blocks.end()) it = blocks.emplace(k, std::vector<VoxelT<T>>(kBlockVol)).first;
+    return it->second;
+  }
+
   std::vector<Key> prepare(const Frame<T>& f, const Pose<T>& T_wc, int stride) {  // :133-145
     std::vector<Key> keys = collectBlocks(f, T_wc, stride);
     for (const Key& k : keys)
@@ -599,6 +607,41 @@ class Volume {
   void integrate(const Frame<T>& f, const Pose<T>& T_wc, int stride) {  // :70-73
     const std::vector<Key> active = prepare(f, T_wc, stride);
     integrateBlocks(active, f, T_wc);
+  }
+
+  // updateVoxelAtWorld (voxel_hash_tsdf.h:952-983): the block is created
+  // before the weight test.
+  void updateVoxelAtWorld(V3<T> pw, T depth, T sd, V3<T> spw, V3<T> nw) {
+    const std::array<int, 3> v = worldToVoxel(pw);
+    std::vector<VoxelT<T>>& blk = blockFor(voxelToKey(v));
+    VoxelT<T>& vx = blk[static_cast<std::size_t>(positiveMod(v[0], kBlock) +
+                                                 kBlock * (positiveMod(v[1], kBlock) + kBlock * positiveMod(v[2], kBlock)))];
+    const T w = observationWeight(depth, sd, trunc(), T(1), T(P.min_view_angle_weight));
+    if (w <= T(0)) return;
+    const int d = dominantDirection(nw);
+    if (d < 0) return;
+    const T coord = dot(spw, dirUnit<T>(d));
+    const T tol = smax(T(1e-6f), voxel() * T(P.layer_merge_scale));
+    ++updates;
+    integrateSample(vx, d, coord, spw, nw, w, tol, T(P.layer_normal_cos), T(P.max_weight));
+  }
+
+  // integratePointCloud (voxel_hash_tsdf.h:75-97): samples every
+  // max(voxel / 2, 2.5 mm) along each point's ray across the truncation band.
+  void integratePointCloud(const std::vector<V3<T>>& points_c, const Pose<T>& T_wc) {
+    const V3<T> origin{T_wc.t[0], T_wc.t[1], T_wc.t[2]};
+    const T step = smax(T(0.5f) * voxel(), T(0.0025f));
+    for (const V3<T>& pc : points_c) {
+      const T range = norm(pc);
+      if (range <= T(1e-6f)) continue;
+      const V3<T> ray_c = pc / range;
+      const V3<T> ray_w = normalized(T_wc.transformVector(ray_c));
+      const V3<T> spw = T_wc.transformPoint(pc);
+      const T start = smax(T(0), range - trunc());
+      const T end = range + trunc();
+      for (T s = start; s <= end; s += step)
+        updateVoxelAtWorld(origin + ray_w * s, range, range - s, spw, ray_w * T(-1));
+    }
   }
 
   static bool observed(const VoxelT<T>& vx) {  // :215-224

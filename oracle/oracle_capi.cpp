@@ -113,11 +113,27 @@ std::size_t meshImpl(const Volume<T>& vol, float iso, float* tris, std::size_t c
   return t.size();
 }
 
+template <typename T>
+void pointCloudImpl(Volume<T>& vol, const float* pts, std::size_t n, const float* p12) {
+  std::vector<V3<T>> pc(n);
+  for (std::size_t i = 0; i < n; ++i) pc[i] = {T(pts[3 * i]), T(pts[3 * i + 1]), T(pts[3 * i + 2])};
+  vol.integratePointCloud(pc, makePose<T>(p12));
+}
+
 }  // namespace
 
 extern "C" {
 
 std::size_t vor_sizeof_params() { return sizeof(Params); }
+
+// integratePointCloud (voxel_hash_tsdf.h:75-97): n camera-space points.
+void vor_integrate_point_cloud(void* hv, const float* pts, std::size_t n, const float* p12) {
+  auto* hd = static_cast<Handle*>(hv);
+  if (hd->fp64)
+    pointCloudImpl(*hd->d, pts, n, p12);
+  else
+    pointCloudImpl(*hd->f, pts, n, p12);
+}
 
 void vor_default_params(Params* p) { *p = Params{}; }
 

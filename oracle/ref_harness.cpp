@@ -9,6 +9,7 @@
 // copied here.  This file only adds an extern "C" surface so that Python tests
 // can drive the reference's CPU functions:
 //   * TsdfIntegrationBackend(kCpu)::integrate  (reconstruction_backend.cpp:259-266)
+//   * VoxelHashTSDF::integratePointCloud       (voxel_hash_tsdf.h:75-97)
 //   * applyDepthBilateralFilter                (reconstruction_backend.cpp:96-189)
 //   * VoxelHashTSDF::{prepareFrameIntegration, integrateBlocks, extractMesh,
 //     blockCount, observedVoxelCount, copyBlocksToFlat} (voxel_hash_tsdf.h)
@@ -76,6 +77,13 @@ void* vref_volume_create(float voxel_size, float truncation) {
 }
 
 void vref_volume_destroy(void* vol) { delete static_cast<VoxelHashTSDF*>(vol); }
+
+// VoxelHashTSDF::integratePointCloud (voxel_hash_tsdf.h:75-97): n camera-space points.
+void vref_integrate_point_cloud(void* vol, const float* pts, std::size_t n, const float* pose12) {
+  std::vector<Vec3f> points(n);
+  for (std::size_t i = 0; i < n; ++i) points[i] = Vec3f{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  static_cast<VoxelHashTSDF*>(vol)->integratePointCloud(points, makePose(pose12));
+}
 
 // TsdfIntegrationBackend(kCpu).integrate when apply_filter != 0; the bare
 // VoxelHashTSDF::integrate (no filter) otherwise.

@@ -55,6 +55,7 @@ def oracle():
             "vor_destroy": (None, [VP]),
             "vor_filter": (None, [C.POINTER(OracleParams), C.c_int, VP, C.c_int, C.c_int, VP, VP]),
             "vor_integrate": (None, [VP, VP, C.c_int, C.c_int, VP, VP, C.c_int, C.c_int]),
+            "vor_integrate_point_cloud": (None, [VP, VP, C.c_size_t, VP]),
             "vor_prepare": (C.c_size_t, [VP, VP, C.c_int, C.c_int, VP, VP, C.c_int, VP, C.c_size_t]),
             "vor_integrate_blocks": (None, [VP, VP, C.c_size_t, VP, C.c_int, C.c_int, VP, VP]),
             "vor_block_count": (C.c_size_t, [VP]),
@@ -98,6 +99,7 @@ def ref():
         sig = {
             "vref_volume_create": (VP, [C.c_float, C.c_float]),
             "vref_volume_destroy": (None, [VP]),
+            "vref_integrate_point_cloud": (None, [VP, VP, C.c_size_t, VP]),
             "vref_integrate": (None, [VP, VP, C.c_int, C.c_int, VP, VP, C.c_int, C.c_int]),
             "vref_filter": (None, [VP, C.c_int, C.c_int, VP, VP]),
             "vref_pixel_normal": (C.c_int, [VP, C.c_int, C.c_int, VP, C.c_int, C.c_int, VP]),
@@ -203,6 +205,10 @@ class OracleVolume:
         d = np.ascontiguousarray(depth_m, np.float32)
         oracle().vor_integrate(self.h, p(d), K.width, K.height, p(k4(K)), p(pose12), stride, int(apply_filter))
 
+    def integrate_point_cloud(self, points_c: np.ndarray, pose12: np.ndarray):
+        pts = np.ascontiguousarray(points_c, np.float32).reshape(-1, 3)
+        oracle().vor_integrate_point_cloud(self.h, p(pts), len(pts), p(np.ascontiguousarray(pose12, np.float32)))
+
     def prepare(self, depth_m, K, pose12, stride=1) -> np.ndarray:
         d = np.ascontiguousarray(depth_m, np.float32)
         cap = 1 << 22
@@ -261,6 +267,10 @@ class RefVolume:
     def integrate(self, depth_m, K, pose12, stride=1, apply_filter=True):
         d = np.ascontiguousarray(depth_m, np.float32)
         ref().vref_integrate(self.h, p(d), K.width, K.height, p(k4(K)), p(pose12), stride, int(apply_filter))
+
+    def integrate_point_cloud(self, points_c: np.ndarray, pose12: np.ndarray):
+        pts = np.ascontiguousarray(points_c, np.float32).reshape(-1, 3)
+        ref().vref_integrate_point_cloud(self.h, p(pts), len(pts), p(np.ascontiguousarray(pose12, np.float32)))
 
     def prepare(self, depth_m, K, pose12, stride=1) -> np.ndarray:
         d = np.ascontiguousarray(depth_m, np.float32)

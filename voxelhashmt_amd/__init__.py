@@ -137,6 +137,7 @@ _SIGS = {
     "vhm_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
     "vhm_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "vhm_integrate_point_cloud": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.POINTER(Pose)]),
     # include/vhm_io.h (host only)
     "vhm_io_last_error": (C.c_char_p, []),
     "vhm_demo_intrinsics": (None, [C.c_int32, C.c_int32, C.POINTER(Intrinsics)]),
@@ -297,6 +298,14 @@ class Volume:
         else:
             self._check(lib().vhm_integrate_f32(self._h, _ptr(depth), _kind(depth), C.byref(K), C.byref(T_wc),
                                                 stride, flags))
+
+    def integrate_point_cloud(self, points, T_wc: Pose):
+        """integratePointCloud (voxel_hash_tsdf.h:75-97): n x 3 float32
+        camera-space points (numpy, or a torch CUDA tensor)."""
+        if isinstance(points, np.ndarray):
+            points = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        n = int(points.shape[0])
+        self._check(lib().vhm_integrate_point_cloud(self._h, _ptr(points), n, _kind(points), C.byref(T_wc)))
 
     def filter_depth(self, depth, K: Intrinsics, depth_scale: float = 0.001) -> np.ndarray:
         out = np.zeros((K.height, K.width), dtype=np.float32)

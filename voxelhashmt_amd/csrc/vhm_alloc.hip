@@ -16,6 +16,7 @@
 // the key's slot to the frame's active list; inserting lanes take a pool index
 // with one atomic add.  LDS overflow falls back to direct global inserts, so
 // the resulting key set equals the reference's for every input.
+#include "vhm_hash.h"
 #include "vhm_internal.h"
 
 #pragma clang fp contract(off)
@@ -32,45 +33,9 @@ __device__ __forceinline__ uint32_t ldsSlot(uint64_t key, int bits) {
   return static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 
-// Inserts key into the persistent table; returns its slot (kInvalid if the
-// probe bound is exhausted).  `created` is set for the inserting lane.
-__device__ uint32_t findOrInsert(const VolumeView& V, uint64_t key, bool& created) {
-  uint32_t slot = hashSlot(key, V.log2cap);
-  const int bound = (V.mask + 1u) < static_cast<uint32_t>(kMaxProbe) ? static_cast<int>(V.mask + 1u) : kMaxProbe;
-  for (int probe = 0; probe < bound; ++probe) {
-    const uint64_t cur = __hip_atomic_load(&V.keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur == key) return slot;
-    if (cur == kEmptyKey) {
-      const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&V.keys[slot]),
-                                      static_cast<unsigned long long>(kEmptyKey),
-                                      static_cast<unsigned long long>(key));
-      if (prev == kEmptyKey) {
-        created = true;
-        return slot;
-      }
-      if (prev == key) return slot;
-    }
-    slot = (slot + 1u) & V.mask;
-  }
-  return kInvalid;
-}
-
 __device__ void activateKey(const VolumeView& V, uint64_t key, uint32_t frame) {
-  bool created = false;
-  const uint32_t slot = findOrInsert(V, key, created);
-  if (slot == kInvalid) {
-    atomicOr(&V.hdr->error, kErrHashFull);
-    return;
-  }
-  if (created) {
-    const uint32_t p = atomicAdd(&V.hdr->num_blocks, 1u);
-    if (p < V.max_blocks) {
-      V.vals[slot] = p;
-      V.pool_keys[p] = key;
-    } else {
-      atomicOr(&V.hdr->error, kErrPoolFull);
-    }
-  }
+  const uint32_t slot = insertBlock(V, key);
+  if (slot == kInvalid) return;
   if (atomicExch(&V.stamp[slot], frame) != frame) {
     const uint32_t i = atomicAdd(&V.hdr->active_count, 1u);
     V.active[i] = slot;  // i < 2^k: at most one entry per table slot per frame

@@ -7,6 +7,7 @@
 // cuda_block_allocator.h, cuda_mesh_extractor.h) and its process-global
 // singletons (cuda_tsdf_integrator.cu:114-124) with an explicit context that
 // owns a pre-reserved block pool, a persistent hash table and one stream.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -73,6 +74,22 @@ struct vhm_ctx {
   // integrator's deferred border blocks (max_blocks each; flags kept zero between frames)
   uint32_t* defer_flags = nullptr;
   uint32_t* defer_list = nullptr;
+  // point clouds (grown on demand)
+  float* pc_points = nullptr;
+  size_t pc_points_cap = 0;
+  uint8_t* pc_info = nullptr;
+  size_t pc_info_cap = 0;
+  uint64_t* pc_keys = nullptr;
+  uint64_t* pc_keys_tmp = nullptr;
+  uint32_t* pc_vals = nullptr;
+  uint32_t* pc_vals_tmp = nullptr;
+  size_t pc_keys_cap = 0, pc_keys_tmp_cap = 0, pc_vals_cap = 0, pc_vals_tmp_cap = 0;
+  uint32_t* pc_count = nullptr;
+  size_t pc_count_cap = 0;
+  uint32_t* pc_hist = nullptr;
+  uint32_t* pc_offsets = nullptr;
+  uint32_t* pc_partials = nullptr;
+  size_t pc_hist_cap = 0, pc_offsets_cap = 0, pc_partials_cap = 0;
   uint32_t* remesh_partials = nullptr;
   uint32_t* tri_owner = nullptr;
   size_t tri_owner_cap = 0;
@@ -461,7 +478,8 @@ int vhm_destroy(vhm_ctx* c) {
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
                   c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand,
-                  c->defer_flags, c->defer_list};
+                  c->defer_flags, c->defer_list, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
+                  c->pc_vals, c->pc_vals_tmp, c->pc_count, c->pc_hist, c->pc_offsets, c->pc_partials};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -537,6 +555,90 @@ int vhm_allocate_active_blocks(vhm_ctx* c, const float* depth, int memkind, cons
             "hipMemcpy D2H keys");
     VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   }
+  return VHM_OK;
+}
+
+// Records per point-cloud chunk: keys, values and their sort temporaries
+// take 24 B each (400 MB at this bound).
+constexpr size_t kPointCloudMaxRecords = size_t(1) << 24;
+
+int vhm_integrate_point_cloud(vhm_ctx* c, const float* points, size_t n, int memkind, const vhm_pose* T_wc) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  if (!T_wc || (n && !points)) return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: null argument");
+  if (memkind != VHM_HOST && memkind != VHM_DEVICE)
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: bad memkind");
+  if (n == 0) return VHM_OK;
+  PointCloudArgs a{};
+  a.T_wc = toPose(T_wc);
+  a.voxel = c->P.voxel_size;
+  a.trunc = c->P.truncation_distance;
+  a.step = std::max(0.5f * c->P.voxel_size, 0.0025f);  // voxel_hash_tsdf.h:77
+  a.min_view = c->P.min_view_angle_weight;
+  a.merge_tol = std::max(1e-6f, c->P.voxel_size * c->P.layer_merge_scale);
+  a.min_cos = c->P.layer_normal_cos;
+  a.max_weight = c->P.max_weight;
+  if (!(a.trunc >= 0.0f) || !std::isfinite(a.trunc) || !(a.step > 0.0f))
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: bad truncation or voxel size");
+  // samples of one ray: at most floor(2 trunc / step) + 1, plus one for the
+  // rounding of the running sum and one of margin
+  const double per_ray = std::floor(2.0 * double(a.trunc) / double(a.step)) + 3.0;
+  if (per_ray > double(kPointCloudMaxRecords))
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: truncation / step too large");
+  a.samples_max = static_cast<uint32_t>(per_ray);
+  // key = slot (log2 buckets) | voxel (9) | direction (3) | sequence: at most 64 bits
+  const int free_bits = 64 - int(c->P.hash_log2_buckets) - 12;
+  size_t chunk = kPointCloudMaxRecords / a.samples_max;
+  const size_t by_bits = free_bits >= 62 ? chunk : (size_t(1) << free_bits) / a.samples_max;
+  chunk = std::min(std::min(chunk, by_bits), n);
+  if (chunk == 0) return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: hash table too large for the record key");
+  int seq_bits = 0;
+  while ((uint64_t(1) << seq_bits) < uint64_t(chunk) * a.samples_max) ++seq_bits;
+  a.seq_bits = seq_bits;
+  const int key_bits = int(c->P.hash_log2_buckets) + 12 + seq_bits;
+  const size_t recs = chunk * a.samples_max;
+  int rc;
+  if ((rc = ensure(c, &c->pc_info, &c->pc_info_cap, chunk * pointInfoBytes(), "hipMalloc point info"))) return rc;
+  if ((rc = ensure(c, &c->pc_keys, &c->pc_keys_cap, recs, "hipMalloc point records"))) return rc;
+  if ((rc = ensure(c, &c->pc_keys_tmp, &c->pc_keys_tmp_cap, recs, "hipMalloc point records"))) return rc;
+  if ((rc = ensure(c, &c->pc_vals, &c->pc_vals_cap, recs, "hipMalloc point records"))) return rc;
+  if ((rc = ensure(c, &c->pc_vals_tmp, &c->pc_vals_tmp_cap, recs, "hipMalloc point records"))) return rc;
+  if ((rc = ensure(c, &c->pc_count, &c->pc_count_cap, 1, "hipMalloc record count"))) return rc;
+  const size_t hist = radixSortCountsSize(static_cast<uint32_t>(recs));
+  if ((rc = ensure(c, &c->pc_hist, &c->pc_hist_cap, hist, "hipMalloc sort counts"))) return rc;
+  if ((rc = ensure(c, &c->pc_offsets, &c->pc_offsets_cap, hist, "hipMalloc sort counts"))) return rc;
+  if ((rc = ensure(c, &c->pc_partials, &c->pc_partials_cap, scanPartialsSize(static_cast<uint32_t>(hist)),
+                   "hipMalloc sort partials")))
+    return rc;
+  if (memkind == VHM_HOST &&
+      (rc = ensure(c, &c->pc_points, &c->pc_points_cap, 3 * chunk, "hipMalloc point staging")))
+    return rc;
+  const VolumeView V = view(c);
+  for (size_t first = 0; first < n; first += chunk) {
+    const uint32_t m = static_cast<uint32_t>(std::min(chunk, n - first));
+    const float* pts = points + 3 * first;
+    if (memkind == VHM_HOST) {
+      VHM_TRY(c, hipMemcpyAsync(c->pc_points, pts, sizeof(float) * 3 * m, hipMemcpyHostToDevice, c->stream),
+              "hipMemcpy H2D points");
+      pts = c->pc_points;
+    }
+    VHM_TRY(c, hipMemsetAsync(c->pc_count, 0, sizeof(uint32_t), c->stream), "hipMemset record count");
+    VHM_TRY(c, launchPointCloudSample(pts, m, V, a, c->pc_info, c->pc_keys, c->pc_vals, c->pc_count, c->stream),
+            "point cloud sample kernel launch");
+    uint32_t nrec = 0;
+    VHM_TRY(c, hipMemcpyAsync(&nrec, c->pc_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H record count");
+    VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    VHM_TRY(c, launchRadixSort(c->pc_keys, c->pc_vals, c->pc_keys_tmp, c->pc_vals_tmp, c->pc_count, nrec, key_bits,
+                               c->pc_hist, c->pc_offsets, c->pc_partials, c->stream),
+            "point cloud sort launch");
+    VHM_TRY(c, launchPointCloudApply(c->pc_keys, c->pc_vals, c->pc_count, nrec, c->pc_info, V, a, c->stream),
+            "point cloud apply kernel launch");
+  }
+  DevHeader h{};
+  rc = readHeader(c, &h);
+  if (rc) return rc;
+  if (h.error & kErrSampleBound)
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: sample bound exceeded");
   return VHM_OK;
 }
 

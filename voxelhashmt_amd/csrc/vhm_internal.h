@@ -38,6 +38,7 @@ constexpr int kMaxProbe = 1024;
 constexpr uint32_t kErrPoolFull = 1u;
 constexpr uint32_t kErrHashFull = 2u;
 constexpr uint32_t kErrKeyRange = 4u;
+constexpr uint32_t kErrSampleBound = 8u;  // point cloud: more samples on a ray than the host bound
 
 struct DevHeader {
   uint32_t num_blocks;    // pool slots handed out (can pass max_blocks on overflow)
@@ -117,6 +118,13 @@ void focalDivision(float f, float* inv, int* exact);
 
 struct MeshArgs {
   float voxel, iso, extract_tol;
+};
+
+struct PointCloudArgs {
+  Pose34 T_wc;
+  float voxel, trunc, step, min_view, merge_tol, min_cos, max_weight;
+  uint32_t samples_max;  // bound on the samples of one ray (host-derived from trunc / step)
+  int seq_bits;          // record key: slot | voxel (9) | direction (3) | point * samples_max + sample
 };
 
 // Marching cubes keeps each lane's cluster centres (at most 8 corners x 3
@@ -212,6 +220,19 @@ hipError_t launchStepIncrement(BatchState* st, hipStream_t s);
 // Incremental meshing of every scene's dirty blocks (prefix: S + 1 scratch).
 hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, float* cand,
                            hipStream_t s);
+
+// Point-cloud integration (vhm_pointcloud.hip): samples -> records, a
+// stable radix sort of the records by key, then the ordered updates.
+size_t pointInfoBytes();
+size_t radixSortCountsSize(uint32_t n);
+hipError_t launchPointCloudSample(const float* pts, uint32_t n, const VolumeView& V, const PointCloudArgs& a,
+                                  void* info, uint64_t* rec_key, uint32_t* rec_val, uint32_t* rec_n, hipStream_t s);
+hipError_t launchRadixSort(uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp, uint32_t* vals_tmp,
+                           const uint32_t* n_dev, uint32_t n_host, int bits, uint32_t* counts, uint32_t* offsets,
+                           uint32_t* partials, hipStream_t s);
+hipError_t launchPointCloudApply(const uint64_t* rec_key, const uint32_t* rec_val, const uint32_t* rec_n,
+                                 uint32_t n_host, const void* info, const VolumeView& V, const PointCloudArgs& a,
+                                 hipStream_t s);
 
 struct SynthDeviceScene {
   float room_min[3], room_max[3];
