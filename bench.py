@@ -93,6 +93,75 @@ def calibrate_copy(nbytes: int = 1 << 30, reps: int = 5):
                       "launches": reps}))
 
 
+def bench_point_cloud(args):
+    """integratePointCloud at scale: the valid pixels of the 1280x720
+    plain-at-scale frames, back-projected to camera space on the GPU (about
+    0.9 M points a frame), integrated one cloud per step into one volume
+    (voxel 5 mm, truncation 2 cm: 19 samples per ray).  The clouds are
+    resident in HBM before the timed region."""
+    import numpy as np
+    import torch
+    import voxelhashmt_amd as vhm
+    from voxelhashmt_amd import synth
+
+    W = min(args.warmup, 10)
+    K = min(args.steps, 40)
+    w = synth.plain_scale(n_frames=W + K)
+    ok, msg = vhm.device_available()
+    if not ok:
+        raise SystemExit(f"no GPU: {msg}")
+    dev = torch.device("cuda:0")
+    frames = torch.empty((W + K, w.K.height, w.K.width), dtype=torch.int16, device=dev)
+    synth.render(w.scene, w.K, w.poses, w.noise, 0, frames)
+    v, u = torch.meshgrid(torch.arange(w.K.height, device=dev, dtype=torch.float32),
+                          torch.arange(w.K.width, device=dev, dtype=torch.float32), indexing="ij")
+    clouds = []
+    for i in range(W + K):
+        d = frames[i].to(torch.int32).to(torch.float32) * 0.001
+        m = d > 0
+        z = d[m]
+        clouds.append(torch.stack([(u[m] - w.K.cx) * z / w.K.fx, (v[m] - w.K.cy) * z / w.K.fy, z], 1).contiguous())
+    torch.cuda.synchronize()
+    vp = vhm.default_params(w.voxel_size, w.truncation, hash_log2_buckets=w.hash_log2_buckets,
+                            max_blocks=args.max_blocks or w.max_blocks, max_width=w.K.width, max_height=w.K.height)
+    vol = vhm.Volume(vp)
+    for i in range(W):
+        vol.integrate_point_cloud(clouds[i], w.poses[i])
+    s0 = vol.stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(W, W + K):
+        vol.integrate_point_cloud(clouds[i], w.poses[i])
+    vol.synchronize()
+    elapsed = time.perf_counter() - t0
+    points = sum(int(c.shape[0]) for c in clouds[W:])
+    updates = vol.stats()["updates"] - s0["updates"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as O
+        sample = clouds[W][:20000].cpu().numpy()
+        ov = O.OracleVolume(O.oracle_params(vp))
+        t = time.perf_counter()
+        ov.integrate_point_cloud(sample, vhm.pose_array(w.poses[W]))
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": len(sample) / cpu_s, "unit": "points/s", "cores": 1, "kind": "port",
+               "sample": f"first {len(sample)} points of the first timed cloud, empty volume"}
+    out = {
+        "metric": "point-cloud points integrated/s", "value": points / elapsed, "unit": "points/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": elapsed * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (plain-at-scale frames back-projected on the GPU, resident in HBM)",
+        "config": {"workload": "point_cloud_1280x720", "points_per_step": points / K, "voxel_size_m": w.voxel_size,
+                   "truncation_m": w.truncation, "samples_per_ray": int(2 * w.truncation / max(w.voxel_size / 2, 0.0025)) + 1,
+                   "blocks_allocated": vol.block_count()},
+        "voxel_updates_per_s": updates / elapsed,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    vol.close()
+
+
 def bench_batched(args):
     """Batched plain method: 48 seeded rooms (target block counts log-uniform
     in 1k-60k, resolutions from {320x240, 640x480, 1280x720}), 60 frames
@@ -199,7 +268,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=480)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="plain_scale", choices=["plain_scale", "plain_small", "batched"])
+    ap.add_argument("--workload", default="plain_scale",
+                    choices=["plain_scale", "plain_small", "batched", "point_cloud"])
     ap.add_argument("--max-blocks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
@@ -217,6 +287,9 @@ def main():
         return
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "point_cloud":
+        bench_point_cloud(args)
         return
 
     import numpy as np
