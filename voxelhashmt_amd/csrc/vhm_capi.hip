@@ -74,7 +74,6 @@ struct vhm_ctx {
   // integrator's deferred border blocks (max_blocks each; flags kept zero between frames)
   uint32_t* defer_flags = nullptr;
   uint32_t* defer_list = nullptr;
-  uint8_t* smooth = nullptr;  // per-pixel smooth-patch map of the frame (max_width x max_height)
   // point clouds (grown on demand)
   float* pc_points = nullptr;
   size_t pc_points_cap = 0;
@@ -272,15 +271,9 @@ int integrateStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, 
   ia.inv_a = invTwoSigmaSq(c->P.normal_sigma_angular);
   focalDivision(ia.K.fx, &ia.inv_fx, &ia.exact_fx);
   focalDivision(ia.K.fy, &ia.inv_fy, &ia.exact_fy);
-  // the smooth-patch map relies on the threshold growing with depth
-  const bool map = ia.disc_rel >= 0.0f && std::isfinite(ia.disc_base) && std::isfinite(ia.disc_rel);
-  ia.smooth = map ? c->smooth : nullptr;
   hipEvent_t ev{};
   beginTimed(c, 2, &ev);
-  hipError_t e = map ? launchSmoothMap(dev_depth, ia.K.width, ia.K.height, ia.disc_base, ia.disc_rel, c->smooth,
-                                       c->stream)
-                     : hipSuccess;
-  if (e == hipSuccess) e = launchIntegrate(dev_depth, view(c), ia, c->num_cus * 8, c->stream);
+  const hipError_t e = launchIntegrate(dev_depth, view(c), ia, c->num_cus * 8, c->stream);
   endTimed(c, ev);
   VHM_TRY(c, e, "integrate kernel launch");
   ++c->frames_integrated;
@@ -461,7 +454,6 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->remesh_dirs, size_t(p->max_blocks));
   if (e == hipSuccess) e = hipMalloc(&c->defer_flags, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->defer_list, size_t(p->max_blocks) * 4u);
-  if (e == hipSuccess) e = hipMalloc(&c->smooth, size_t(p->max_width) * size_t(p->max_height));
   if (e == hipSuccess) e = hipMalloc(&c->remesh_partials, scanPartialsSize(p->max_blocks) * 4u);
   if (e != hipSuccess) {
     vhm_destroy(c);
@@ -486,7 +478,7 @@ int vhm_destroy(vhm_ctx* c) {
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
                   c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand,
-                  c->defer_flags, c->defer_list, c->smooth, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
+                  c->defer_flags, c->defer_list, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
                   c->pc_vals, c->pc_vals_tmp, c->pc_count, c->pc_hist, c->pc_offsets, c->pc_partials};
   for (void* b : bufs)
     if (b) hipFree(b);

@@ -51,21 +51,14 @@ __device__ __forceinline__ NormalParams normalParams(const IntegrateArgs& a) {
 // to a queue in LDS, and runs the expensive part 256 queued voxels at a
 // time, one per lane.  Voxels are independent within a frame (every block
 // is in the active list once), so the order of the updates does not matter.
-//
-// The queue has two ends.  Voxels whose 6x6 depth patch is smooth (a
-// per-pixel map computed once per frame, smoothPatch) grow from the front
-// and run a normal whose taps all interpolate plainly; the others grow from
-// the back and run the taps with the discontinuity fallback.  That fallback
-// executes for a whole wavefront when one lane needs it; sorting the lanes
-// this way confines it to wavefronts of edge voxels.
-constexpr int kQueueCap = 1024;  // < 256 left over on each end + 512 appended per block
+constexpr int kQueueCap = 768;  // < 256 left over + 512 appended per block
 
 struct IntegrateQueue {
   uint32_t p[kQueueCap];
   uint32_t vi[kQueueCap];
   float u[kQueueCap], v[kQueueCap], dep[kQueueCap];
   float pcx[kQueueCap], pcy[kQueueCap], pcz[kQueueCap];
-  uint32_t ns, ne;  // smooth entries [0, ns), edge entries [kQueueCap - ne, kQueueCap)
+  uint32_t n;
 };
 
 struct IntegrateTally {
@@ -84,7 +77,7 @@ enum class Pass { kFast, kDeferred };
 constexpr int kDeferredGrid = 512;  // the deferred list is a few percent of the active blocks
 
 // Expensive part for queue entry i (one lane).
-template <Pass P, bool kSmooth>
+template <Pass P>
 __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, const DepthView<float>& D,
                                                 const NormalParams& np, const VolumeView& V, const IntegrateArgs& a,
                                                 IntegrateTally& t) {
@@ -95,8 +88,8 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
   const uint32_t dmask = V.dirmask[p];
   const uint32_t dirty_word = V.dirty[p];
   const F3 pc{Q.pcx[i], Q.pcy[i], Q.pcz[i]};
-  const VoxelObservation o = observeSurface<DepthView<float>, P == Pass::kFast, kSmooth>(
-      D, np, a.T_wc, pc, Q.u[i], Q.v[i], Q.dep[i], a.trunc, a.min_view);
+  const VoxelObservation o = observeSurface<DepthView<float>, P == Pass::kFast>(D, np, a.T_wc, pc, Q.u[i], Q.v[i],
+                                                                                Q.dep[i], a.trunc, a.min_view);
   if (o.dir < 0) return;
   ++t.updates;
   const int d = o.dir;
@@ -130,28 +123,22 @@ __device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, 
   if (!(dmask & (1u << d))) atomicOr(&V.dirmask[p], 1u << d);
 }
 
-// Runs the expensive part on the last min(n, 256) entries appended to one
-// end of the queue and removes them.  Workgroup-wide.
-template <Pass P, bool kSmooth>
+// Runs the expensive part on queue entries [Q.n - m, Q.n) with m = min(Q.n, 256)
+// and removes them.  Workgroup-wide.
+template <Pass P>
 __device__ __forceinline__ void drainQueue(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
                                            const VolumeView& V, const IntegrateArgs& a, IntegrateTally& t) {
-  const uint32_t n = kSmooth ? Q.ns : Q.ne;
+  const uint32_t n = Q.n;
   const uint32_t m = n < 256u ? n : 256u;
-  const uint32_t first = kSmooth ? n - m : kQueueCap - n;
-  if (threadIdx.x < m) integrateQueued<P, kSmooth>(Q, static_cast<int>(first + threadIdx.x), D, np, V, a, t);
+  if (threadIdx.x < m) integrateQueued<P>(Q, static_cast<int>(n - m + threadIdx.x), D, np, V, a, t);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (kSmooth)
-      Q.ns = n - m;
-    else
-      Q.ne = n - m;
-  }
+  if (threadIdx.x == 0) Q.n = n - m;
   __syncthreads();
 }
 
 // Cheap part for all 512 voxels of pool block p: append the voxels that pass
-// observeDepth (and belong to pass P) to the queue.  Requires Q.ns < 256 and
-// Q.ne < 256 on entry.  Workgroup-wide.
+// observeDepth (and belong to pass P) to the queue.  Requires Q.n < 256 on
+// entry.  Workgroup-wide.
 template <Pass P>
 __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
                                              const VolumeView& V, const IntegrateArgs& a, uint32_t p) {
@@ -177,22 +164,12 @@ __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<
     } else {
       pass = border;
     }
-    // the deferred pass (general normal path) queues everything at the back
-    bool smooth = false;
-    if constexpr (P == Pass::kFast) {
-      if (pass && a.smooth)
-        smooth = a.smooth[static_cast<size_t>(static_cast<int>(floorf(v))) * a.K.width + static_cast<int>(floorf(u))] != 0;
-    }
-    const unsigned long long mask_s = __ballot(pass && smooth), mask_e = __ballot(pass && !smooth);
-    uint32_t base_s = 0, base_e = 0;
-    if (lane == 0 && mask_s != 0ull) base_s = atomicAdd(&Q.ns, static_cast<uint32_t>(__popcll(mask_s)));
-    if (lane == 0 && mask_e != 0ull) base_e = atomicAdd(&Q.ne, static_cast<uint32_t>(__popcll(mask_e)));
-    base_s = __shfl(base_s, 0);
-    base_e = __shfl(base_e, 0);
+    const unsigned long long mask = __ballot(pass);
+    uint32_t base = 0;
+    if (lane == 0 && mask != 0ull) base = atomicAdd(&Q.n, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, 0);
     if (pass) {
-      const unsigned long long below = (1ull << lane) - 1ull;
-      const uint32_t i = smooth ? base_s + static_cast<uint32_t>(__popcll(mask_s & below))
-                                : kQueueCap - 1u - (base_e + static_cast<uint32_t>(__popcll(mask_e & below)));
+      const uint32_t i = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
       Q.p[i] = p;
       Q.vi[i] = static_cast<uint32_t>(vi);
       Q.u[i] = u;
@@ -208,7 +185,7 @@ __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<
 
 // Integrates active-list entries [begin, end) of one volume (then drains
 // the queue completely); for kDeferred, deferred-list entries.
-// Workgroup-wide; the queue is empty on entry and exit.
+// Workgroup-wide; Q.n == 0 on entry and exit.
 template <Pass P>
 __device__ __forceinline__ void integrateRange(IntegrateQueue& Q, const DepthView<float>& D, const NormalParams& np,
                                const VolumeView& V, const IntegrateArgs& a, uint32_t begin, uint32_t end,
@@ -226,15 +203,9 @@ __device__ __forceinline__ void integrateRange(IntegrateQueue& Q, const DepthVie
     if constexpr (P == Pass::kDeferred) {
       if (threadIdx.x == 0) V.defer_flags[p] = 0u;
     }
-    if constexpr (P == Pass::kFast) {
-      while (Q.ns >= 256u) drainQueue<P, true>(Q, D, np, V, a, t);
-    }
-    while (Q.ne >= 256u) drainQueue<P, false>(Q, D, np, V, a, t);
+    while (Q.n >= 256u) drainQueue<P>(Q, D, np, V, a, t);
   }
-  if constexpr (P == Pass::kFast) {
-    while (Q.ns > 0u) drainQueue<P, true>(Q, D, np, V, a, t);
-  }
-  while (Q.ne > 0u) drainQueue<P, false>(Q, D, np, V, a, t);
+  while (Q.n > 0u) drainQueue<P>(Q, D, np, V, a, t);
 }
 
 __device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& t) {
@@ -258,7 +229,7 @@ __device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void integrateKernel(const float* __restrict__ depth, VolumeView V,
                                                        IntegrateArgs a) {
   __shared__ IntegrateQueue Q;
-  if (threadIdx.x == 0) Q.ns = Q.ne = 0u;
+  if (threadIdx.x == 0) Q.n = 0;
   __syncthreads();
   const DepthView<float> D{depth, a.K.width, a.K.height};
   const NormalParams np = normalParams(a);
@@ -275,7 +246,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 __global__ __launch_bounds__(256) void integrateDeferredKernel(const float* __restrict__ depth, VolumeView V,
                                                                IntegrateArgs a) {
   __shared__ IntegrateQueue Q;
-  if (threadIdx.x == 0) Q.ns = Q.ne = 0u;
+  if (threadIdx.x == 0) Q.n = 0;
   __syncthreads();
   const DepthView<float> D{depth, a.K.width, a.K.height};
   const NormalParams np = normalParams(a);
@@ -295,7 +266,7 @@ __global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __
                                                             const uint32_t* __restrict__ prefix,
                                                             const BatchState* __restrict__ st) {
   __shared__ IntegrateQueue Q;
-  if (threadIdx.x == 0) Q.ns = Q.ne = 0u;
+  if (threadIdx.x == 0) Q.n = 0;
   __syncthreads();
   const uint32_t step = st->step;
   const uint32_t total = prefix[S];
@@ -328,7 +299,7 @@ __global__ __launch_bounds__(256) void integrateDeferredBatchKernel(const BatchS
   const uint32_t end = begin + chunk < n ? begin + chunk : n;
   if (begin >= end) return;  // workgroup-uniform
   __shared__ IntegrateQueue Q;
-  if (threadIdx.x == 0) Q.ns = Q.ne = 0u;
+  if (threadIdx.x == 0) Q.n = 0;
   __syncthreads();
   const uint32_t step = st->step;
   IntegrateArgs a = sc.ia;
@@ -339,30 +310,6 @@ __global__ __launch_bounds__(256) void integrateDeferredBatchKernel(const BatchS
   IntegrateTally t;
   integrateRange<Pass::kDeferred>(Q, D, np, sc.V, a, begin, end, t);
   flushTally(sc.V, t);
-}
-
-// Per-pixel smoothPatch map: a 32x8 pixel tile per workgroup reads its
-// depth with the patch apron (2 left/up, 3 right/down, clamped) into LDS.
-struct TileView {
-  const float* t;
-  int ox, oy, W, H;
-  __device__ float at(int x, int y) const { return t[(y - oy) * 37 + (x - ox)]; }
-};
-
-__global__ __launch_bounds__(256) void smoothMapKernel(const float* __restrict__ depth, int W, int H, float base,
-                                                       float rel, uint8_t* __restrict__ out) {
-  __shared__ float tile[13 * 37];
-  const int tx0 = blockIdx.x * 32, ty0 = blockIdx.y * 8;
-  for (int k = threadIdx.x; k < 13 * 37; k += 256) {
-    const int r = k / 37, c = k % 37;
-    const int x = imin(tx0 - 2 + c < 0 ? 0 : tx0 - 2 + c, W - 1), y = imin(ty0 - 2 + r < 0 ? 0 : ty0 - 2 + r, H - 1);
-    tile[k] = depth[static_cast<size_t>(y) * W + x];
-  }
-  __syncthreads();
-  const int x0 = tx0 + (threadIdx.x & 31), y0 = ty0 + (threadIdx.x >> 5);
-  if (x0 >= W || y0 >= H) return;
-  const TileView tv{tile, tx0 - 2, ty0 - 2, W, H};
-  out[static_cast<size_t>(y0) * W + x0] = smoothPatch(tv, x0, y0, base, rel) ? 1u : 0u;
 }
 
 // Copies blocks into the reference's AoS Voxel layout (voxel_hash_tsdf.h:18-31):
@@ -403,13 +350,6 @@ hipError_t launchIntegrate(const float* depth, const VolumeView& V, const Integr
                            hipStream_t s) {
   integrateKernel<<<grid, 256, 0, s>>>(depth, V, a);
   integrateDeferredKernel<<<kDeferredGrid, 256, 0, s>>>(depth, V, a);
-  return hipGetLastError();
-}
-
-hipError_t launchSmoothMap(const float* depth, int width, int height, float disc_base, float disc_rel, uint8_t* out,
-                           hipStream_t s) {
-  smoothMapKernel<<<dim3((width + 31) / 32, (height + 7) / 8), 256, 0, s>>>(depth, width, height, disc_base, disc_rel,
-                                                                          out);
   return hipGetLastError();
 }
 

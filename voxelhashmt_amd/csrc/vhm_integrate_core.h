@@ -175,31 +175,6 @@ VHM_HD bool fastNormalApplies(float u, float v) {
   return u >= 2.0f && v >= 2.0f && (u + 2.0f) - u == 2.0f && (v + 2.0f) - v == 2.0f;
 }
 
-// True when every tap of surfaceNormalFast at a voxel projecting into pixel
-// cell (x0, y0) interpolates plainly: all depths of the 6x6 patch
-// x0-2..x0+3, y0-2..y0+3 (clamped to the image, as the taps clamp) are > 0
-// and their range is at most discontinuityThreshold(patch minimum).  Each
-// tap's four depths lie in the patch, so its range is no larger, and its
-// threshold (of a reference depth >= the minimum) no smaller: float
-// subtraction and the threshold are monotone, so this holds after rounding
-// too (disc_rel >= 0).
-template <class D>
-VHM_HD bool smoothPatch(const D& dv, int x0, int y0, float disc_base, float disc_rel) {
-  float mn = 3.402823466e+38f, mx = 0.0f;
-  bool pos = true;
-  for (int j = 0; j < 6; ++j) {
-    const int yy = imin(y0 - 2 + j < 0 ? 0 : y0 - 2 + j, dv.H - 1);
-    for (int i = 0; i < 6; ++i) {
-      const int xx = imin(x0 - 2 + i < 0 ? 0 : x0 - 2 + i, dv.W - 1);
-      const float d = dv.at(xx, yy);
-      pos = pos && d > 0.0f;
-      mn = fminf(mn, d);
-      mx = fmaxf(mx, d);
-    }
-  }
-  return pos && (mx - mn <= discontinuityThreshold(mn, disc_base, disc_rel));
-}
-
 // 21 offsets (a, b), |a|, |b| <= 2 without the 4 corners, row-major in b.
 constexpr int kTapCount = 21;
 VHM_HD int tapIndex(int a, int b) {
@@ -208,10 +183,8 @@ VHM_HD int tapIndex(int a, int b) {
 }
 
 // estimateSurfaceNormalCamera with the 21 distinct samples evaluated once.
-// Requires fastNormalApplies(u, v).  kSmooth: the caller guarantees that the
-// 6x6 depth patch around (floor(u), floor(v)) is smooth (smoothPatch), so
-// every in-range tap takes sampleDepthBilinear's plain interpolation.
-template <class D, bool kSmooth = false>
+// Requires fastNormalApplies(u, v).
+template <class D>
 VHM_HD bool surfaceNormalFast(const D& dv, const NormalParams& a, float u, float v, float cd, F3& n) {
   const int x0 = static_cast<int>(floorf(u)), y0 = static_cast<int>(floorf(v));
   const float tx = u - static_cast<float>(x0), ty = v - static_cast<float>(y0);
@@ -245,14 +218,8 @@ VHM_HD bool surfaceNormalFast(const D& dv, const NormalParams& a, float u, float
       float out = 0.0f;
       if (valid) {
         const float d00 = P[b + 2][a2 + 2], d10 = P[b + 2][a2 + 3], d01 = P[b + 3][a2 + 2], d11 = P[b + 3][a2 + 3];
-        if constexpr (kSmooth) {  // bilinearFromTaps' first branch, always taken here
-          const float top = d00 + (d10 - d00) * tx;
-          const float bottom = d01 + (d11 - d01) * tx;
-          out = top + (bottom - top) * ty;
-        } else {
-          const float nearest = rx ? (ry ? d11 : d10) : (ry ? d01 : d00);
-          valid = bilinearFromTaps(d00, d10, d01, d11, tx, ty, nearest, true, a.disc_base, a.disc_rel, out);
-        }
+        const float nearest = rx ? (ry ? d11 : d10) : (ry ? d01 : d00);
+        valid = bilinearFromTaps(d00, d10, d01, d11, tx, ty, nearest, true, a.disc_base, a.disc_rel, out);
       }
       S[t] = out;
       ok[t] = valid;
@@ -453,9 +420,8 @@ VHM_HD bool observeDepth(const D& dv, const NormalParams& a, const Pose34& T_cw,
 
 // Second half (voxel_hash_tsdf.h:1023-1046): surface normal, weight,
 // surface point and direction of a voxel that passed observeDepth.
-// kFastNormal: the caller guarantees fastNormalApplies(u, v); kSmooth, in
-// addition, smoothPatch at (floor(u), floor(v)).
-template <class D, bool kFastNormal = false, bool kSmooth = false>
+// kFastNormal: the caller guarantees fastNormalApplies(u, v).
+template <class D, bool kFastNormal = false>
 VHM_HD VoxelObservation observeSurface(const D& dv, const NormalParams& a, const Pose34& T_wc, F3 pc, float u,
                                        float v, float dep, float trunc, float min_view) {
   VoxelObservation o;
@@ -466,7 +432,7 @@ VHM_HD VoxelObservation observeSurface(const D& dv, const NormalParams& a, const
   F3 nc = mul3(ray, -1.0f);
   bool have;
   if constexpr (kFastNormal)
-    have = surfaceNormalFast<D, kSmooth>(dv, a, u, v, dep, nc);
+    have = surfaceNormalFast(dv, a, u, v, dep, nc);
   else
     have = surfaceNormal(dv, a, u, v, dep, nc);
   if (have) vaw = fabsf(dot3(nc, ray));

@@ -110,7 +110,6 @@ struct IntegrateArgs {
   float inv_r, inv_a;
   float inv_fx, inv_fy;  // RN(1 / fx), RN(1 / fy)
   int exact_fx, exact_fy;  // divByConst by fx / fy is exact (host-checked)
-  const uint8_t* smooth;   // per pixel: smoothPatch of the filtered frame (null: treat all as edges)
 };
 
 // RN(1 / f) and whether divByConst by f equals the division for every
@@ -142,9 +141,6 @@ hipError_t launchFilterU16(const uint16_t* in, float* out, const FilterArgs& a, 
 hipError_t launchFrameBegin(DevHeader* hdr, uint32_t frame, hipStream_t s);
 hipError_t launchAllocate(const float* depth, const VolumeView& V, const AllocArgs& a, hipStream_t s);
 hipError_t launchIntegrate(const float* depth, const VolumeView& V, const IntegrateArgs& a, int grid,
-                           hipStream_t s);
-// Per-pixel smoothPatch map of a filtered frame (for IntegrateArgs::smooth).
-hipError_t launchSmoothMap(const float* depth, int width, int height, float disc_base, float disc_rel, uint8_t* out,
                            hipStream_t s);
 hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeView& V, uint32_t* out,
                                  hipStream_t s);
