@@ -28,6 +28,7 @@ NormalParams makeParams(int W, int H, const float* K4, const float* p7) {
   a.inv_fy = 1.0f / a.K.fy;
   a.exact_fx = reciprocalDivisionExact(a.K.fx) ? 1 : 0;
   a.exact_fy = reciprocalDivisionExact(a.K.fy) ? 1 : 0;
+  a.exp_tab = kExp2Tab32;
   return a;
 }
 

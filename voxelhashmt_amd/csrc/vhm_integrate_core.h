@@ -35,6 +35,7 @@ struct NormalParams {
   // (reciprocalDivisionExact); otherwise the plain division is used
   float inv_fx, inv_fy;
   int exact_fx, exact_fy;
+  const uint64_t* exp_tab;  // kExp2Tab32 or a copy (the integrator keeps one in LDS)
 };
 
 // backProjectContinuous (voxel_hash_tsdf.h:586-593) with the divisions by the
@@ -152,7 +153,8 @@ VHM_HD bool surfaceNormalReference(const D& dv, const NormalParams& a, float u, 
       // parameter block (which would force it into scratch memory)
       const int s2 = duu * duu + dvv * dvv;
       const float sw = s2 == 0 ? a.spatial_w[0] : (s2 == 1 ? a.spatial_w[1] : a.spatial_w[2]);
-      const float w = sw * gaussianWeight(delta * delta, a.inv_r) * gaussianWeight(ad * ad, a.inv_a);
+      const float w = sw * gaussianWeightTab(delta * delta, a.inv_r, a.exp_tab) *
+                      gaussianWeightTab(ad * ad, a.inv_a, a.exp_tab);
       acc = add3(acc, mul3(sn, w));
       ws += w;
     }
@@ -269,8 +271,8 @@ VHM_HD bool surfaceNormalFast(const D& dv, const NormalParams& a, float u, float
       const float delta = S[t] - cd;
       const float c = sclamp(dot3(cn, sn), -1.0f, 1.0f);
       const float ad = 1.0f - c;
-      const float w = a.spatial_w[du * du + dvv * dvv] * gaussianWeight(delta * delta, a.inv_r) *
-                      gaussianWeight(ad * ad, a.inv_a);
+      const float w = a.spatial_w[du * du + dvv * dvv] * gaussianWeightTab(delta * delta, a.inv_r, a.exp_tab) *
+                      gaussianWeightTab(ad * ad, a.inv_a, a.exp_tab);
       acc = add3(acc, mul3(sn, w));
       ws += w;
     }

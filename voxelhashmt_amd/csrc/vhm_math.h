@@ -51,8 +51,9 @@ VHM_TABLE static const uint64_t kExp2Tab32[32] = {
 VHM_HD double bitsToDouble(uint64_t u) { return __builtin_bit_cast(double, u); }
 VHM_HD uint64_t doubleToBits(double d) { return __builtin_bit_cast(uint64_t, d); }
 
-// exp(x) for float x, identical to glibc 2.35 expf on an FMA-capable x86-64.
-VHM_HD float expf_glibc(float x) {
+// exp(x) for float x, identical to glibc 2.35 expf on an FMA-capable x86-64;
+// tab is kExp2Tab32 or a copy of it (on the GPU, one in LDS).
+VHM_HD float expf_glibc_tab(float x, const uint64_t* tab) {
   if (!(x == x)) return x + x;
   if (x < -0x1.9fe368p6f) return 0.0f;                 // below log(2^-150), and -inf
   if (x > 0x1.62e42ep6f) return __builtin_inff();      // above log(2^128)
@@ -66,7 +67,7 @@ VHM_HD float expf_glibc(float x) {
   const uint64_t ki = doubleToBits(kd);
   kd -= kShift;
   const double r = __builtin_fma(kInvLn2N, xd, -kd);
-  const uint64_t t = kExp2Tab32[ki & 31u] + (ki << 47);
+  const uint64_t t = tab[ki & 31u] + (ki << 47);
   const double s = bitsToDouble(t);
   const double z = __builtin_fma(kC0, r, kC1);
   const double r2 = r * r;
@@ -75,6 +76,7 @@ VHM_HD float expf_glibc(float x) {
   y = y * s;
   return static_cast<float>(y);
 }
+VHM_HD float expf_glibc(float x) { return expf_glibc_tab(x, kExp2Tab32); }
 
 // ----------------------------------------------------------------- vectors --
 
@@ -199,6 +201,9 @@ inline bool reciprocalDivisionExact(float d) {
 }
 
 VHM_HD float gaussianWeight(float sq, float inv_two_sigma_sq) { return expf_glibc(-sq * inv_two_sigma_sq); }
+VHM_HD float gaussianWeightTab(float sq, float inv_two_sigma_sq, const uint64_t* tab) {
+  return expf_glibc_tab(-sq * inv_two_sigma_sq, tab);
+}
 
 // ------------------------------------------------------- octahedral normals --
 
