@@ -38,6 +38,7 @@ constexpr uint8_t kBitAbove = 32;      // some observed slot's corner value may 
 
 struct MeshShared {
   float4 plane[kSlots][kRegion];  // slot k's unit normal (x, y, z) and plane offset (w): one 16-B read
+  uint8_t num_tris[256];          // c_mc.num_tris, read once per cluster
   uint8_t bits[kRegion];       // bit k: slot k observed; kBit* above
   uint32_t src[8];        // pool index of self + 7 positive neighbours (kInvalid if absent)
   uint32_t any_dir;       // OR over the 8 source blocks' direction masks
@@ -398,12 +399,18 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
       valid |= 1u << c;
     }
     if (completeCube(val, valid)) {
-      const uint32_t t = c_mc.num_tris[cubeCase(val, a.iso)];
+      const uint32_t t = S.num_tris[cubeCase(val, a.iso)];
       if (kEmit && t > 0) emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
       ntri += t;
     }
   }
   return ntri;
+}
+
+// Copies the per-case triangle counts into LDS (workgroup-wide, 256 lanes).
+__device__ __forceinline__ void loadCaseCounts(MeshShared& S) {
+  S.num_tris[threadIdx.x] = c_mc.num_tris[threadIdx.x];
+  __syncthreads();
 }
 
 // Workgroup-wide exclusive scan of one value per lane (lane order).
@@ -468,9 +475,12 @@ __device__ uint32_t countItem(MeshShared& S, float* __restrict__ cand, const Vol
     cube_counts[cube] = static_cast<uint8_t>(c);
     mine += c;
   }
-  blockExclusiveScan(S, mine);
-  const uint32_t total = S.total;
-  __syncthreads();  // S.total / S.scan / S.nlist are reused by the next item
+  // workgroup sum: wavefront reduction, then the 4 partial sums
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
+  if ((threadIdx.x & 63) == 0) S.scan[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  const uint32_t total = S.scan[0] + S.scan[1] + S.scan[2] + S.scan[3];
+  __syncthreads();  // S.scan / S.nlist are reused by the next item
   return total;
 }
 
@@ -521,6 +531,7 @@ __global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, const uint3
                                                        uint8_t* __restrict__ cube_counts,
                                                        const uint8_t* __restrict__ dirs, float* __restrict__ cand) {
   __shared__ MeshShared S;
+  loadCaseCounts(S);
   float* const mine = cand + kMeshCandWords * blockIdx.x;
   const uint32_t items = kDirs * n;
   for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
@@ -541,6 +552,7 @@ __global__ __launch_bounds__(256) void meshEmitKernel(VolumeView V, const uint32
                                                       uint32_t* __restrict__ tri_owner, uint32_t capacity,
                                                       float* __restrict__ cand) {
   __shared__ MeshShared S;
+  loadCaseCounts(S);
   float* const mine = cand + kMeshCandWords * blockIdx.x;
   const uint32_t items = kDirs * n;
   for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
@@ -759,6 +771,7 @@ __global__ __launch_bounds__(256) void meshCountBatchKernel(const BatchScene* __
                                                             const uint32_t* __restrict__ prefix,
                                                             float* __restrict__ cand) {
   __shared__ MeshShared M;
+  loadCaseCounts(M);
   float* const mine = cand + kMeshCandWords * blockIdx.x;
   const uint32_t total = prefix[S];
   for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
@@ -784,6 +797,7 @@ __global__ __launch_bounds__(256) void meshEmitBatchKernel(const BatchScene* __r
                                                            const uint32_t* __restrict__ prefix,
                                                            float* __restrict__ cand) {
   __shared__ MeshShared M;
+  loadCaseCounts(M);
   float* const mine = cand + kMeshCandWords * blockIdx.x;
   const uint32_t total = prefix[S];
   for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
