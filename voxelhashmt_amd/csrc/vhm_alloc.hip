@@ -36,7 +36,11 @@ __device__ __forceinline__ uint32_t ldsSlot(uint64_t key, int bits) {
 __device__ void activateKey(const VolumeView& V, uint64_t key, uint32_t frame) {
   const uint32_t slot = insertBlock(V, key);
   if (slot == kInvalid) return;
-  if (atomicExch(&V.stamp[slot], frame) != frame) {
+  // Neighbouring tiles activate many of the same keys: a plain load that
+  // already sees this frame's stamp skips the contended exchange (the
+  // exchanger that stamped it has appended the key).
+  if (__hip_atomic_load(&V.stamp[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != frame &&
+      atomicExch(&V.stamp[slot], frame) != frame) {
     const uint32_t i = atomicAdd(&V.hdr->active_count, 1u);
     V.active[i] = slot;  // i < 2^k: at most one entry per table slot per frame
   }
