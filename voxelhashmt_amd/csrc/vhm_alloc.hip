@@ -33,14 +33,21 @@ __device__ __forceinline__ uint32_t ldsSlot(uint64_t key, int bits) {
   return static_cast<uint32_t>((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 
+// Inserts key (when new) and elects one finder per key per frame: true, with
+// the key's slot, for the lane that must append the slot to the frame's
+// active list.  Neighbouring tiles activate many of the same keys: a plain
+// load that already sees this frame's stamp skips the contended exchange
+// (stamps only ever move to the current frame, so a stale read merely
+// falls through to the exchange).
+__device__ __forceinline__ bool claimKey(const VolumeView& V, uint64_t key, uint32_t frame, uint32_t& slot) {
+  slot = insertBlock(V, key);
+  if (slot == kInvalid) return false;
+  return V.stamp[slot] != frame && atomicExch(&V.stamp[slot], frame) != frame;
+}
+
 __device__ void activateKey(const VolumeView& V, uint64_t key, uint32_t frame) {
-  const uint32_t slot = insertBlock(V, key);
-  if (slot == kInvalid) return;
-  // Neighbouring tiles activate many of the same keys: a plain load that
-  // already sees this frame's stamp skips the contended exchange (the
-  // exchanger that stamped it has appended the key).
-  if (__hip_atomic_load(&V.stamp[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != frame &&
-      atomicExch(&V.stamp[slot], frame) != frame) {
+  uint32_t slot;
+  if (claimKey(V, key, frame, slot)) {
     const uint32_t i = atomicAdd(&V.hdr->active_count, 1u);
     V.active[i] = slot;  // i < 2^k: at most one entry per table slot per frame
   }
@@ -56,12 +63,13 @@ __device__ void allocTile(const float* __restrict__ depth, const VolumeView& V, 
   __shared__ uint64_t centres[kCentreSlots];
   __shared__ uint64_t centre_list[3 * 256];
   __shared__ uint64_t nbrs[kNbrSlots];
-  __shared__ uint32_t n_centres;
+  __shared__ uint32_t n_centres, n_nbrs;
+  static_assert(kCentreSlots * sizeof(uint64_t) >= kNbrSlots * sizeof(uint16_t), "set list fits the centre set");
 
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   for (int i = tid; i < kCentreSlots; i += 256) centres[i] = kEmptyKey;
   for (int i = tid; i < kNbrSlots; i += 256) nbrs[i] = kEmptyKey;
-  if (tid == 0) n_centres = 0;
+  if (tid == 0) n_centres = n_nbrs = 0;
   __syncthreads();
 
   // 1. centre keys of this lane's pixel (voxel_hash_tsdf.h:645-661).
@@ -123,11 +131,36 @@ __device__ void allocTile(const float* __restrict__ depth, const VolumeView& V, 
   }
   __syncthreads();
 
-  // 3. persistent insert + frame activation of the unique neighbourhood keys.
+  // 3. compact the set (a few hundred keys in 4096 slots, usually) into a
+  // list of slot indices, kept in the centre set's storage (dead by now), so
+  // that the global activations below run in one or two rounds of dependent
+  // memory round trips instead of 16 sparse ones.
+  uint16_t* const list = reinterpret_cast<uint16_t*>(centres);
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int i = tid; i < kNbrSlots; i += 256) {
-    const uint64_t key = nbrs[i];
-    if (key != kEmptyKey) activateKey(V, key, a.frame);
+    const bool has = nbrs[i] != kEmptyKey;
+    const unsigned long long m = __ballot(has);
+    uint32_t first = 0;
+    if (lane == 0 && m != 0ull) first = atomicAdd(&n_nbrs, static_cast<uint32_t>(__popcll(m)));
+    first = __shfl(first, 0);
+    if (has) list[first + static_cast<uint32_t>(__popcll(m & below))] = static_cast<uint16_t>(i);
   }
+  __syncthreads();
+
+  // 4. persistent insert + frame activation of the unique neighbourhood keys;
+  // one append counter update per wavefront.
+  const uint32_t n = n_nbrs;
+  for (uint32_t b = 0; b < n; b += 256) {
+    const uint32_t j = b + static_cast<uint32_t>(tid);
+    uint32_t slot = kInvalid;
+    const bool append = j < n && claimKey(V, nbrs[list[j]], a.frame, slot);
+    const unsigned long long m = __ballot(append);
+    uint32_t first = 0;
+    if (lane == 0 && m != 0ull) first = atomicAdd(&V.hdr->active_count, static_cast<uint32_t>(__popcll(m)));
+    first = __shfl(first, 0);
+    if (append) V.active[first + static_cast<uint32_t>(__popcll(m & below))] = slot;
+  }
+  __syncthreads();  // the sets and the list may be reset by a next tile of the same workgroup
 }
 
 __global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ depth, VolumeView V, AllocArgs a) {

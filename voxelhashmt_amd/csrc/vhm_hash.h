@@ -12,7 +12,10 @@ __device__ inline uint32_t findOrInsert(const VolumeView& V, uint64_t key, bool&
   uint32_t slot = hashSlot(key, V.log2cap);
   const int bound = (V.mask + 1u) < static_cast<uint32_t>(kMaxProbe) ? static_cast<int>(V.mask + 1u) : kMaxProbe;
   for (int probe = 0; probe < bound; ++probe) {
-    const uint64_t cur = __hip_atomic_load(&V.keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // A plain (cacheable) load: a slot's key only ever changes from empty to
+    // its key, so a stale read can only show "empty", which the CAS below
+    // then resolves.
+    const uint64_t cur = V.keys[slot];
     if (cur == key) return slot;
     if (cur == kEmptyKey) {
       const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&V.keys[slot]),
