@@ -33,8 +33,8 @@ static_assert(kMeshCandWords == 8 * kSlots * 256, "one centre per (corner, slot)
 
 // Per-voxel bits of the staged region.
 constexpr uint8_t kBitAllocated = 8;   // the voxel's block is allocated
-constexpr uint8_t kBitBelow = 16;      // some observed slot's corner value may be < iso
-constexpr uint8_t kBitAbove = 32;      // some observed slot's corner value may be >= iso
+constexpr uint8_t kBitBelow = 16;      // some selectable slot's corner value may be < iso
+constexpr uint8_t kBitAbove = 32;      // some selectable slot's corner value may be >= iso
 
 struct MeshShared {
   float4 plane[kSlots][kRegion];  // slot k's unit normal (x, y, z) and plane offset (w): one 16-B read
@@ -108,6 +108,14 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
                                                         poolWord(dir, 0, 0, vi));
       const F3 centre{(static_cast<float>(bx + x) + 0.5f) * a.voxel, (static_cast<float>(by + y) + 0.5f) * a.voxel,
                       (static_cast<float>(bz + z) + 0.5f) * a.voxel};
+      // Only a slot that some cluster within reach of iso (processCube,
+      // step 3) can select decides a triangle: its value at the cell centre
+      // is within extract_tol of the cluster's, which is within reach
+      // (extract_tol + voxel + rounding) of iso, and its value here differs
+      // from that by at most 0.87 voxel plus rounding.  The sign bits count
+      // such slots only.
+      const float near = 2.0f * a.extract_tol + 1.875f * a.voxel +
+                         (fabsf(centre.x) + fabsf(centre.y) + fabsf(centre.z) + 4.0f * a.voxel) * 0x1.0p-18f;
 #pragma unroll
       for (int k = 0; k < kSlots; ++k) {
         const uint4 slot = rec[k];  // weight, surface_coord, plane_offset, packed_normal
@@ -117,8 +125,10 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
           const float po = __uint_as_float(slot.z);
           const float v = dot3(n, centre) - po;
           S.plane[k][r] = make_float4(n.x, n.y, n.z, po);
-          if (!(v >= below)) b |= kBitBelow;
-          if (!(v <= above)) b |= kBitAbove;
+          if (!(fabsf(v - a.iso) > near)) {
+            if (!(v >= below)) b |= kBitBelow;
+            if (!(v <= above)) b |= kBitAbove;
+          }
         }
       }
     }
@@ -344,8 +354,19 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
     if (__any(slot2)) sortPairs<kSortPairs16, kSortPairs>(srt);
   }
 
+  // A selected slot lies within extract_tol of its cluster's centre at the
+  // cell centre, and its value at a corner centre differs from that by at
+  // most |n . (corner - cell centre)| <= 0.87 voxel plus the rounding of two
+  // dot products of magnitude <= R (the cube's coordinate extent); a cluster
+  // whose centre is farther than extract_tol + voxel + 2^-20 R from iso has
+  // every corner value (and every completion mean of them) on one side of
+  // iso and yields no triangle.
+  const float reach = a.extract_tol + a.voxel +
+                      (fabsf(cc.x) + fabsf(cc.y) + fabsf(cc.z) + 2.0f * a.voxel) * 0x1.0p-20f;
+
   // (2) clusters: a candidate joins the open cluster unless it lies more than
-  // extract_tol from its running mean
+  // extract_tol from its running mean.  Only centres within reach of iso are
+  // kept (in ascending order, as the reference visits them).
   int ncl = 0;
   float centre = srt[0];
   int count = 1;
@@ -357,27 +378,18 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
         centre = (centre * static_cast<float>(count) + sk) / static_cast<float>(count + 1);
         ++count;
       } else {
-        cand[256 * ncl++ + tid] = centre;
+        if (!(centre - reach > a.iso || centre + reach < a.iso)) cand[256 * ncl++ + tid] = centre;
         centre = sk;
         count = 1;
       }
     }
   }
-  cand[256 * ncl++ + tid] = centre;
+  if (!(centre - reach > a.iso || centre + reach < a.iso)) cand[256 * ncl++ + tid] = centre;
 
-  // (3) per cluster.  A selected slot lies within extract_tol of the
-  // centre at the cell centre, and its value at a corner centre differs from
-  // that by at most |n . (corner - cell centre)| <= 0.87 voxel plus the
-  // rounding of two dot products of magnitude <= R (the cube's coordinate
-  // extent); a cluster whose centre is farther than extract_tol + voxel +
-  // 2^-20 R from iso has every corner value (and every completion mean of
-  // them) on one side of iso and yields no triangle.
-  const float reach = a.extract_tol + a.voxel +
-                      (fabsf(cc.x) + fabsf(cc.y) + fabsf(cc.z) + 2.0f * a.voxel) * 0x1.0p-20f;
+  // (3) per cluster within reach
   uint32_t ntri = 0;
   for (int j = 0; j < ncl; ++j) {
     const float cen = cand[256 * j + tid];
-    if (cen - reach > a.iso || cen + reach < a.iso) continue;
     float val[8];
     uint32_t valid = 0;
 #pragma unroll
