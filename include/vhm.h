@@ -189,8 +189,12 @@ int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capa
  * counterpart (the reference always meshes the whole volume); each item's
  * triangles equal vhm_extract_mesh's for that block and direction.
  * block_keys_out / block_dirs_out are host memory; tris_out / tri_item_out
- * are in out_memkind memory (VHM_DEVICE: written in place on the GPU,
- * nothing copied to the host).  Synchronous. */
+ * are in out_memkind memory.  VHM_HOST: two passes (count, scan, emit), the
+ * items in item order.  VHM_DEVICE: one pass writes in place on the GPU,
+ * nothing copied to the host; each item's triangles are one contiguous run
+ * in the same order, the runs in the order the items finished (as the
+ * reference's atomic appends), and a cube whose triangles would cross
+ * capacity is not written.  Synchronous. */
 int vhm_extract_mesh_incremental(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out, uint8_t* block_dirs_out,
                                  size_t block_capacity, size_t* n_blocks_out, float* tris_out,
                                  uint32_t* tri_item_out, size_t tri_capacity, int out_memkind, size_t* n_tris_out);

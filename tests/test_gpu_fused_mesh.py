@@ -1,0 +1,118 @@
+"""Incremental marching cubes into device buffers (the fused count + emit
+pass, one atomic output range per item) against the same call with host
+output (the two-pass count, scan, emit path), on two volumes fed the same
+frames.
+
+Pool order (and so item numbering and output order) differs between two
+volumes -- blocks take pool slots in the order the allocator's atomics run --
+so the outputs are compared per (block key, direction) item: the same items,
+and for each the same triangles in the same order (cube order, then cluster
+order).  Within one output each item's triangles must be one contiguous run;
+the host output has its items in item order, the device output in the order
+the items finished.  With a device buffer smaller
+than the output, only whole cubes' triangles are written, nothing past the
+capacity, and every written triangle is its item's.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import voxelhashmt_amd as vhm
+from voxelhashmt_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _volumes(w):
+    vp = vhm.default_params(w.voxel_size, w.truncation, hash_log2_buckets=w.hash_log2_buckets,
+                            max_blocks=w.max_blocks, max_width=w.K.width, max_height=w.K.height)
+    return vhm.Volume(vp), vhm.Volume(vp)
+
+
+def _device_call(vol, dev_tris, dev_items):
+    """vhm_extract_mesh_incremental with host keys / directions and device triangles / items."""
+    cap_b = max(vol.block_count(), 1)
+    keys = np.zeros((cap_b, 3), np.int32)
+    dirs = np.zeros(cap_b, np.uint8)
+    nb, nt = C.c_size_t(), C.c_size_t()
+    rc = vhm.lib().vhm_extract_mesh_incremental(vol._h, 0.0, keys.ctypes.data, dirs.ctypes.data, cap_b, C.byref(nb),
+                                                dev_tris.data_ptr(), dev_items.data_ptr(), dev_tris.numel() // 9,
+                                                vhm.VHM_DEVICE, C.byref(nt))
+    assert rc == 0, vhm.lib().vhm_last_error(vol._h)
+    return keys[: nb.value], dirs[: nb.value], nb.value, nt.value
+
+
+def _per_item(keys, tris, items, in_order=False):
+    """{(key, direction): triangles in output order}; each item must be one contiguous run."""
+    m = len(keys)
+    items = items.astype(np.int64)
+    if in_order and len(items):
+        assert np.all(np.diff(items) >= 0), "items not in item order"
+    run_starts = np.flatnonzero(np.r_[True, items[1:] != items[:-1]]) if len(items) else np.zeros(0, np.int64)
+    run_ends = np.append(run_starts[1:], len(items))
+    out = {}
+    for s, e in zip(run_starts, run_ends):
+        t = int(items[s])
+        d, i = t // m, t % m
+        k = (tuple(keys[i]), d)
+        assert k not in out, f"item {t} split into several runs"
+        out[k] = tris[s:e]
+    return out
+
+
+@pytest.mark.parametrize("workload", ["tiny", "plain_small"])
+def test_fused_device_output_equals_two_pass_host_output(gpu, workload):
+    import torch
+    w = synth.tiny(n_frames=6, seed=11) if workload == "tiny" else synth.plain_small(n_frames=6)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    a, b = _volumes(w)
+    cap = 3_000_000
+    dev_tris = torch.empty(cap * 9, dtype=torch.float32, device="cuda:0")
+    dev_items = torch.empty(cap, dtype=torch.int32, device="cuda:0")
+    for i in range(len(frames)):
+        a.integrate(frames[i], w.K, w.poses[i])
+        b.integrate(frames[i], w.K, w.poses[i])
+        if i % 2 == 1:
+            ka, da, m, n = _device_call(a, dev_tris, dev_items)
+            kb, db, tris, items = b.extract_mesh_incremental()
+            print(f"[fused] {workload} frame {i}: {m} blocks, {n} triangles", flush=True)
+            assert (m, n) == (len(kb), len(tris)) and n > 0
+            got = dev_tris[: 9 * n].cpu().numpy().reshape(-1, 3, 3)
+            got_items = dev_items[:n].cpu().numpy().view(np.uint32)
+            ga, gb = _per_item(ka, got, got_items), _per_item(kb, tris, items, in_order=True)
+            assert ga.keys() == gb.keys()
+            for k in ga:
+                assert np.array_equal(ga[k].view(np.uint32), gb[k].view(np.uint32)), k
+            # the re-meshed (block, direction) masks agree too
+            assert dict(zip(map(tuple, ka), da)) == dict(zip(map(tuple, kb), db))
+    a.close()
+    b.close()
+
+
+def test_fused_device_output_past_capacity(gpu):
+    import torch
+    w = synth.tiny(n_frames=3, seed=4)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    a, b = _volumes(w)
+    for i in range(len(frames)):
+        a.integrate(frames[i], w.K, w.poses[i])
+        b.integrate(frames[i], w.K, w.poses[i])
+    kb, db, tris, items = b.extract_mesh_incremental()
+    full = _per_item(kb, tris, items, in_order=True)
+    cap = len(tris) // 3
+    dev_tris = torch.full((cap * 9 + 9,), -7.0, dtype=torch.float32, device="cuda:0")
+    dev_items = torch.full((cap,), -1, dtype=torch.int32, device="cuda:0")
+    ka, da, m, n = _device_call(a, dev_tris[: cap * 9], dev_items)
+    assert (m, n) == (len(kb), len(tris)) and n > cap
+    got = dev_tris[: 9 * cap].cpu().numpy().reshape(-1, 3, 3)
+    got_items = dev_items.cpu().numpy()
+    written = got_items >= 0
+    # a cube whose triangles do not all fit is not written (at most 24 x 5 triangles)
+    assert written[: cap - 120].all()
+    assert np.all(got[~written] == -7.0) and np.all(dev_tris[cap * 9:].cpu().numpy() == -7.0)
+    part = _per_item(ka, got[written], got_items[written].astype(np.uint32))
+    for k, t in part.items():  # each written item's triangles are a prefix of its full list
+        assert np.array_equal(t.view(np.uint32), full[k][: len(t)].view(np.uint32)), k
+    a.close()
+    b.close()

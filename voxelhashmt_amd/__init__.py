@@ -388,7 +388,8 @@ class Volume:
         masks, triangles (n,3,3), items (n,): d * m + i for direction d of
         listed block i).  With device_tris (a torch CUDA float tensor of
         n_max*9 elements) and device_items (int32, n_max) the triangles stay
-        on the GPU and (m, n) is returned."""
+        on the GPU (each item's triangles one contiguous run, the runs in the
+        order the items finished) and (m, n) is returned."""
         nb, nt = C.c_size_t(), C.c_size_t()
         if device_tris is not None:
             cap = device_tris.numel() // 9

@@ -60,6 +60,8 @@ struct vhm_ctx {
   size_t cube_counts_cap = 0;
   float* mesh_cand = nullptr;  // marching cubes' cluster-centre scratch (kMeshCandBytes)
   size_t mesh_cand_cap = 0;
+  uint32_t* mesh_counters = nullptr;  // fused incremental pass: triangle total
+  size_t mesh_counters_cap = 0;
   float* tris = nullptr;
   size_t tris_cap = 0;
   int32_t* key_buf = nullptr;
@@ -477,7 +479,7 @@ int vhm_destroy(vhm_ctx* c) {
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
-                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand,
+                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->mesh_counters,
                   c->defer_flags, c->defer_list, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
                   c->pc_vals, c->pc_vals_tmp, c->pc_count, c->pc_hist, c->pc_offsets, c->pc_partials};
   for (void* b : bufs)
@@ -841,28 +843,13 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
     return VHM_OK;
   }
   const size_t entries = size_t(m) * kDirs;
-  if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
-  if ((rc = ensure(c, &c->cube_counts, &c->cube_counts_cap, entries * 512, "hipMalloc cube counts"))) return rc;
   if ((rc = ensure(c, &c->mesh_cand, &c->mesh_cand_cap, kMeshCandBytes / sizeof(float), "hipMalloc mesh scratch")))
-    return rc;
-  if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
-  if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
-                   "hipMalloc mesh partials")))
     return rc;
   MeshArgs ma{};
   ma.voxel = c->P.voxel_size;
   ma.iso = iso;
   ma.extract_tol = c->P.voxel_size * c->P.layer_extract_scale;
   ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
-  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->stream), "mesh count kernel launch");
-  VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
-          "mesh scan kernel launch");
-  uint32_t total = 0;
-  VHM_TRY(c, hipMemcpyAsync(&total, c->mesh_partials + (scanPartialsSize(static_cast<uint32_t>(entries)) - 1),
-                            sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
-          "hipMemcpy D2H triangle count");
-  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-  *n_tris_out = total;
   if (block_keys_out && block_capacity > 0) {
     if ((rc = ensure(c, &c->key_buf, &c->key_buf_cap, 3 * size_t(m), "hipMalloc key buffer"))) return rc;
     VHM_TRY(c, launchListKeys(view(c), c->remesh_list, m, c->key_buf, c->stream), "remesh key launch");
@@ -875,13 +862,41 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
     VHM_TRY(c, hipMemcpyAsync(block_dirs_out, c->remesh_dirs, kb, hipMemcpyDeviceToHost, c->stream),
             "hipMemcpy D2H remesh directions");
   }
-  if (total > 0 && tris_out && tri_capacity > 0 && out_memkind == VHM_DEVICE) {
-    // straight into the caller's device buffers; triangles past the capacity are dropped
-    const uint32_t cap = static_cast<uint32_t>(tri_capacity < total ? tri_capacity : total);
-    VHM_TRY(c, launchMeshEmit(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, tris_out, tri_item_out, cap,
-                              c->mesh_cand, c->stream),
-            "mesh emit kernel launch");
-  } else if (total > 0 && out_memkind != VHM_DEVICE) {
+  if (tris_out && tri_capacity > 0 && out_memkind == VHM_DEVICE) {
+    // straight into the caller's device buffers in one fused pass (items in
+    // completion order; triangles past the capacity are dropped)
+    if ((rc = ensure(c, &c->mesh_counters, &c->mesh_counters_cap, 1, "hipMalloc mesh counters"))) return rc;
+    VHM_TRY(c, hipMemsetAsync(c->mesh_counters, 0, sizeof(uint32_t), c->stream), "hipMemset mesh counters");
+    const uint32_t cap = static_cast<uint32_t>(tri_capacity < 0xFFFFFFFFu ? tri_capacity : 0xFFFFFFFFu);
+    VHM_TRY(c, launchMeshFused(view(c), c->remesh_list, m, ma, c->remesh_dirs, c->mesh_counters, tris_out,
+                               tri_item_out, cap, c->mesh_cand, c->stream),
+            "fused mesh kernel launch");
+    uint32_t total = 0;
+    VHM_TRY(c, hipMemcpyAsync(&total, c->mesh_counters, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H triangle count");
+    endTimed(c, ev);
+    VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (c->timing) drainEvents(c);
+    *n_tris_out = total;
+    return VHM_OK;
+  }
+  // two passes: count, scan, emit (item order)
+  if ((rc = ensure(c, &c->mesh_counts, &c->mesh_counts_cap, entries, "hipMalloc mesh counts"))) return rc;
+  if ((rc = ensure(c, &c->cube_counts, &c->cube_counts_cap, entries * 512, "hipMalloc cube counts"))) return rc;
+  if ((rc = ensure(c, &c->mesh_offsets, &c->mesh_offsets_cap, entries, "hipMalloc mesh offsets"))) return rc;
+  if ((rc = ensure(c, &c->mesh_partials, &c->mesh_partials_cap, scanPartialsSize(static_cast<uint32_t>(entries)),
+                   "hipMalloc mesh partials")))
+    return rc;
+  VHM_TRY(c, launchMeshCount(view(c), c->remesh_list, m, ma, c->mesh_counts, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->stream), "mesh count kernel launch");
+  VHM_TRY(c, launchScan(c->mesh_counts, c->mesh_offsets, static_cast<uint32_t>(entries), c->mesh_partials, c->stream),
+          "mesh scan kernel launch");
+  uint32_t total = 0;
+  VHM_TRY(c, hipMemcpyAsync(&total, c->mesh_partials + (scanPartialsSize(static_cast<uint32_t>(entries)) - 1),
+                            sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H triangle count");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *n_tris_out = total;
+  if (total > 0 && out_memkind != VHM_DEVICE) {
     // host output: emitted into the context's buffers (kept for
     // vhm_incremental_mesh_copy), then min(capacity, total) copied
     if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;

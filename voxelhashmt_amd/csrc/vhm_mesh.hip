@@ -48,6 +48,8 @@ struct MeshShared {
   uint16_t list_cube[kBlockVoxels];
   uint32_t list_start[kBlockVoxels];
   uint32_t nlist;
+  uint8_t cnt[kBlockVoxels];  // fused pass: triangles per cube
+  uint32_t base;              // fused pass: the item's output offset
 };
 
 __device__ uint32_t lookupPool(const VolumeView& V, int x, int y, int z) {
@@ -504,7 +506,7 @@ __device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeVi
                          const MeshArgs& a, uint32_t base,
                          uint32_t total, const uint8_t* __restrict__ cube_counts, float* tris, uint32_t* tri_owner,
                          uint32_t owner, uint32_t capacity) {
-  if (total == 0) return;  // workgroup-uniform
+  if (total == 0 || base >= capacity) return;  // workgroup-uniform: nothing of this item is written
   const uint64_t key = V.pool_keys[b];
   if (!stageRegion(S, V, key, b, dir, a)) {
     __syncthreads();
@@ -531,6 +533,95 @@ __device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeVi
     }
   }
   __syncthreads();
+}
+
+// ---- fused count + emit (one staging per item) ----
+//
+// An item stages its region once, counts every candidate cube, takes its
+// output range with one atomic add on the running total, and emits from the
+// same staged region.  Items land in the order they finish (as in the
+// reference's GPU extractor, whose triangles are appended atomically); within
+// an item the triangles keep the two-pass order (cube, then cluster), and
+// every triangle carries its item.
+
+// One item: stage once, count every candidate cube, take the offset, emit.
+__device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const VolumeView& V, uint32_t b, int dir,
+                              const MeshArgs& a, uint32_t item, uint32_t* total_out, float* tris, uint32_t* tri_owner,
+                              uint32_t capacity) {
+  const int tid = threadIdx.x;
+  const uint64_t key = V.pool_keys[b];
+  uint32_t total = 0;
+  const bool staged = stageRegion(S, V, key, b, dir, a);  // workgroup-uniform
+  if (staged) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int cube = tid + 256 * half;
+      S.cnt[cube] = 0;
+      listAppend(S, cubeMayCross(S, cube), cube, 0u);
+    }
+    __syncthreads();
+    const uint32_t n = S.nlist;
+    uint32_t mine = 0;
+    for (uint32_t j = tid; j < n; j += 256) {
+      const int cube = S.list_cube[j];
+      const uint32_t c = processCube<false>(S, key, cube, a, cand, nullptr);
+      S.cnt[cube] = static_cast<uint8_t>(c);
+      mine += c;
+    }
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
+    if ((tid & 63) == 0) S.scan[tid >> 6] = mine;
+    __syncthreads();
+    total = S.scan[0] + S.scan[1] + S.scan[2] + S.scan[3];
+  }
+  if (tid == 0) S.base = total ? atomicAdd(total_out, total) : 0u;
+  __syncthreads();
+  uint32_t base = S.base;
+  if (total > 0 && base < capacity) {  // workgroup-uniform
+    const uint32_t c0 = S.cnt[tid], c1 = S.cnt[tid + 256];
+    if (tid == 0) S.nlist = 0u;
+    __syncthreads();
+    // offsets in the reference's cube order (cubes 0..255, then 256..511)
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const uint32_t c = half ? c1 : c0;
+      const uint32_t start = base + blockExclusiveScan(S, c);
+      listAppend(S, c > 0, tid + 256 * half, start);
+      base += S.total;
+    }
+    __syncthreads();
+    const uint32_t n = S.nlist;
+    for (uint32_t j = tid; j < n; j += 256) {
+      const int cube = S.list_cube[j];
+      const uint32_t start = S.list_start[j];
+      const uint32_t c = S.cnt[cube];
+      if (start + c <= capacity) {
+        processCube<true>(S, key, cube, a, cand, tris + 9ull * start);
+        if (tri_owner)
+          for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = item;
+      }
+    }
+  }
+  __syncthreads();  // the item's LDS is reused by the next item
+  return total;
+}
+
+// Fused incremental pass, persistent over the 6 n items (item dir * n + i:
+// listed block list[i], direction dir, skipped unless dirs[i] has bit dir).
+// total_out: zeroed; receives the number of triangles.
+__global__ __launch_bounds__(256) void meshFusedKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
+                                                       MeshArgs a, const uint8_t* __restrict__ dirs,
+                                                       uint32_t* __restrict__ total_out, float* __restrict__ tris,
+                                                       uint32_t* __restrict__ tri_owner, uint32_t capacity,
+                                                       float* __restrict__ cand) {
+  __shared__ MeshShared S;
+  loadCaseCounts(S);
+  float* const mine = cand + kMeshCandWords * blockIdx.x;
+  const uint32_t items = kDirs * n;
+  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    const uint32_t dir = item / n, i = item % n;
+    if (dirs[i] & (1u << dir))  // workgroup-uniform
+      fusedItem(S, mine, V, list[i], static_cast<int>(dir), a, item, total_out, tris, tri_owner, capacity);
+  }
 }
 
 // Persistent over the 6 n items: item dir * n + i meshes block list[i] (or
@@ -825,6 +916,14 @@ __global__ __launch_bounds__(256) void meshEmitBatchKernel(const BatchScene* __r
 }  // namespace
 
 static uint32_t meshGrid(uint32_t items) { return items < kMeshGridMax ? items : kMeshGridMax; }
+
+hipError_t launchMeshFused(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
+                           const uint8_t* dirs, uint32_t* total, float* tris, uint32_t* tri_owner, uint32_t capacity,
+                           float* cand, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  meshFusedKernel<<<meshGrid(kDirs * n), 256, 0, s>>>(V, list, n, a, dirs, total, tris, tri_owner, capacity, cand);
+  return hipGetLastError();
+}
 
 hipError_t launchMeshCount(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                            uint32_t* counts, uint8_t* cube_counts, const uint8_t* dirs, float* cand, hipStream_t s) {
