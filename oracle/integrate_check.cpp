@@ -3,6 +3,7 @@
 // Lets the CPU test suite check, on real frames and without a GPU, that the
 // deduplicated fast normal path equals the reference-order path, and that
 // the whole per-voxel update equals the oracle.
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <unordered_map>
@@ -111,6 +112,27 @@ uint64_t vcheck_integrate_blocks(void* blocks_aos, const int32_t* keys, std::siz
       }
     }
   return updates;
+}
+
+// The slot search's normal test: for each (packed, n, min_cos), the filtered
+// decision octNormalCosBelowWith (with an rsqrt moved `rsqrt_ulps` ulp off
+// the host's 1/sqrtf, as the GPU's may be) and the reference computation's
+// decision; cos_out = the reference's rounded dot product.
+void vic_oct_cos_below(const uint32_t* packed, const float* n3, const float* min_cos, size_t count, int rsqrt_ulps,
+                       uint8_t* fast_out, uint8_t* exact_out, float* cos_out) {
+  const auto rsqrt = [rsqrt_ulps](float x) {
+    float r = 1.0f / sqrtf(x);
+    for (int i = 0; i < rsqrt_ulps; ++i) r = std::nextafter(r, 2.0f * r);
+    for (int i = 0; i < -rsqrt_ulps; ++i) r = std::nextafter(r, 0.0f);
+    return r;
+  };
+  for (size_t i = 0; i < count; ++i) {
+    const F3 n{n3[3 * i], n3[3 * i + 1], n3[3 * i + 2]};
+    const float c = dot3(unpackNormalOct(packed[i]), n);
+    cos_out[i] = c;
+    exact_out[i] = c < min_cos[i];
+    fast_out[i] = octNormalCosBelowWith(packed[i], n, min_cos[i], rsqrt);
+  }
 }
 
 }  // extern "C"

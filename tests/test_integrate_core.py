@@ -116,3 +116,63 @@ def test_product_voxel_update_equals_oracle(chk, size):
                                         O.k4(w.K).ctypes.data, pose.ctypes.data)
         assert n_upd == ov.updates() - before
     assert np.array_equal(blocks.view(np.uint32), ov.dump_blocks(keys).view(np.uint32))
+
+
+def test_slot_normal_filter_equals_reference_decision(chk):
+    """The slot search's filtered cosine test (estimate from the unnormalised
+    octahedral direction and an approximate rsqrt; the reference computation
+    inside a 2^-17 band) decides dot3(unpackNormalOct(p), n) < min_cos as the
+    reference does: at min_cos equal to the reference's rounded cosine and
+    every float within 64 ulp of it, at random thresholds, with the rsqrt
+    moved up to 2 ulp either way."""
+    f = chk.vic_oct_cos_below
+    f.argtypes = [C.c_void_p] * 3 + [C.c_size_t, C.c_int] + [C.c_void_p] * 3
+    rng = np.random.default_rng(7)
+    m = 20000
+    packed = rng.integers(0, 2**32, m, dtype=np.uint64).astype(np.uint32)
+    packed[:64] = [0, 0xFFFFFFFF, 0x7FFF7FFF, 0x80008000, 0xFFFF0000, 0x0000FFFF] * 10 + [0x8000FFFF] * 4
+    n = rng.normal(size=(m, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    n = n.astype(np.float32)
+    # a quarter of the normals close to the stored one (cosines near 1)
+    q = m // 4
+    n[:q] = np.float32(0.0)
+
+    def call(pk, nn, mc, ulps):
+        k = len(pk)
+        fast, exact = np.zeros(k, np.uint8), np.zeros(k, np.uint8)
+        cos = np.zeros(k, np.float32)
+        f(np.ascontiguousarray(pk).ctypes.data, np.ascontiguousarray(nn).ctypes.data,
+          np.ascontiguousarray(mc, dtype=np.float32).ctypes.data, k, ulps, fast.ctypes.data, exact.ctypes.data,
+          cos.ctypes.data)
+        return fast, exact, cos
+
+    # stored directions for the first quarter: the reference's own unpacked normals, nudged
+    base = np.zeros((q, 3), np.float32)
+    for j in range(3):  # unit axis probes give the unpacked normal's components
+        e = np.zeros((q, 3), np.float32)
+        e[:, j] = 1.0
+        base[:, j] = call(packed[:q], e, np.zeros(q, np.float32), 0)[2]
+    n[:q] = (base + rng.normal(scale=1e-3, size=(q, 3))).astype(np.float32)
+    n[:q] /= np.linalg.norm(n[:q].astype(np.float64), axis=1, keepdims=True).astype(np.float32)
+
+    cos = call(packed, n, np.zeros(m, np.float32), 0)[2]
+    ks = np.arange(-64, 65)
+    pk = np.repeat(packed, len(ks))
+    nn = np.repeat(n, len(ks), axis=0)
+    c = np.repeat(cos, len(ks))
+    steps = np.tile(ks, m)
+    mc = c.copy()
+    up = steps > 0
+    for s in range(1, 65):  # move each threshold |k| floats away from the reference cosine
+        sel = np.abs(steps) >= s
+        mc[sel] = np.where(up[sel], np.nextafter(mc[sel], np.float32(2)), np.nextafter(mc[sel], np.float32(-2)))
+    mc = np.concatenate([mc, rng.uniform(-1, 1, m).astype(np.float32), np.float32([0.5, 0.8, 0.9, 0.95, 1.0])[
+        rng.integers(0, 5, m)]])
+    pk = np.concatenate([pk, packed, packed])
+    nn = np.concatenate([nn, n, n])
+    for ulps in (-2, 0, 2):
+        fast, exact, _ = call(pk, nn, mc, ulps)
+        bad = np.flatnonzero(fast != exact)
+        assert len(bad) == 0, (ulps, bad[:5], pk[bad[:5]], mc[bad[:5]])
+    assert len(np.unique(exact)) == 2

@@ -352,7 +352,7 @@ VHM_HD bool integrateSample(Slots& L, float coord, F3 pw, F3 nw, float w, float 
       if (empty < 0) empty = k;
       continue;
     }
-    if (dot3(unpackNormalOct(L.pn[k]), n) < min_cos) continue;
+    if (octNormalCosBelow(L.pn[k], n, min_cos)) continue;
     const float delta = fabsf(L.sc[k] - coord);
     if (delta < best_delta) {
       best_delta = delta;

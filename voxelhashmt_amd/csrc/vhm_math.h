@@ -237,7 +237,9 @@ VHM_HD float dequantOct(uint32_t c) {
   return quot * 2.0f - 1.0f;
 }
 
-VHM_HD F3 unpackNormalOct(uint32_t packed) {
+// unpackNormalOctahedral before its final normalisation: a vector with
+// |x| + |y| + |z| = 1 (up to rounding), so its length is at least 1/sqrt(3).
+VHM_HD F3 octDirection(uint32_t packed) {
   F3 n{dequantOct(packed), dequantOct(packed >> 16u), 0.0f};
   n.z = 1.0f - fabsf(n.x) - fabsf(n.y);
   if (n.z < 0.0f) {
@@ -246,7 +248,40 @@ VHM_HD F3 unpackNormalOct(uint32_t packed) {
     n.x = px;
     n.y = py;
   }
-  return normalized3(n);
+  return n;
+}
+
+VHM_HD F3 unpackNormalOct(uint32_t packed) { return normalized3(octDirection(packed)); }
+
+// 1/sqrt(x) to within 2 ulp: the hardware reciprocal square root on the
+// GPU (1 ulp), two correctly rounded operations on the host.
+VHM_HD float approxRsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
+
+// dot3(unpackNormalOct(packed), n) < min_cos, exactly, for |n| <= 1 + 2^-20.
+// The rounded reference value R (normalise with three correctly rounded
+// divisions, then a rounded dot product) and the estimate A = dot3(m, n) *
+// rsqrt(dot3(m, m)) of the unnormalised direction m both lie within 8 ulp of
+// 1 of the exact m.n / |m| (R: 2.5 ulp in the length, 1 in each quotient, 3
+// in the dot product; A: 3 in each dot product, 2 in the rsqrt, 1 in the
+// product), so |A - R| < 2^-20.  Outside a 2^-17 band around min_cos the
+// estimate decides; inside it the reference computation does.  `rsqrt` is
+// any 1/sqrt within 2 ulp.
+template <class Rsqrt>
+VHM_HD bool octNormalCosBelowWith(uint32_t packed, F3 n, float min_cos, Rsqrt rsqrt) {
+  const F3 m = octDirection(packed);
+  const float est = dot3(m, n) * rsqrt(dot3(m, m));
+  if (fabsf(est - min_cos) > 0x1p-17f) return est < min_cos;
+  return dot3(normalized3(m), n) < min_cos;
+}
+
+VHM_HD bool octNormalCosBelow(uint32_t packed, F3 n, float min_cos) {
+  return octNormalCosBelowWith(packed, n, min_cos, [](float x) { return approxRsqrt(x); });
 }
 
 // ------------------------------------------------------------ block keys --
