@@ -61,15 +61,16 @@ __device__ __forceinline__ NormalParams normalParams(const IntegrateArgs& a, con
 // to a queue in LDS, and runs the expensive part 256 queued voxels at a
 // time, one per lane.  Voxels are independent within a frame (every block
 // is in the active list once), so the order of the updates does not matter.
-constexpr int kQueueCap = 768;  // < 256 left over + 512 appended per block
-
-struct IntegrateQueue {
-  uint32_t p[kQueueCap];
-  uint32_t vi[kQueueCap];
-  float u[kQueueCap], v[kQueueCap], dep[kQueueCap];
-  float pcx[kQueueCap], pcy[kQueueCap], pcz[kQueueCap];
+template <int kCap>
+struct VoxelQueue {
+  uint32_t p[kCap];
+  uint32_t vi[kCap];
+  float u[kCap], v[kCap], dep[kCap];
+  float pcx[kCap], pcy[kCap], pcz[kCap];
   uint32_t n;
 };
+using IntegrateQueue = VoxelQueue<768>;  // workgroup queue: < 256 left over + 512 appended per block
+using WaveQueue = VoxelQueue<128>;       // wavefront queue: < 64 left over + 64 appended per round
 
 struct IntegrateTally {
   unsigned long long updates = 0, new_obs = 0, visited = 0;
@@ -87,8 +88,8 @@ enum class Pass { kFast, kDeferred };
 constexpr int kDeferredGrid = 512;  // the deferred list is a few percent of the active blocks
 
 // Expensive part for queue entry i (one lane).
-template <Pass P>
-__device__ __forceinline__ void integrateQueued(const IntegrateQueue& Q, int i, const DepthView<float>& D,
+template <Pass P, class Queue>
+__device__ __forceinline__ void integrateQueued(const Queue& Q, int i, const DepthView<float>& D,
                                                 const NormalParams& np, const VolumeView& V, const IntegrateArgs& a,
                                                 IntegrateTally& t) {
   const uint32_t p = Q.p[i], vi = Q.vi[i];
@@ -218,6 +219,99 @@ __device__ __forceinline__ void integrateRange(IntegrateQueue& Q, const DepthVie
   while (Q.n > 0u) drainQueue<P>(Q, D, np, V, a, t);
 }
 
+// Orders one wavefront's LDS accesses across its lanes (no workgroup barrier).
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront on its own: takes blocks by ticket (entries of the active
+// list for kFast, of the deferred list for kDeferred) until the list is
+// exhausted, runs the cheap part over a block 64 voxels at a time, queues
+// the passing voxels in its own LDS queue and runs the expensive part
+// whenever 64 are queued (then on the remainder).  No workgroup barrier, so
+// a wavefront never waits for its neighbours' slow lanes, and the tickets
+// balance dense and sparse blocks over the whole grid.
+template <Pass P>
+__device__ __forceinline__ void integrateWave(WaveQueue& Q, const DepthView<float>& D, const NormalParams& np,
+                                              const VolumeView& V, const IntegrateArgs& a, uint32_t n_items,
+                                              uint32_t* ticket, IntegrateTally& t) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t qn = 0;  // wavefront-uniform
+  uint32_t p = 0;
+  int bx = 0, by = 0, bz = 0;
+  int round = 8;  // 64-voxel round of block p; 8: take the next block
+  bool more = true;
+  for (;;) {
+    while (more && qn < 64u) {
+      if (round == 8) {
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(ticket, 1u);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= n_items) {
+          more = false;
+          break;
+        }
+        if constexpr (P == Pass::kDeferred) {
+          p = V.defer_list[item];
+        } else {
+          p = V.vals[V.active[item]];
+          if (p >= V.max_blocks) continue;  // pool exhausted for this key (error already flagged)
+          if (lane == 0) t.visited += kBlockVoxels;
+        }
+        const uint64_t key = V.pool_keys[p];
+        bx = keyX(key) * kBlockSide;
+        by = keyY(key) * kBlockSide;
+        bz = keyZ(key) * kBlockSide;
+        round = 0;
+      }
+      const int vi = round * 64 + lane;
+      const int lx = vi & 7, ly = (vi >> 3) & 7, lz = vi >> 6;
+      // voxel centre, voxelToWorld (voxel_hash_tsdf.h:603-609)
+      const F3 pw{(static_cast<float>(bx + lx) + 0.5f) * a.voxel, (static_cast<float>(by + ly) + 0.5f) * a.voxel,
+                  (static_cast<float>(bz + lz) + 0.5f) * a.voxel};
+      F3 pc;
+      float u, v, dep;
+      bool pass = observeDepth(D, np, a.T_cw, pw, a.trunc, pc, u, v, dep);
+      const bool border = pass && !fastNormalApplies(u, v);
+      if constexpr (P == Pass::kFast) {
+        // list the block once (the flag is cleared by the deferred pass)
+        if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
+          V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
+        pass = pass && !border;
+      } else {
+        pass = border;
+      }
+      const unsigned long long mask = __ballot(pass);
+      if (pass) {
+        const uint32_t i = qn + static_cast<uint32_t>(__popcll(mask & below));
+        Q.p[i] = p;
+        Q.vi[i] = static_cast<uint32_t>(vi);
+        Q.u[i] = u;
+        Q.v[i] = v;
+        Q.dep[i] = dep;
+        Q.pcx[i] = pc.x;
+        Q.pcy[i] = pc.y;
+        Q.pcz[i] = pc.z;
+      }
+      qn += static_cast<uint32_t>(__popcll(mask));
+      if (++round == 8) {
+        if constexpr (P == Pass::kDeferred) {
+          if (lane == 0) V.defer_flags[p] = 0u;
+        }
+      }
+    }
+    if (qn == 0u) break;  // and no block left
+    waveSync();
+    const uint32_t m = qn < 64u ? qn : 64u;
+    if (static_cast<uint32_t>(lane) < m) integrateQueued<P>(Q, static_cast<int>(qn - m) + lane, D, np, V, a, t);
+    qn -= m;
+    waveSync();  // the drained entries are rewritten by the next rounds
+  }
+}
+
 __device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& t) {
   // one atomic per wavefront: reduce the lanes' counts first
   unsigned long long u = t.updates, o = t.new_obs;
@@ -228,8 +322,8 @@ __device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& 
   if ((threadIdx.x & 63) == 0) {
     if (u) atomicAdd(&V.hdr->updates, u);
     if (o) atomicAdd(&V.hdr->observed, o);
+    if (t.visited) atomicAdd(&V.hdr->visited, t.visited);  // counted by lane 0 of a wavefront
   }
-  if (threadIdx.x == 0 && t.visited) atomicAdd(&V.hdr->visited, t.visited);
   t = IntegrateTally{};
 }
 
@@ -238,36 +332,25 @@ __device__ __forceinline__ void flushTally(const VolumeView& V, IntegrateTally& 
 // 4 wavefronts per SIMD: the allocator lands just above 128 VGPRs without the hint.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void integrateKernel(const float* __restrict__ depth, VolumeView V,
                                                        IntegrateArgs a) {
-  __shared__ IntegrateQueue Q;
-  if (threadIdx.x == 0) Q.n = 0;
-  __syncthreads();
+  __shared__ WaveQueue Q[4];
   const DepthView<float> D{depth, a.K.width, a.K.height};
   __shared__ uint64_t exp_tab[32];
   const NormalParams np = normalParams(a, expTableLds(exp_tab));
-  const uint32_t n_active = V.hdr->active_count;
-  const uint32_t chunk = (n_active + gridDim.x - 1) / gridDim.x;
-  const uint32_t begin = blockIdx.x * chunk;
-  const uint32_t end = begin + chunk < n_active ? begin + chunk : n_active;
   IntegrateTally t;
-  if (begin < end) integrateRange<Pass::kFast>(Q, D, np, V, a, begin, end, t);
+  integrateWave<Pass::kFast>(Q[threadIdx.x >> 6], D, np, V, a, V.hdr->active_count, &V.hdr->integrate_ticket, t);
   flushTally(V, t);
 }
 
 // The deferred pass: the border voxels of the blocks the main pass listed.
 __global__ __launch_bounds__(256) void integrateDeferredKernel(const float* __restrict__ depth, VolumeView V,
                                                                IntegrateArgs a) {
-  __shared__ IntegrateQueue Q;
-  if (threadIdx.x == 0) Q.n = 0;
-  __syncthreads();
+  __shared__ WaveQueue Q[4];
   const DepthView<float> D{depth, a.K.width, a.K.height};
   __shared__ uint64_t exp_tab[32];
   const NormalParams np = normalParams(a, expTableLds(exp_tab));
-  const uint32_t n = V.hdr->defer_count;  // reset by the next frame's frameBeginKernel
-  const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const uint32_t begin = blockIdx.x * chunk;
-  const uint32_t end = begin + chunk < n ? begin + chunk : n;
   IntegrateTally t;
-  if (begin < end) integrateRange<Pass::kDeferred>(Q, D, np, V, a, begin, end, t);
+  // defer_count and the ticket are reset by the next frame's frameBeginKernel
+  integrateWave<Pass::kDeferred>(Q[threadIdx.x >> 6], D, np, V, a, V.hdr->defer_count, &V.hdr->defer_ticket, t);
   flushTally(V, t);
 }
 

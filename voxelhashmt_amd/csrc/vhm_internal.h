@@ -50,6 +50,8 @@ struct DevHeader {
   unsigned long long visited;   // voxels visited (512 per integrated block)
   unsigned long long mesh_total;
   uint32_t defer_count;  // blocks appended to the integrator's deferred list this frame
+  uint32_t integrate_ticket;  // integrator: next active-list entry to hand out this frame
+  uint32_t defer_ticket;      // deferred pass: next deferred-list entry to hand out this frame
   uint32_t pad;
 };
 
