@@ -132,10 +132,13 @@ struct PointCloudArgs {
 // Marching cubes keeps each lane's cluster centres (at most 8 corners x 3
 // slots) in a per-workgroup global scratch column rather than in LDS, which
 // lets 4 workgroups share a CU.  The mesh kernels run persistent grids of at
-// most kMeshGridMax workgroups, each owning kMeshCandWords floats.
+// most kMeshGridMax workgroups, each owning kMeshCandWords floats; the fused
+// pass keeps the centres of both of a lane's cubes until it emits them, with
+// one word of cube and cluster count each (kMeshFusedCandWords).
 constexpr uint32_t kMeshGridMax = 2048;
 constexpr size_t kMeshCandWords = 24 * 256;
-constexpr size_t kMeshCandBytes = kMeshGridMax * kMeshCandWords * sizeof(float);
+constexpr size_t kMeshFusedCandWords = 2 * 25 * 256;
+constexpr size_t kMeshCandBytes = kMeshGridMax * kMeshFusedCandWords * sizeof(float);
 
 // Launchers (each defined next to its kernel).  All are asynchronous on `s`.
 hipError_t launchFilterF32(const float* in, float* out, const FilterArgs& a, hipStream_t s);

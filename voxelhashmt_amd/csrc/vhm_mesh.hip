@@ -30,6 +30,11 @@ __device__ __constant__ McTables c_mc = makeMcTables();
 constexpr int kR = 9;  // corner region edge (8 voxels + 1 from the + neighbours)
 constexpr int kRegion = kR * kR * kR;
 static_assert(kMeshCandWords == 8 * kSlots * 256, "one centre per (corner, slot) for each of 256 lanes");
+// fused pass: per lane and cube slot (j >> 8), one word of cube | ncl << 16,
+// then the 24 centre rows
+constexpr size_t kFusedSetWords = (1 + 8 * kSlots) * 256;
+static_assert(kMeshFusedCandWords == 2 * kFusedSetWords, "two cubes per lane");
+static_assert(kMeshFusedCandWords >= kMeshCandWords, "one scratch buffer serves every mesh kernel");
 
 // Per-voxel bits of the staged region.
 constexpr uint8_t kBitAllocated = 8;   // the voxel's block is allocated
@@ -302,19 +307,85 @@ __device__ __forceinline__ void sortPairs(float* v) {
 // clustering (888-907), whose centres go to the lane's LDS column; (3) per
 // cluster in ascending order, the closest slot of each corner, the corner
 // values, completion and the triangles (909-934).
+// (3) of processCube: the triangles of the ncl clusters whose centres are
+// centres[0], centres[256], ... (sc: the cube's candidates, rc: its corner
+// voxels in the staged region, (ox, oy, oz): its first voxel).
+template <bool kEmit>
+__device__ __forceinline__ uint32_t clusterTriangles(const MeshShared& S, const float (&sc)[8][kSlots],
+                                                     const int (&rc)[8], int ox, int oy, int oz, const MeshArgs& a,
+                                                     const float* __restrict__ centres, int ncl, float* out) {
+  const float cx[2] = {(static_cast<float>(ox) + 0.5f) * a.voxel, (static_cast<float>(ox + 1) + 0.5f) * a.voxel};
+  const float cy[2] = {(static_cast<float>(oy) + 0.5f) * a.voxel, (static_cast<float>(oy + 1) + 0.5f) * a.voxel};
+  const float cz[2] = {(static_cast<float>(oz) + 0.5f) * a.voxel, (static_cast<float>(oz + 1) + 0.5f) * a.voxel};
+  uint32_t ntri = 0;
+  for (int j = 0; j < ncl; ++j) {
+    const float cen = centres[256 * j];
+    float val[8];
+    uint32_t valid = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      val[c] = 0.0f;
+      int best = -1;
+      float best_delta = 3.402823466e+38f;
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) {
+        const float delta = fabsf(sc[c][k] - cen);  // +inf for a missing candidate
+        if (delta <= a.extract_tol && delta < best_delta) {
+          best_delta = delta;
+          best = k;
+        }
+      }
+      if (best < 0) continue;
+      // planeSD at the corner voxel's own centre, formed as the staging forms it
+      val[c] = planeSD(S, best, rc[c], F3{cx[kMcCornerOffset[c][0]], cy[kMcCornerOffset[c][1]], cz[kMcCornerOffset[c][2]]});
+      valid |= 1u << c;
+    }
+    if (completeCube(val, valid)) {
+      const uint32_t t = S.num_tris[cubeCase(val, a.iso)];
+      if (kEmit && t > 0) emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
+      ntri += t;
+    }
+  }
+  return ntri;
+}
+
+// Emits the triangles of a cube whose in-reach cluster centres a count pass
+// (processCube) left at centres[0], centres[256], ...: the candidates again,
+// without the sort and the clustering.
+__device__ __forceinline__ void emitFromCentres(const MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
+                                                const float* __restrict__ centres, int ncl, float* out) {
+  const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
+  const int ox = keyX(key) * kBlockSide + lx, oy = keyY(key) * kBlockSide + ly, oz = keyZ(key) * kBlockSide + lz;
+  const float hv = 0.5f * a.voxel;
+  const F3 cc{(static_cast<float>(ox) + 0.5f) * a.voxel + hv, (static_cast<float>(oy) + 0.5f) * a.voxel + hv,
+              (static_cast<float>(oz) + 0.5f) * a.voxel + hv};
+  int rc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    rc[c] = (lx + kMcCornerOffset[c][0]) + kR * ((ly + kMcCornerOffset[c][1]) + kR * (lz + kMcCornerOffset[c][2]));
+  float sc[8][kSlots];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t b = S.bits[rc[c]];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k)
+      sc[c][k] = ((b & kBitAllocated) && (b & (1u << k))) ? planeSD(S, k, rc[c], cc) : __builtin_huge_valf();
+  }
+  clusterTriangles<true>(S, sc, rc, ox, oy, oz, a, centres, ncl, out);
+}
+
+// One cube: candidates, sort, clusters (the in-reach centres go to
+// cand[256 j + lane], their number to ncl_out), and the triangles: counted,
+// and written to out when kEmit.
 template <bool kEmit>
 __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
-                                                float* __restrict__ cand, float* out) {
+                                                float* __restrict__ cand, float* out, int& ncl_out) {
   const int tid = threadIdx.x;
   const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
   const int ox = keyX(key) * kBlockSide + lx, oy = keyY(key) * kBlockSide + ly, oz = keyZ(key) * kBlockSide + lz;
   const float hv = 0.5f * a.voxel;
   const F3 cc{(static_cast<float>(ox) + 0.5f) * a.voxel + hv, (static_cast<float>(oy) + 0.5f) * a.voxel + hv,
               (static_cast<float>(oz) + 0.5f) * a.voxel + hv};
-  // centres of the corner voxels along each axis (offset 0 and 1)
-  const float cx[2] = {(static_cast<float>(ox) + 0.5f) * a.voxel, (static_cast<float>(ox + 1) + 0.5f) * a.voxel};
-  const float cy[2] = {(static_cast<float>(oy) + 0.5f) * a.voxel, (static_cast<float>(oy + 1) + 0.5f) * a.voxel};
-  const float cz[2] = {(static_cast<float>(oz) + 0.5f) * a.voxel, (static_cast<float>(oz + 1) + 0.5f) * a.voxel};
   int rc[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c)
@@ -342,6 +413,7 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
       srt[8 * k + c] = s;
     }
   }
+  ncl_out = 0;
   if (nc == 0) return 0;
   // Observed slots are contiguous from slot 0 (sortDirectionalLayers), so
   // when no lane of the wavefront has a slot-2 (or slot-1) candidate the
@@ -387,38 +459,10 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
     }
   }
   if (!(centre - reach > a.iso || centre + reach < a.iso)) cand[256 * ncl++ + tid] = centre;
+  ncl_out = ncl;
 
   // (3) per cluster within reach
-  uint32_t ntri = 0;
-  for (int j = 0; j < ncl; ++j) {
-    const float cen = cand[256 * j + tid];
-    float val[8];
-    uint32_t valid = 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      val[c] = 0.0f;
-      int best = -1;
-      float best_delta = 3.402823466e+38f;
-#pragma unroll
-      for (int k = 0; k < kSlots; ++k) {
-        const float delta = fabsf(sc[c][k] - cen);  // +inf for a missing candidate
-        if (delta <= a.extract_tol && delta < best_delta) {
-          best_delta = delta;
-          best = k;
-        }
-      }
-      if (best < 0) continue;
-      // planeSD at the corner voxel's own centre, formed as the staging forms it
-      val[c] = planeSD(S, best, rc[c], F3{cx[kMcCornerOffset[c][0]], cy[kMcCornerOffset[c][1]], cz[kMcCornerOffset[c][2]]});
-      valid |= 1u << c;
-    }
-    if (completeCube(val, valid)) {
-      const uint32_t t = S.num_tris[cubeCase(val, a.iso)];
-      if (kEmit && t > 0) emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
-      ntri += t;
-    }
-  }
-  return ntri;
+  return clusterTriangles<kEmit>(S, sc, rc, ox, oy, oz, a, cand + tid, ncl, out);
 }
 
 // Copies the per-case triangle counts into LDS (workgroup-wide, 256 lanes).
@@ -444,8 +488,9 @@ __device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
   return incl - x;
 }
 
-// Appends `cube` (with `start`) to the compacted list when `take`; the
-// whole wavefront calls it together.
+// Appends `cube` (with `start`, unless kNoStart) to the compacted list when
+// `take`; the whole wavefront calls it together.
+template <bool kNoStart = false>
 __device__ __forceinline__ void listAppend(MeshShared& S, bool take, int cube, uint32_t start) {
   const int lane = threadIdx.x & 63;
   const unsigned long long mask = __ballot(take);
@@ -455,7 +500,7 @@ __device__ __forceinline__ void listAppend(MeshShared& S, bool take, int cube, u
   if (take) {
     const uint32_t i = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
     S.list_cube[i] = static_cast<uint16_t>(cube);
-    S.list_start[i] = start;
+    if (!kNoStart) S.list_start[i] = start;
   }
 }
 
@@ -485,7 +530,8 @@ __device__ uint32_t countItem(MeshShared& S, float* __restrict__ cand, const Vol
   uint32_t mine = 0;
   for (uint32_t j = threadIdx.x; j < n; j += 256) {
     const int cube = S.list_cube[j];
-    const uint32_t c = processCube<false>(S, key, cube, a, cand, nullptr);
+    int ncl;
+    const uint32_t c = processCube<false>(S, key, cube, a, cand, nullptr, ncl);
     cube_counts[cube] = static_cast<uint8_t>(c);
     mine += c;
   }
@@ -527,7 +573,8 @@ __device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeVi
     const uint32_t start = S.list_start[j];
     const uint32_t c = cube_counts[cube];
     if (start + c <= capacity) {
-      processCube<true>(S, key, cube, a, cand, tris + 9ull * start);
+      int ncl;
+      processCube<true>(S, key, cube, a, cand, tris + 9ull * start, ncl);
       if (tri_owner)
         for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = owner;
     }
@@ -562,10 +609,13 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
     __syncthreads();
     const uint32_t n = S.nlist;
     uint32_t mine = 0;
-    for (uint32_t j = tid; j < n; j += 256) {
+    for (uint32_t j = tid; j < n; j += 256) {  // at most 2 cubes per lane (n <= 512)
       const int cube = S.list_cube[j];
-      const uint32_t c = processCube<false>(S, key, cube, a, cand, nullptr);
+      float* const set = cand + kFusedSetWords * (j >> 8);
+      int ncl;
+      const uint32_t c = processCube<false>(S, key, cube, a, set + 256, nullptr, ncl);
       S.cnt[cube] = static_cast<uint8_t>(c);
+      set[tid] = __uint_as_float(static_cast<uint32_t>(cube) | (static_cast<uint32_t>(ncl) << 16));
       mine += c;
     }
     for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
@@ -577,25 +627,39 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
   __syncthreads();
   uint32_t base = S.base;
   if (total > 0 && base < capacity) {  // workgroup-uniform
-    const uint32_t c0 = S.cnt[tid], c1 = S.cnt[tid + 256];
-    if (tid == 0) S.nlist = 0u;
-    __syncthreads();
-    // offsets in the reference's cube order (cubes 0..255, then 256..511)
+    // offsets in the reference's cube order (cubes 0..255, then 256..511),
+    // kept per cube in list_start
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
-      const uint32_t c = half ? c1 : c0;
-      const uint32_t start = base + blockExclusiveScan(S, c);
-      listAppend(S, c > 0, tid + 256 * half, start);
-      base += S.total;
+      const int cube = tid + 256 * half;
+      S.list_start[cube] = base + blockExclusiveScan(S, S.cnt[cube]);
+      base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
     }
-    __syncthreads();
+    // the counted cubes with triangles, by their place j in the count list
+    // (which names the scratch column holding their centres)
     const uint32_t n = S.nlist;
-    for (uint32_t j = tid; j < n; j += 256) {
-      const int cube = S.list_cube[j];
-      const uint32_t start = S.list_start[j];
+    bool take[2];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const uint32_t j = tid + 256u * half;
+      take[half] = j < n && S.cnt[S.list_cube[j]] > 0;
+    }
+    __syncthreads();  // every lane has read the count list before it is overwritten
+    if (tid == 0) S.nlist = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int half = 0; half < 2; ++half) listAppend<true>(S, take[half], tid + 256 * half, 0u);  // list_start is per cube here
+    __syncthreads();
+    const uint32_t m = S.nlist;
+    for (uint32_t e = tid; e < m; e += 256) {
+      const uint32_t j = S.list_cube[e];
+      const float* const set = cand + kFusedSetWords * (j >> 8);
+      const uint32_t meta = __float_as_uint(set[j & 255u]);
+      const int cube = static_cast<int>(meta & 0xffffu), ncl = static_cast<int>(meta >> 16);
+      const uint32_t start = S.list_start[cube];
       const uint32_t c = S.cnt[cube];
       if (start + c <= capacity) {
-        processCube<true>(S, key, cube, a, cand, tris + 9ull * start);
+        emitFromCentres(S, key, cube, a, set + 256 + (j & 255u), ncl, tris + 9ull * start);
         if (tri_owner)
           for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = item;
       }
@@ -615,7 +679,7 @@ __global__ __launch_bounds__(256) void meshFusedKernel(VolumeView V, const uint3
                                                        float* __restrict__ cand) {
   __shared__ MeshShared S;
   loadCaseCounts(S);
-  float* const mine = cand + kMeshCandWords * blockIdx.x;
+  float* const mine = cand + kMeshFusedCandWords * blockIdx.x;
   const uint32_t items = kDirs * n;
   for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
     const uint32_t dir = item / n, i = item % n;
