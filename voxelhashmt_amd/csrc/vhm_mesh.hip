@@ -102,17 +102,38 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
   const float margin = fabsf(a.iso) * 1e-6f + 1e-30f;
   const float below = a.iso + margin, above = a.iso - margin;
   const int bx = keyX(key) * kBlockSide, by = keyY(key) * kBlockSide, bz = keyZ(key) * kBlockSide;
-  for (int r = tid; r < kRegion; r += 256) {
+  // All of a lane's loads first (its region voxels tid, tid + 256, tid + 512:
+  // the voxel's 3 slots of direction dir, 48 contiguous bytes each), so that
+  // they are in flight together.
+  constexpr int kIters = (kRegion + 255) / 256;
+  uint4 slots[kIters][kSlots];
+  bool alloc[kIters];
+#pragma unroll
+  for (int it = 0; it < kIters; ++it) {
+    const int r = tid + 256 * it;
+    alloc[it] = false;
+    if (r < kRegion) {
+      const int x = r % kR, y = (r / kR) % kR, z = r / (kR * kR);
+      const int sb = (x == 8 ? 1 : 0) | (y == 8 ? 2 : 0) | (z == 8 ? 4 : 0);
+      const uint32_t p = S.src[sb];
+      if (p != kInvalid) {
+        alloc[it] = true;
+        const int vi = (x & 7) + kBlockSide * ((y & 7) + kBlockSide * (z & 7));
+        const uint4* rec = reinterpret_cast<const uint4*>(V.pool + static_cast<size_t>(p) * kBlockWords +
+                                                          poolWord(dir, 0, 0, vi));
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) slots[it][k] = rec[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < kIters; ++it) {
+    const int r = tid + 256 * it;
+    if (r >= kRegion) break;
     const int x = r % kR, y = (r / kR) % kR, z = r / (kR * kR);
-    const int sb = (x == 8 ? 1 : 0) | (y == 8 ? 2 : 0) | (z == 8 ? 4 : 0);
-    const uint32_t p = S.src[sb];
     uint8_t b = 0;
-    if (p != kInvalid) {
+    if (alloc[it]) {
       b = kBitAllocated;
-      const int vi = (x & 7) + kBlockSide * ((y & 7) + kBlockSide * (z & 7));
-      // the voxel's 3 slots of direction dir: 48 contiguous bytes
-      const uint4* rec = reinterpret_cast<const uint4*>(V.pool + static_cast<size_t>(p) * kBlockWords +
-                                                        poolWord(dir, 0, 0, vi));
       const F3 centre{(static_cast<float>(bx + x) + 0.5f) * a.voxel, (static_cast<float>(by + y) + 0.5f) * a.voxel,
                       (static_cast<float>(bz + z) + 0.5f) * a.voxel};
       // Only a slot that some cluster within reach of iso (processCube,
@@ -125,7 +146,7 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
                          (fabsf(centre.x) + fabsf(centre.y) + fabsf(centre.z) + 4.0f * a.voxel) * 0x1.0p-18f;
 #pragma unroll
       for (int k = 0; k < kSlots; ++k) {
-        const uint4 slot = rec[k];  // weight, surface_coord, plane_offset, packed_normal
+        const uint4 slot = slots[it][k];  // weight, surface_coord, plane_offset, packed_normal
         if (__uint_as_float(slot.x) > 0.0f) {
           b |= static_cast<uint8_t>(1u << k);
           const F3 n = unpackNormalOct(slot.w);
@@ -672,7 +693,7 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
 // Fused incremental pass, persistent over the 6 n items (item dir * n + i:
 // listed block list[i], direction dir, skipped unless dirs[i] has bit dir).
 // total_out: zeroed; receives the number of triangles.
-__global__ __launch_bounds__(256) void meshFusedKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void meshFusedKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
                                                        MeshArgs a, const uint8_t* __restrict__ dirs,
                                                        uint32_t* __restrict__ total_out, float* __restrict__ tris,
                                                        uint32_t* __restrict__ tri_owner, uint32_t capacity,
