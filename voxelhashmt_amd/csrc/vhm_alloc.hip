@@ -171,7 +171,6 @@ __global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) {
   hdr->active_count = 0u;
   hdr->defer_count = 0u;
   hdr->integrate_ticket = 0u;
-  hdr->defer_ticket = 0u;
   hdr->frame = frame;
 }
 
@@ -183,7 +182,6 @@ __global__ void batchFrameBeginKernel(const BatchScene* __restrict__ scenes, int
   scenes[s].V.hdr->active_count = 0u;
   scenes[s].V.hdr->defer_count = 0u;
   scenes[s].V.hdr->integrate_ticket = 0u;
-  scenes[s].V.hdr->defer_ticket = 0u;
   scenes[s].V.hdr->frame = st->step + 1u;
 }
 

@@ -226,14 +226,12 @@ __device__ __forceinline__ void waveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront on its own: takes blocks by ticket (entries of the active
-// list for kFast, of the deferred list for kDeferred) until the list is
-// exhausted, runs the cheap part over a block 64 voxels at a time, queues
-// the passing voxels in its own LDS queue and runs the expensive part
-// whenever 64 are queued (then on the remainder).  No workgroup barrier, so
-// a wavefront never waits for its neighbours' slow lanes, and the tickets
-// balance dense and sparse blocks over the whole grid.
-template <Pass P>
+// The main pass, one wavefront on its own: takes active-list entries by
+// ticket until the list is exhausted, runs the cheap part over a block 64
+// voxels at a time, queues the passing voxels in its own LDS queue and runs
+// the expensive part whenever 64 are queued (then on the remainder).  No
+// workgroup barrier, so a wavefront never waits for its neighbours' slow
+// lanes, and the tickets balance dense and sparse blocks over the whole grid.
 __device__ __forceinline__ void integrateWave(WaveQueue& Q, const DepthView<float>& D, const NormalParams& np,
                                               const VolumeView& V, const IntegrateArgs& a, uint32_t n_items,
                                               uint32_t* ticket, IntegrateTally& t) {
@@ -254,13 +252,9 @@ __device__ __forceinline__ void integrateWave(WaveQueue& Q, const DepthView<floa
           more = false;
           break;
         }
-        if constexpr (P == Pass::kDeferred) {
-          p = V.defer_list[item];
-        } else {
-          p = V.vals[V.active[item]];
-          if (p >= V.max_blocks) continue;  // pool exhausted for this key (error already flagged)
-          if (lane == 0) t.visited += kBlockVoxels;
-        }
+        p = V.vals[V.active[item]];
+        if (p >= V.max_blocks) continue;  // pool exhausted for this key (error already flagged)
+        if (lane == 0) t.visited += kBlockVoxels;
         const uint64_t key = V.pool_keys[p];
         bx = keyX(key) * kBlockSide;
         by = keyY(key) * kBlockSide;
@@ -276,14 +270,10 @@ __device__ __forceinline__ void integrateWave(WaveQueue& Q, const DepthView<floa
       float u, v, dep;
       bool pass = observeDepth(D, np, a.T_cw, pw, a.trunc, pc, u, v, dep);
       const bool border = pass && !fastNormalApplies(u, v);
-      if constexpr (P == Pass::kFast) {
-        // list the block once (the flag is cleared by the deferred pass)
-        if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
-          V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
-        pass = pass && !border;
-      } else {
-        pass = border;
-      }
+      // list the block once for the deferred pass (which clears the flag)
+      if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
+        V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
+      pass = pass && !border;
       const unsigned long long mask = __ballot(pass);
       if (pass) {
         const uint32_t i = qn + static_cast<uint32_t>(__popcll(mask & below));
@@ -297,16 +287,13 @@ __device__ __forceinline__ void integrateWave(WaveQueue& Q, const DepthView<floa
         Q.pcz[i] = pc.z;
       }
       qn += static_cast<uint32_t>(__popcll(mask));
-      if (++round == 8) {
-        if constexpr (P == Pass::kDeferred) {
-          if (lane == 0) V.defer_flags[p] = 0u;
-        }
-      }
+      ++round;
     }
     if (qn == 0u) break;  // and no block left
     waveSync();
     const uint32_t m = qn < 64u ? qn : 64u;
-    if (static_cast<uint32_t>(lane) < m) integrateQueued<P>(Q, static_cast<int>(qn - m) + lane, D, np, V, a, t);
+    if (static_cast<uint32_t>(lane) < m)
+      integrateQueued<Pass::kFast>(Q, static_cast<int>(qn - m) + lane, D, np, V, a, t);
     qn -= m;
     waveSync();  // the drained entries are rewritten by the next rounds
   }
@@ -337,20 +324,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   __shared__ uint64_t exp_tab[32];
   const NormalParams np = normalParams(a, expTableLds(exp_tab));
   IntegrateTally t;
-  integrateWave<Pass::kFast>(Q[threadIdx.x >> 6], D, np, V, a, V.hdr->active_count, &V.hdr->integrate_ticket, t);
+  integrateWave(Q[threadIdx.x >> 6], D, np, V, a, V.hdr->active_count, &V.hdr->integrate_ticket, t);
   flushTally(V, t);
 }
 
 // The deferred pass: the border voxels of the blocks the main pass listed.
 __global__ __launch_bounds__(256) void integrateDeferredKernel(const float* __restrict__ depth, VolumeView V,
                                                                IntegrateArgs a) {
-  __shared__ WaveQueue Q[4];
+  // A few percent of the blocks, each with few border voxels: a workgroup
+  // per block keeps the latency of a block at two 256-voxel rounds.
+  __shared__ IntegrateQueue Q;
+  if (threadIdx.x == 0) Q.n = 0;
+  __syncthreads();
   const DepthView<float> D{depth, a.K.width, a.K.height};
   __shared__ uint64_t exp_tab[32];
   const NormalParams np = normalParams(a, expTableLds(exp_tab));
+  const uint32_t n = V.hdr->defer_count;  // reset by the next frame's frameBeginKernel
+  const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t begin = blockIdx.x * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
   IntegrateTally t;
-  // defer_count and the ticket are reset by the next frame's frameBeginKernel
-  integrateWave<Pass::kDeferred>(Q[threadIdx.x >> 6], D, np, V, a, V.hdr->defer_count, &V.hdr->defer_ticket, t);
+  if (begin < end) integrateRange<Pass::kDeferred>(Q, D, np, V, a, begin, end, t);
   flushTally(V, t);
 }
 

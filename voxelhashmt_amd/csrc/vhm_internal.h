@@ -51,8 +51,6 @@ struct DevHeader {
   unsigned long long mesh_total;
   uint32_t defer_count;  // blocks appended to the integrator's deferred list this frame
   uint32_t integrate_ticket;  // integrator: next active-list entry to hand out this frame
-  uint32_t defer_ticket;      // deferred pass: next deferred-list entry to hand out this frame
-  uint32_t pad;
 };
 
 struct VolumeView {

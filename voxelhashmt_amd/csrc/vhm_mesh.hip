@@ -47,8 +47,7 @@ struct MeshShared {
   uint8_t bits[kRegion];       // bit k: slot k observed; kBit* above
   uint32_t src[8];        // pool index of self + 7 positive neighbours (kInvalid if absent)
   uint32_t any_dir;       // OR over the 8 source blocks' direction masks
-  uint32_t scan[256];
-  uint32_t total;
+  uint32_t scan[4];  // per-wavefront partial sums
   // cubes that may produce triangles, compacted so that busy lanes are dense
   uint16_t list_cube[kBlockVoxels];
   uint32_t list_start[kBlockVoxels];
@@ -492,21 +491,23 @@ __device__ __forceinline__ void loadCaseCounts(MeshShared& S) {
   __syncthreads();
 }
 
-// Workgroup-wide exclusive scan of one value per lane (lane order).
-__device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x) {
-  const int tid = threadIdx.x;
-  S.scan[tid] = x;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const uint32_t y = tid >= off ? S.scan[tid - off] : 0u;
-    __syncthreads();
-    S.scan[tid] += y;
-    __syncthreads();
+// Workgroup-wide exclusive scan of one value per lane (lane order); total
+// receives the sum.  Each wavefront scans in registers, so the workgroup
+// meets at two barriers only.
+__device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x, uint32_t& total) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t v = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(v, off);
+    if (lane >= off) v += y;
   }
-  const uint32_t incl = S.scan[tid];
-  if (tid == 255) S.total = incl;
+  if (lane == 63) S.scan[wave] = v;
   __syncthreads();
-  return incl - x;
+  const uint32_t s0 = S.scan[0], s1 = S.scan[1], s2 = S.scan[2], s3 = S.scan[3];
+  __syncthreads();  // S.scan is rewritten by the next scan
+  total = s0 + s1 + s2 + s3;
+  return (wave > 0 ? s0 : 0u) + (wave > 1 ? s1 : 0u) + (wave > 2 ? s2 : 0u) + v - x;
 }
 
 // Appends `cube` (with `start`, unless kNoStart) to the compacted list when
@@ -583,9 +584,10 @@ __device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeVi
   for (int half = 0; half < 2; ++half) {
     const int cube = threadIdx.x + 256 * half;
     const uint32_t c = cube_counts[cube];
-    const uint32_t start = base + blockExclusiveScan(S, c);
+    uint32_t sum;
+    const uint32_t start = base + blockExclusiveScan(S, c, sum);
     listAppend(S, c > 0, cube, start);
-    base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
+    base += sum;
   }
   __syncthreads();
   const uint32_t n = S.nlist;
@@ -653,8 +655,9 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
       const int cube = tid + 256 * half;
-      S.list_start[cube] = base + blockExclusiveScan(S, S.cnt[cube]);
-      base += S.total;  // read before the next scan can overwrite it (that scan syncs first)
+      uint32_t sum;
+      S.list_start[cube] = base + blockExclusiveScan(S, S.cnt[cube], sum);
+      base += sum;
     }
     // the counted cubes with triangles, by their place j in the count list
     // (which names the scratch column holding their centres)
