@@ -195,9 +195,6 @@ struct BatchScene {
   uint32_t* remesh_flags;     // [max_blocks]
   uint32_t* remesh_list;      // [max_blocks]
   uint8_t* remesh_dirs;       // [max_blocks] directions to re-mesh per listed block
-  uint32_t* mesh_counts;      // [6 * max_blocks]
-  uint32_t* mesh_offsets;     // [6 * max_blocks]
-  uint8_t* cube_counts;       // [6 * max_blocks][512]
   float* tris;                // [tri_capacity][9]
   uint32_t tri_capacity;
   uint32_t* out;              // [4]: blocks re-meshed, triangles (total, may exceed capacity)

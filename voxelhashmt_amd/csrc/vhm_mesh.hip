@@ -948,56 +948,36 @@ __global__ __launch_bounds__(256) void compactRemeshBatchKernel(const BatchScene
   if (threadIdx.x == 0) sc.out[0] = m;
 }
 
+// Item prefix over the scenes (6 items per listed block); also zeroes each
+// scene's triangle total, which the fused pass accumulates.
 __global__ void meshPrefixKernel(const BatchScene* __restrict__ scenes, int S, uint32_t* __restrict__ prefix) {
   if (threadIdx.x != 0) return;
   uint32_t acc = 0;
   for (int s = 0; s < S; ++s) {
     prefix[s] = acc;
     acc += 6u * scenes[s].out[0];
+    scenes[s].out[1] = 0u;
   }
   prefix[S] = acc;
 }
 
-__global__ __launch_bounds__(256) void meshCountBatchKernel(const BatchScene* __restrict__ scenes, int S,
-                                                            const uint32_t* __restrict__ prefix,
-                                                            float* __restrict__ cand) {
+// Fused count + emit over the concatenated items of every scene: each item's
+// triangles go to one range of its scene's buffer, taken with an atomic add
+// on the scene's total (items in completion order).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void meshFusedBatchKernel(
+    const BatchScene* __restrict__ scenes, int S, const uint32_t* __restrict__ prefix, float* __restrict__ cand) {
   __shared__ MeshShared M;
   loadCaseCounts(M);
-  float* const mine = cand + kMeshCandWords * blockIdx.x;
+  float* const mine = cand + kMeshFusedCandWords * blockIdx.x;
   const uint32_t total = prefix[S];
   for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
-    const int s = findSegment(prefix, S, i);
+    const int s = __builtin_amdgcn_readfirstlane(findSegment(prefix, S, i));  // uniform: scene fields in SGPRs
     const BatchScene& sc = scenes[s];
     const uint32_t m = sc.out[0], j = i - prefix[s];
     const uint32_t dir = j / m, idx = j % m;
-    uint32_t c = 0;
     if (sc.remesh_dirs[idx] & (1u << dir))  // workgroup-uniform
-      c = countItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma,
-                    sc.cube_counts + 512ull * (dir * m + idx));
-    if (threadIdx.x == 0) sc.mesh_counts[dir * m + idx] = c;
-  }
-}
-
-__global__ __launch_bounds__(256) void meshScanBatchKernel(const BatchScene* __restrict__ scenes) {
-  const BatchScene& sc = scenes[blockIdx.x];
-  const uint32_t t = workgroupScan(sc.mesh_counts, 6u * sc.out[0], sc.mesh_offsets, nullptr, nullptr);
-  if (threadIdx.x == 0) sc.out[1] = t;
-}
-
-__global__ __launch_bounds__(256) void meshEmitBatchKernel(const BatchScene* __restrict__ scenes, int S,
-                                                           const uint32_t* __restrict__ prefix,
-                                                           float* __restrict__ cand) {
-  __shared__ MeshShared M;
-  loadCaseCounts(M);
-  float* const mine = cand + kMeshCandWords * blockIdx.x;
-  const uint32_t total = prefix[S];
-  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
-    const int s = findSegment(prefix, S, i);
-    const BatchScene& sc = scenes[s];
-    const uint32_t m = sc.out[0], j = i - prefix[s];
-    const uint32_t dir = j / m, idx = j % m, item = dir * m + idx;
-    emitItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, sc.mesh_offsets[item],
-             sc.mesh_counts[item], sc.cube_counts + 512ull * item, sc.tris, nullptr, 0u, sc.tri_capacity);
+      fusedItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, dir * m + idx, sc.out + 1, sc.tris,
+                nullptr, sc.tri_capacity);
   }
 }
 
@@ -1076,9 +1056,7 @@ hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, in
   markRemeshBatchKernel<<<dim3(64, S), 256, 0, s>>>(scenes);
   compactRemeshBatchKernel<<<S, 256, 0, s>>>(scenes);
   meshPrefixKernel<<<1, 64, 0, s>>>(scenes, S, prefix);
-  meshCountBatchKernel<<<g, 256, 0, s>>>(scenes, S, prefix, cand);
-  meshScanBatchKernel<<<S, 256, 0, s>>>(scenes);
-  meshEmitBatchKernel<<<g, 256, 0, s>>>(scenes, S, prefix, cand);
+  meshFusedBatchKernel<<<g, 256, 0, s>>>(scenes, S, prefix, cand);
   return hipGetLastError();
 }
 
