@@ -9,14 +9,16 @@
 // integrateActiveBlocksKernel (cuda_tsdf_integrator.cu:834-903), which
 // reads/writes 288-B AoS voxels.
 //
-// Work decomposition: a persistent grid of 256-lane workgroups strides over
-// the frame's active list (its length is read on the device, so the host never
-// waits for allocation).  A workgroup owns one block at a time, 2 voxels per
-// lane.  A voxel reads and writes only the 3 slots x 4 fields of the direction
-// its normal selects; each (field, slot) row of a block is 2 KiB contiguous in
-// the SoA pool, so lanes of a wavefront touch consecutive words.  The observed
-// bitmask is updated per wavefront from a ballot (each wavefront owns 64
-// consecutive voxels of the block), which keeps observedVoxelCount O(1).
+// Work decomposition (main pass): every wavefront takes blocks of the
+// frame's active list by ticket (the list's length is read on the device,
+// so the host never waits for allocation), runs the cheap half of each
+// voxel's observation over its block 64 voxels at a time and the expensive
+// half over the voxels that passed, 64 at a time from its own LDS queue.
+// The deferred pass (border voxels) and the batched kernels share one queue
+// per workgroup instead.  A voxel reads and writes only the 48-byte record
+// (3 slots x 4 fields) of the direction its normal selects.  A per-block
+// observed bitmask, set with an atomic OR the first time a voxel is
+// observed, keeps observedVoxelCount O(1).
 #include "vhm_integrate_core.h"
 #include "vhm_internal.h"
 
