@@ -62,6 +62,10 @@ struct vhm_ctx {
   size_t mesh_cand_cap = 0;
   uint32_t* mesh_counters = nullptr;  // fused incremental pass: triangle total
   size_t mesh_counters_cap = 0;
+  uint32_t* mesh_nbr = nullptr;  // fused incremental pass: 8 source blocks per listed block
+  size_t mesh_nbr_cap = 0;
+  uint8_t* mesh_obs_dirs = nullptr;  // fused incremental pass: observed directions over the sources
+  size_t mesh_obs_dirs_cap = 0;
   float* tris = nullptr;
   size_t tris_cap = 0;
   int32_t* key_buf = nullptr;
@@ -479,7 +483,7 @@ int vhm_destroy(vhm_ctx* c) {
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
-                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->mesh_counters,
+                  c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->mesh_counters, c->mesh_nbr, c->mesh_obs_dirs,
                   c->defer_flags, c->defer_list, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
                   c->pc_vals, c->pc_vals_tmp, c->pc_count, c->pc_hist, c->pc_offsets, c->pc_partials};
   for (void* b : bufs)
@@ -866,10 +870,13 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
     // straight into the caller's device buffers in one fused pass (items in
     // completion order; triangles past the capacity are dropped)
     if ((rc = ensure(c, &c->mesh_counters, &c->mesh_counters_cap, 1, "hipMalloc mesh counters"))) return rc;
+    if ((rc = ensure(c, &c->mesh_nbr, &c->mesh_nbr_cap, 8 * size_t(m), "hipMalloc mesh sources"))) return rc;
+    if ((rc = ensure(c, &c->mesh_obs_dirs, &c->mesh_obs_dirs_cap, size_t(m), "hipMalloc mesh source directions")))
+      return rc;
     VHM_TRY(c, hipMemsetAsync(c->mesh_counters, 0, sizeof(uint32_t), c->stream), "hipMemset mesh counters");
     const uint32_t cap = static_cast<uint32_t>(tri_capacity < 0xFFFFFFFFu ? tri_capacity : 0xFFFFFFFFu);
-    VHM_TRY(c, launchMeshFused(view(c), c->remesh_list, m, ma, c->remesh_dirs, c->mesh_counters, tris_out,
-                               tri_item_out, cap, c->mesh_cand, c->stream),
+    VHM_TRY(c, launchMeshFused(view(c), c->remesh_list, m, ma, c->remesh_dirs, c->mesh_nbr, c->mesh_obs_dirs,
+                               c->mesh_counters, tris_out, tri_item_out, cap, c->mesh_cand, c->stream),
             "fused mesh kernel launch");
     uint32_t total = 0;
     VHM_TRY(c, hipMemcpyAsync(&total, c->mesh_counters, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),

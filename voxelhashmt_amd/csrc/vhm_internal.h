@@ -160,9 +160,11 @@ size_t scanPartialsSize(uint32_t n);
 // Fused count + emit of listed (block, direction) items into a device
 // buffer: each item's triangles contiguous and in the two-pass order, the
 // items in completion order; total: one zeroed word, receives the count.
+// nbr (8 n words) and observed_dirs (n bytes): scratch for the items'
+// source blocks.
 hipError_t launchMeshFused(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           const uint8_t* dirs, uint32_t* total, float* tris, uint32_t* tri_owner, uint32_t capacity,
-                           float* cand, hipStream_t s);
+                           const uint8_t* dirs, uint32_t* nbr, uint8_t* observed_dirs, uint32_t* total, float* tris,
+                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s);
 hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                           const uint32_t* counts, const uint8_t* cube_counts, const uint32_t* offsets, float* tris,
                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s);

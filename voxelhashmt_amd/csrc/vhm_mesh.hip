@@ -82,22 +82,31 @@ __device__ uint32_t lookupPool(const VolumeView& V, int x, int y, int z) {
 // drawn from these, so a cube whose 8 corners never have both bits can
 // produce no triangle.  The margin (1e-6 |iso| + 1e-30) exceeds the rounding
 // of those means, and NaN sets both bits.
+//
+// src8 (optional): the 8 source blocks, already looked up (meshNeighbours),
+// for an item the caller knows to have an observed source in `dir`.
 __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, uint32_t self, int dir,
-                            const MeshArgs& a) {
+                            const MeshArgs& a, const uint32_t* __restrict__ src8 = nullptr) {
   const int tid = threadIdx.x;
-  if (tid == 0) {
-    S.any_dir = 0u;
-    S.nlist = 0u;
+  if (src8) {
+    if (tid < 8) S.src[tid] = src8[tid];
+    if (tid == 0) S.nlist = 0u;
+    __syncthreads();
+  } else {
+    if (tid == 0) {
+      S.any_dir = 0u;
+      S.nlist = 0u;
+    }
+    __syncthreads();
+    if (tid < 8) {
+      const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
+      uint32_t p = tid == 0 ? self : lookupPool(V, keyX(key) + ox, keyY(key) + oy, keyZ(key) + oz);
+      S.src[tid] = (p < V.max_blocks) ? p : kInvalid;
+      if (p < V.max_blocks && (V.dirmask[p] & (1u << dir))) atomicOr(&S.any_dir, 1u);
+    }
+    __syncthreads();
+    if (S.any_dir == 0u) return false;
   }
-  __syncthreads();
-  if (tid < 8) {
-    const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
-    uint32_t p = tid == 0 ? self : lookupPool(V, keyX(key) + ox, keyY(key) + oy, keyZ(key) + oz);
-    S.src[tid] = (p < V.max_blocks) ? p : kInvalid;
-    if (p < V.max_blocks && (V.dirmask[p] & (1u << dir))) atomicOr(&S.any_dir, 1u);
-  }
-  __syncthreads();
-  if (S.any_dir == 0u) return false;
   const float margin = fabsf(a.iso) * 1e-6f + 1e-30f;
   const float below = a.iso + margin, above = a.iso - margin;
   const int bx = keyX(key) * kBlockSide, by = keyY(key) * kBlockSide, bz = keyZ(key) * kBlockSide;
@@ -617,11 +626,11 @@ __device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeVi
 // One item: stage once, count every candidate cube, take the offset, emit.
 __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const VolumeView& V, uint32_t b, int dir,
                               const MeshArgs& a, uint32_t item, uint32_t* total_out, float* tris, uint32_t* tri_owner,
-                              uint32_t capacity) {
+                              uint32_t capacity, const uint32_t* src8 = nullptr) {
   const int tid = threadIdx.x;
   const uint64_t key = V.pool_keys[b];
   uint32_t total = 0;
-  const bool staged = stageRegion(S, V, key, b, dir, a);  // workgroup-uniform
+  const bool staged = stageRegion(S, V, key, b, dir, a, src8);  // workgroup-uniform
   if (staged) {
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
@@ -698,6 +707,8 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
 // total_out: zeroed; receives the number of triangles.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void meshFusedKernel(VolumeView V, const uint32_t* __restrict__ list, uint32_t n,
                                                        MeshArgs a, const uint8_t* __restrict__ dirs,
+                                                       const uint32_t* __restrict__ nbr,
+                                                       const uint8_t* __restrict__ observed_dirs,
                                                        uint32_t* __restrict__ total_out, float* __restrict__ tris,
                                                        uint32_t* __restrict__ tri_owner, uint32_t capacity,
                                                        float* __restrict__ cand) {
@@ -707,9 +718,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const uint32_t items = kDirs * n;
   for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
     const uint32_t dir = item / n, i = item % n;
-    if (dirs[i] & (1u << dir))  // workgroup-uniform
-      fusedItem(S, mine, V, list[i], static_cast<int>(dir), a, item, total_out, tris, tri_owner, capacity);
+    // an item none of whose 8 sources observed the direction has no triangle
+    if (dirs[i] & observed_dirs[i] & (1u << dir))  // workgroup-uniform
+      fusedItem(S, mine, V, list[i], static_cast<int>(dir), a, item, total_out, tris, tri_owner, capacity,
+                nbr + 8ull * i);
   }
+}
+
+// For listed block i: its 8 source blocks (self and the 7 positive
+// neighbours, kInvalid when absent) at nbr[8 i ..], and the OR of their
+// observed-direction masks at observed_dirs[i].  Eight lanes per block.
+__global__ __launch_bounds__(256) void meshNeighboursKernel(VolumeView V, const uint32_t* __restrict__ list,
+                                                            uint32_t n, uint32_t* __restrict__ nbr,
+                                                            uint8_t* __restrict__ observed_dirs) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = g >> 3;
+  const int k = static_cast<int>(g & 7u);
+  uint32_t dm = 0;
+  if (i < n) {
+    const uint32_t self = list[i];
+    const uint64_t key = V.pool_keys[self];
+    const uint32_t p = k == 0 ? self : lookupPool(V, keyX(key) + (k & 1), keyY(key) + ((k >> 1) & 1), keyZ(key) + (k >> 2));
+    const bool ok = p < V.max_blocks;
+    nbr[8ull * i + k] = ok ? p : kInvalid;
+    dm = ok ? V.dirmask[p] : 0u;
+  }
+  // OR over the block's 8 lanes (aligned groups of 8 within the wavefront)
+  dm |= __shfl_xor(dm, 1);
+  dm |= __shfl_xor(dm, 2);
+  dm |= __shfl_xor(dm, 4);
+  if (i < n && k == 0) observed_dirs[i] = static_cast<uint8_t>(dm);
 }
 
 // Persistent over the 6 n items: item dir * n + i meshes block list[i] (or
@@ -986,10 +1024,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 static uint32_t meshGrid(uint32_t items) { return items < kMeshGridMax ? items : kMeshGridMax; }
 
 hipError_t launchMeshFused(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
-                           const uint8_t* dirs, uint32_t* total, float* tris, uint32_t* tri_owner, uint32_t capacity,
-                           float* cand, hipStream_t s) {
+                           const uint8_t* dirs, uint32_t* nbr, uint8_t* observed_dirs, uint32_t* total, float* tris,
+                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  meshFusedKernel<<<meshGrid(kDirs * n), 256, 0, s>>>(V, list, n, a, dirs, total, tris, tri_owner, capacity, cand);
+  meshNeighboursKernel<<<(8ull * n + 255) / 256, 256, 0, s>>>(V, list, n, nbr, observed_dirs);
+  meshFusedKernel<<<meshGrid(kDirs * n), 256, 0, s>>>(V, list, n, a, dirs, nbr, observed_dirs, total, tris, tri_owner,
+                                                       capacity, cand);
   return hipGetLastError();
 }
 
