@@ -47,7 +47,8 @@ struct MeshShared {
   uint8_t bits[kRegion];       // bit k: slot k observed; kBit* above
   uint32_t src[8];        // pool index of self + 7 positive neighbours (kInvalid if absent)
   uint32_t any_dir;       // OR over the 8 source blocks' direction masks
-  uint32_t scan[4];  // per-wavefront partial sums
+  uint32_t scan[8];  // per-wavefront partial sums (fused pass: two halves)
+  uint32_t nemit;    // fused pass: length of the emit list
   // cubes that may produce triangles, compacted so that busy lanes are dense
   uint16_t list_cube[kBlockVoxels];
   uint32_t list_start[kBlockVoxels];
@@ -90,12 +91,16 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
   const int tid = threadIdx.x;
   if (src8) {
     if (tid < 8) S.src[tid] = src8[tid];
-    if (tid == 0) S.nlist = 0u;
+    if (tid == 0) {
+      S.nlist = 0u;
+      S.nemit = 0u;
+    }
     __syncthreads();
   } else {
     if (tid == 0) {
       S.any_dir = 0u;
       S.nlist = 0u;
+      S.nemit = 0u;
     }
     __syncthreads();
     if (tid < 8) {
@@ -627,69 +632,87 @@ __device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeVi
 __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const VolumeView& V, uint32_t b, int dir,
                               const MeshArgs& a, uint32_t item, uint32_t* total_out, float* tris, uint32_t* tri_owner,
                               uint32_t capacity, const uint32_t* src8 = nullptr) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t key = V.pool_keys[b];
-  uint32_t total = 0;
-  const bool staged = stageRegion(S, V, key, b, dir, a, src8);  // workgroup-uniform
-  if (staged) {
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const int cube = tid + 256 * half;
-      S.cnt[cube] = 0;
-      listAppend(S, cubeMayCross(S, cube), cube, 0u);
-    }
-    __syncthreads();
-    const uint32_t n = S.nlist;
-    uint32_t mine = 0;
-    for (uint32_t j = tid; j < n; j += 256) {  // at most 2 cubes per lane (n <= 512)
-      const int cube = S.list_cube[j];
-      float* const set = cand + kFusedSetWords * (j >> 8);
-      int ncl;
-      const uint32_t c = processCube<false>(S, key, cube, a, set + 256, nullptr, ncl);
-      S.cnt[cube] = static_cast<uint8_t>(c);
-      set[tid] = __uint_as_float(static_cast<uint32_t>(cube) | (static_cast<uint32_t>(ncl) << 16));
-      mine += c;
-    }
-    for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
-    if ((tid & 63) == 0) S.scan[tid >> 6] = mine;
-    __syncthreads();
-    total = S.scan[0] + S.scan[1] + S.scan[2] + S.scan[3];
+  if (!stageRegion(S, V, key, b, dir, a, src8)) {  // workgroup-uniform
+    __syncthreads();  // the item's LDS is reused by the next item
+    return 0u;
   }
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int cube = tid + 256 * half;
+    S.cnt[cube] = 0;
+    listAppend(S, cubeMayCross(S, cube), cube, 0u);
+  }
+  __syncthreads();
+  // count: a lane takes places j = tid and tid + 256 of the count list; the
+  // centres of each go to scratch set j >> 8, column tid
+  const uint32_t n = S.nlist;
+  uint32_t cj0 = 0u, cj1 = 0u;
+#pragma unroll 1
+  for (uint32_t j = tid; j < n; j += 256) {  // one processCube in the code
+    const int cube = S.list_cube[j];
+    float* const set = cand + kFusedSetWords * (j >> 8);
+    int ncl;
+    const uint32_t c = processCube<false>(S, key, cube, a, set + 256, nullptr, ncl);
+    S.cnt[cube] = static_cast<uint8_t>(c);
+    set[tid] = __uint_as_float(static_cast<uint32_t>(cube) | (static_cast<uint32_t>(ncl) << 16));
+    if (j < 256u) cj0 = c; else cj1 = c;
+  }
+  __syncthreads();  // every count is in S.cnt, every lane has read its count-list places
+  // offsets in the reference's cube order (cubes 0..255, then 256..511):
+  // both halves scanned per wavefront in registers
+  const uint32_t c0 = S.cnt[tid], c1 = S.cnt[tid + 256];
+  uint32_t v0 = c0, v1 = c1;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y0 = __shfl_up(v0, off), y1 = __shfl_up(v1, off);
+    if (lane >= off) {
+      v0 += y0;
+      v1 += y1;
+    }
+  }
+  if (lane == 63) {
+    S.scan[wave] = v0;
+    S.scan[4 + wave] = v1;
+  }
+  // the count-list places j of the cubes with triangles (they name the
+  // scratch columns holding the centres), compacted into list_cube
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const bool take = (half ? cj1 : cj0) > 0u;
+    const unsigned long long mask = __ballot(take);
+    uint32_t at = 0;
+    if (lane == 0 && mask != 0ull) at = atomicAdd(&S.nemit, static_cast<uint32_t>(__popcll(mask)));
+    at = __shfl(at, 0);
+    if (take) S.list_cube[at + __popcll(mask & ((1ull << lane) - 1ull))] = static_cast<uint16_t>(tid + 256 * half);
+  }
+  __syncthreads();
+  uint32_t t0 = 0, t1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t s0 = S.scan[w], s1 = S.scan[4 + w];
+    t0 += s0;
+    t1 += s1;
+    if (w < wave) {
+      b0 += s0;
+      b1 += s1;
+    }
+  }
+  const uint32_t total = t0 + t1;
+  S.list_start[tid] = b0 + v0 - c0;  // relative to the item's first triangle
+  S.list_start[tid + 256] = t0 + b1 + v1 - c1;
   if (tid == 0) S.base = total ? atomicAdd(total_out, total) : 0u;
   __syncthreads();
-  uint32_t base = S.base;
+  const uint32_t base = S.base;
   if (total > 0 && base < capacity) {  // workgroup-uniform
-    // offsets in the reference's cube order (cubes 0..255, then 256..511),
-    // kept per cube in list_start
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const int cube = tid + 256 * half;
-      uint32_t sum;
-      S.list_start[cube] = base + blockExclusiveScan(S, S.cnt[cube], sum);
-      base += sum;
-    }
-    // the counted cubes with triangles, by their place j in the count list
-    // (which names the scratch column holding their centres)
-    const uint32_t n = S.nlist;
-    bool take[2];
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const uint32_t j = tid + 256u * half;
-      take[half] = j < n && S.cnt[S.list_cube[j]] > 0;
-    }
-    __syncthreads();  // every lane has read the count list before it is overwritten
-    if (tid == 0) S.nlist = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int half = 0; half < 2; ++half) listAppend<true>(S, take[half], tid + 256 * half, 0u);  // list_start is per cube here
-    __syncthreads();
-    const uint32_t m = S.nlist;
+    const uint32_t m = S.nemit;
     for (uint32_t e = tid; e < m; e += 256) {
       const uint32_t j = S.list_cube[e];
       const float* const set = cand + kFusedSetWords * (j >> 8);
       const uint32_t meta = __float_as_uint(set[j & 255u]);
       const int cube = static_cast<int>(meta & 0xffffu), ncl = static_cast<int>(meta >> 16);
-      const uint32_t start = S.list_start[cube];
+      const uint32_t start = base + S.list_start[cube];
       const uint32_t c = S.cnt[cube];
       if (start + c <= capacity) {
         emitFromCentres(S, key, cube, a, set + 256 + (j & 255u), ncl, tris + 9ull * start);
