@@ -524,9 +524,8 @@ __device__ uint32_t blockExclusiveScan(MeshShared& S, uint32_t x, uint32_t& tota
   return (wave > 0 ? s0 : 0u) + (wave > 1 ? s1 : 0u) + (wave > 2 ? s2 : 0u) + v - x;
 }
 
-// Appends `cube` (with `start`, unless kNoStart) to the compacted list when
-// `take`; the whole wavefront calls it together.
-template <bool kNoStart = false>
+// Appends `cube` (with `start`) to the compacted list when `take`; the
+// whole wavefront calls it together.
 __device__ __forceinline__ void listAppend(MeshShared& S, bool take, int cube, uint32_t start) {
   const int lane = threadIdx.x & 63;
   const unsigned long long mask = __ballot(take);
@@ -536,7 +535,7 @@ __device__ __forceinline__ void listAppend(MeshShared& S, bool take, int cube, u
   if (take) {
     const uint32_t i = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
     S.list_cube[i] = static_cast<uint16_t>(cube);
-    if (!kNoStart) S.list_start[i] = start;
+    S.list_start[i] = start;
   }
 }
 
