@@ -1,5 +1,8 @@
-// Directional multi-layer marching cubes in two passes (count, then emit
-// at scanned offsets), with a deterministic output order.
+// Directional multi-layer marching cubes.  Full extraction runs two passes
+// (count, then emit at scanned offsets) with a deterministic output order;
+// incremental extraction into device buffers and the batched step run one
+// fused pass that counts, takes the item's output range with an atomic add
+// and emits.
 //
 // Reference: VoxelHashTSDF::extractMesh / extractDirectionalMultiLayerMesh
 // (voxel_hash_tsdf.h:99-107, 842-939) with selectClosestLayerForSurface
@@ -9,14 +12,15 @@
 // with a host-built lookup table and an atomic output cursor
 // (cuda_mesh_extractor.cu:553-720).
 //
-// One workgroup handles one (direction, block): it looks up the 7 positive
-// neighbour blocks in the persistent hash table, stages the 9x9x9 corner
-// region of that direction in LDS (unpacked slot normals, plane offsets,
-// observed bits -- each voxel's normals are unpacked once instead of once per
-// cube corner), and each lane runs two cubes.  Candidate plane distances are
-// kept sorted in a per-lane LDS column; clusters are consumed as they close,
-// so no cluster array is stored.  Triangles of a workgroup are written in
-// cube order at offset[(dir, block)] + the workgroup's exclusive scan.
+// One workgroup handles one (direction, block) item at a time: it finds the
+// 7 positive neighbour blocks, stages the 9x9x9 corner region of that
+// direction in LDS (unpacked slot normals and plane offsets, observed and
+// sign bits -- each voxel's normals are unpacked once instead of once per
+// cube corner), compacts the cubes whose corners may straddle iso over the
+// lanes, and runs each such cube on one lane: its 24 candidate distances
+// sorted in registers, greedy clusters (the centres within reach of iso in a
+// per-workgroup global scratch column), and per cluster the corner values,
+// completion and triangles.  Triangles of an item are written in cube order.
 #include "vhm_internal.h"
 #include "vhm_mc_tables.h"
 
