@@ -5,9 +5,14 @@
 * 1280x720 plain-at-scale room, 5 mm voxels: frame 0 against the oracle
   (keys, layers, counters), then 40 frames through the GPU only, checked by
   size-independent properties: incremental meshes per (block, direction)
-  reassemble the full extraction after every call, and a second volume fed
-  the same frames ends bit-identical on a block sample.
+  reassemble the full extraction after every call, a second volume fed the
+  same frames and meshed into device buffers (the fused pass) yields the
+  same triangles per (block, direction) as the first one's host output
+  (the two-pass path), and the two volumes end bit-identical on a block
+  sample.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -59,6 +64,35 @@ def test_plain_scale_1280x720_first_frame(gpu):
     _against_oracle(synth.plain_scale(n_frames=1), 1, mesh=False)
 
 
+def _items(keys, tris, items):
+    """{(key, direction): triangles of that item in output order}."""
+    m = len(keys)
+    out = {}
+    items = items.astype(np.int64)
+    order = np.argsort(items, kind="stable")
+    ids, starts = np.unique(items[order], return_index=True)
+    ends = np.append(starts[1:], len(order))
+    for t, s0, e0 in zip(ids, starts, ends):
+        out[(tuple(keys[int(t) % m]), int(t) // m)] = tris[order[s0:e0]]
+    return out
+
+
+def _device_incremental(vol, dev_tris, dev_items):
+    """vhm_extract_mesh_incremental into device buffers (host keys and directions)."""
+    cap_b = max(vol.block_count(), 1)
+    keys = np.zeros((cap_b, 3), np.int32)
+    dirs = np.zeros(cap_b, np.uint8)
+    nb, nt = C.c_size_t(), C.c_size_t()
+    rc = vhm.lib().vhm_extract_mesh_incremental(vol._h, 0.0, keys.ctypes.data, dirs.ctypes.data, cap_b, C.byref(nb),
+                                                dev_tris.data_ptr(), dev_items.data_ptr(), dev_tris.numel() // 9,
+                                                vhm.VHM_DEVICE, C.byref(nt))
+    assert rc == 0, vhm.lib().vhm_last_error(vol._h)
+    assert nt.value <= dev_items.numel()
+    tris = dev_tris[: 9 * nt.value].cpu().numpy().reshape(-1, 3, 3)
+    items = dev_items[: nt.value].cpu().numpy().view(np.uint32)
+    return keys[: nb.value], dirs[: nb.value], tris, items
+
+
 def test_plain_scale_1280x720_properties(gpu):
     import torch
     w = synth.plain_scale(n_frames=40)
@@ -67,6 +101,9 @@ def test_plain_scale_1280x720_properties(gpu):
     vp = _params(w)
     vp.max_blocks = 200_000
     a, b = vhm.Volume(vp), vhm.Volume(vp)
+    cap = 16_000_000
+    dev_tris = torch.empty(cap * 9, dtype=torch.float32, device="cuda:0")
+    dev_items = torch.empty(cap, dtype=torch.int32, device="cuda:0")
     per_item = {}
     for i in range(40):
         a.integrate(frames[i], w.K, w.poses[i])
@@ -84,6 +121,12 @@ def test_plain_scale_1280x720_properties(gpu):
                         s0, e0 = got.get(d * m + j, (0, 0))
                         per_item[(tuple(keys[j]), d)] = tris[order[s0:e0]]
             assert set(got) <= {d * m + j for j in range(m) for d in range(6) if dirs[j] & (1 << d)}
+            # the other volume through the fused device-output pass: same items, same triangles per item
+            kb_, db_, tb_, ib_ = _device_incremental(b, dev_tris, dev_items)
+            assert dict(zip(map(tuple, kb_), db_)) == dict(zip(map(tuple, keys), dirs))
+            ga, gb = _items(keys, tris, items), _items(kb_, tb_, ib_)
+            assert ga.keys() == gb.keys() and len(tb_) == len(tris)
+            assert all(np.array_equal(ga[k].view(np.uint32), gb[k].view(np.uint32)) for k in ga)
             if i in (19, 39):
                 full = a.extract_mesh()
                 union = np.concatenate([t for t in per_item.values() if len(t)])
