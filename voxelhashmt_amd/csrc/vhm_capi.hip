@@ -565,8 +565,9 @@ int vhm_allocate_active_blocks(vhm_ctx* c, const float* depth, int memkind, cons
 }
 
 // Records per point-cloud chunk: keys, values and their sort temporaries
-// take 24 B each (400 MB at this bound).
-constexpr size_t kPointCloudMaxRecords = size_t(1) << 24;
+// take 24 B each (800 MB at this bound, which holds a 1280x720 frame's
+// cloud at 19 samples per ray in one chunk).
+constexpr size_t kPointCloudMaxRecords = size_t(1) << 25;
 
 int vhm_integrate_point_cloud(vhm_ctx* c, const float* points, size_t n, int memkind, const vhm_pose* T_wc) {
   if (!c) return VHM_ERR_INVALID_ARGUMENT;
