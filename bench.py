@@ -274,7 +274,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--mesh-every", type=int, default=10, help="incremental MC period in frames (0 = off)")
-    ap.add_argument("--mesh-tri-capacity", type=int, default=24_000_000,
+    ap.add_argument("--mesh-tri-capacity", type=int, default=64_000_000,
                     help="device triangle buffer of the incremental calls (a call that overflows it is counted)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration under rocprofv3)")
@@ -331,7 +331,7 @@ def main():
     tri_buf = torch.empty(args.mesh_tri_capacity * 9, dtype=torch.float32, device=dev) if mesh_every else None
     if mesh_every:
         vol.extract_mesh_incremental(device_tris=tri_buf)  # consume the warm-up's dirty blocks
-    inc_blocks = inc_tris = inc_calls = inc_overflows = 0
+    inc_blocks = inc_tris = inc_calls = inc_overflows = inc_max = 0
     vol.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -343,6 +343,7 @@ def main():
             inc_tris += nt
             inc_calls += 1
             inc_overflows += int(nt > args.mesh_tri_capacity)
+            inc_max = max(inc_max, nt)
     vol.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -416,7 +417,8 @@ def main():
         "kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items()},
         "incremental_mesh": {"every_frames": args.mesh_every, "calls": inc_calls, "blocks_remeshed": inc_blocks,
                              "triangles": inc_tris, "ms": round(kt["mesh"]["ms"], 3),
-                             "tri_capacity": args.mesh_tri_capacity, "capacity_overflows": inc_overflows},
+                             "tri_capacity": args.mesh_tri_capacity, "max_triangles_per_call": inc_max,
+                             "capacity_overflows": inc_overflows},
         "roofline": roofline,
         "cpu_baseline": cpu,
         **mesh,
