@@ -232,6 +232,8 @@ int vhm_batch_set_sequence(vhm_batch* b, int scene, const uint16_t* frames_devic
   B_TRY(b, hipMemset(sc.remesh_flags, 0, 4 * mb), "hipMemset remesh flags");
   B_TRY(b, bmalloc(b, &sc.remesh_list, 4 * mb), "hipMalloc remesh list");
   B_TRY(b, bmalloc(b, &sc.remesh_dirs, mb), "hipMalloc remesh directions");
+  B_TRY(b, bmalloc(b, &sc.mesh_nbr, 4 * 8 * mb), "hipMalloc mesh sources");
+  B_TRY(b, bmalloc(b, &sc.mesh_obs_dirs, mb), "hipMalloc mesh source directions");
   sc.tri_capacity = static_cast<uint32_t>(tri_capacity);
   B_TRY(b, bmalloc(b, &sc.tris, 36 * (tri_capacity ? tri_capacity : 1)), "hipMalloc triangles");
   B_TRY(b, bmalloc(b, &sc.out, 16), "hipMalloc mesh outputs");

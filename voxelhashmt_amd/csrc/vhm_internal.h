@@ -197,6 +197,8 @@ struct BatchScene {
   uint32_t* remesh_flags;     // [max_blocks]
   uint32_t* remesh_list;      // [max_blocks]
   uint8_t* remesh_dirs;       // [max_blocks] directions to re-mesh per listed block
+  uint32_t* mesh_nbr;         // [max_blocks][8] source blocks per listed block
+  uint8_t* mesh_obs_dirs;     // [max_blocks] observed directions over those sources
   float* tris;                // [tri_capacity][9]
   uint32_t tri_capacity;
   uint32_t* out;              // [4]: blocks re-meshed, triangles (total, may exceed capacity)

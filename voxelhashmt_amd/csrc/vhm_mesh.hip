@@ -751,17 +751,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   }
 }
 
-// For listed block i: its 8 source blocks (self and the 7 positive
-// neighbours, kInvalid when absent) at nbr[8 i ..], and the OR of their
-// observed-direction masks at observed_dirs[i].  Eight lanes per block.
-__global__ __launch_bounds__(256) void meshNeighboursKernel(VolumeView V, const uint32_t* __restrict__ list,
-                                                            uint32_t n, uint32_t* __restrict__ nbr,
-                                                            uint8_t* __restrict__ observed_dirs) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t i = g >> 3;
-  const int k = static_cast<int>(g & 7u);
+// Source block k of listed block i (self and the 7 positive neighbours,
+// kInvalid when absent) into nbr[8 i + k], and the OR of the 8 sources'
+// observed-direction masks into observed_dirs[i].  Called by 8 consecutive
+// lanes of a wavefront per block (all lanes of the wavefront take part).
+__device__ __forceinline__ void neighbourLane(const VolumeView& V, const uint32_t* __restrict__ list, bool active,
+                                              uint32_t i, int k, uint32_t* __restrict__ nbr,
+                                              uint8_t* __restrict__ observed_dirs) {
   uint32_t dm = 0;
-  if (i < n) {
+  if (active) {
     const uint32_t self = list[i];
     const uint64_t key = V.pool_keys[self];
     const uint32_t p = k == 0 ? self : lookupPool(V, keyX(key) + (k & 1), keyY(key) + ((k >> 1) & 1), keyZ(key) + (k >> 2));
@@ -773,7 +771,15 @@ __global__ __launch_bounds__(256) void meshNeighboursKernel(VolumeView V, const 
   dm |= __shfl_xor(dm, 1);
   dm |= __shfl_xor(dm, 2);
   dm |= __shfl_xor(dm, 4);
-  if (i < n && k == 0) observed_dirs[i] = static_cast<uint8_t>(dm);
+  if (active && k == 0) observed_dirs[i] = static_cast<uint8_t>(dm);
+}
+
+// The sources of listed blocks 0..n-1: eight lanes per block.
+__global__ __launch_bounds__(256) void meshNeighboursKernel(VolumeView V, const uint32_t* __restrict__ list,
+                                                            uint32_t n, uint32_t* __restrict__ nbr,
+                                                            uint8_t* __restrict__ observed_dirs) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  neighbourLane(V, list, (g >> 3) < n, g >> 3, static_cast<int>(g & 7u), nbr, observed_dirs);
 }
 
 // Persistent over the 6 n items: item dir * n + i meshes block list[i] (or
@@ -1025,6 +1031,23 @@ __global__ void meshPrefixKernel(const BatchScene* __restrict__ scenes, int S, u
   prefix[S] = acc;
 }
 
+// The sources of every scene's listed blocks: a grid-stride loop (the grid
+// is fixed for the captured step) over 8 lanes per block of the
+// concatenation (block b of the concatenation is item 6 b of prefix's).
+__global__ __launch_bounds__(256) void meshNeighboursBatchKernel(const BatchScene* __restrict__ scenes, int S,
+                                                                 const uint32_t* __restrict__ prefix) {
+  const uint32_t lanes = 8u * (prefix[S] / 6u);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < lanes; base += gridDim.x * blockDim.x) {
+    const uint32_t g = base + threadIdx.x;  // the whole wavefront stays in the loop together
+    const bool active = g < lanes;
+    const uint32_t bg = (active ? g : lanes - 1u) >> 3;
+    const int s = findSegment(prefix, S, 6u * bg);
+    const BatchScene& sc = scenes[s];
+    neighbourLane(sc.V, sc.remesh_list, active, bg - prefix[s] / 6u, static_cast<int>(g & 7u), sc.mesh_nbr,
+                  sc.mesh_obs_dirs);
+  }
+}
+
 // Fused count + emit over the concatenated items of every scene: each item's
 // triangles go to one range of its scene's buffer, taken with an atomic add
 // on the scene's total (items in completion order).
@@ -1039,9 +1062,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const BatchScene& sc = scenes[s];
     const uint32_t m = sc.out[0], j = i - prefix[s];
     const uint32_t dir = j / m, idx = j % m;
-    if (sc.remesh_dirs[idx] & (1u << dir))  // workgroup-uniform
+    if (sc.remesh_dirs[idx] & sc.mesh_obs_dirs[idx] & (1u << dir))  // workgroup-uniform
       fusedItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, dir * m + idx, sc.out + 1, sc.tris,
-                nullptr, sc.tri_capacity);
+                nullptr, sc.tri_capacity, sc.mesh_nbr + 8ull * idx);
   }
 }
 
@@ -1122,6 +1145,7 @@ hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, in
   markRemeshBatchKernel<<<dim3(64, S), 256, 0, s>>>(scenes);
   compactRemeshBatchKernel<<<S, 256, 0, s>>>(scenes);
   meshPrefixKernel<<<1, 64, 0, s>>>(scenes, S, prefix);
+  meshNeighboursBatchKernel<<<1024, 256, 0, s>>>(scenes, S, prefix);
   meshFusedBatchKernel<<<g, 256, 0, s>>>(scenes, S, prefix, cand);
   return hipGetLastError();
 }
