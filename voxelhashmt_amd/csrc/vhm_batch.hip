@@ -196,6 +196,7 @@ int vhm_batch_set_sequence(vhm_batch* b, int scene, const uint16_t* frames_devic
   sc.ia.merge_tol = sc.ia.merge_tol < 1e-6f ? 1e-6f : sc.ia.merge_tol;
   sc.ia.min_cos = P.layer_normal_cos;
   sc.ia.max_weight = P.max_weight;
+  sc.ia.near_base = meshNearBase(P.voxel_size, P.layer_extract_scale);
   sc.ia.min_view = P.min_view_angle_weight;
   sc.ia.disc_base = P.disc_base;
   sc.ia.disc_rel = P.disc_rel;

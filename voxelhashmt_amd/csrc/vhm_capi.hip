@@ -268,6 +268,7 @@ int integrateStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, 
   ia.merge_tol = ia.merge_tol < 1e-6f ? 1e-6f : ia.merge_tol;  // std::max(1e-6f, .)
   ia.min_cos = c->P.layer_normal_cos;
   ia.max_weight = c->P.max_weight;
+  ia.near_base = meshNearBase(c->P.voxel_size, c->P.layer_extract_scale);
   ia.min_view = c->P.min_view_angle_weight;
   ia.disc_base = c->P.disc_base;
   ia.disc_rel = c->P.disc_rel;

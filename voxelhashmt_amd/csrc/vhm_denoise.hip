@@ -677,7 +677,12 @@ __global__ __launch_bounds__(256) void dnApplyKernel(VolumeView V, DnApplyArgs a
     }
   }
   if (replaced) atomicAdd(&a.counters[0], replaced);
-  if (threadIdx.x == 0) V.dirty[p] = (1u << kDirs) - 1u;  // plane offsets of any direction may have changed
+  if (threadIdx.x == 0) {
+    V.dirty[p] = (1u << kDirs) - 1u;  // plane offsets of any direction may have changed
+    uint32_t signs = 0;
+    for (int d = 0; d < kDirs; ++d) signs |= signBitsBoth(d);
+    atomicOr(&V.dirmask[p], signs);
+  }
 }
 
 // 16x16x16 bf16 MFMA layout probe: A[i][k] = i + 16 k / 256, B = identity;

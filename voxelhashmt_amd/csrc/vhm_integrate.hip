@@ -118,7 +118,10 @@ __device__ __forceinline__ void integrateQueued(const Queue& Q, int i, const Dep
     L.po[k] = __uint_as_float(old[k].z);
     L.pn[k] = old[k].w;
   }
-  if (!integrateSample(L, o.coord, o.pw, o.nw, o.weight, a.merge_tol, a.min_cos, a.max_weight)) return;
+  F3 n_new;
+  float po_new;
+  if (!integrateSample(L, o.coord, o.pw, o.nw, o.weight, a.merge_tol, a.min_cos, a.max_weight, &n_new, &po_new))
+    return;
   // write back only the slots that changed (usually the one updated slot)
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -133,7 +136,20 @@ __device__ __forceinline__ void integrateQueued(const Queue& Q, int i, const Dep
     ++t.new_obs;
   }
   if (!(dirty_word & (1u << d))) atomicOr(&V.dirty[p], 1u << d);
-  if (!(dmask & (1u << d))) atomicOr(&V.dirmask[p], 1u << d);
+  // The written slot's plane distance at the voxel centre, from the normal
+  // before packing and a centre re-formed from the camera-space point: within
+  // (|c|_1 + 1) 2^-12 of the distance marching cubes forms from the packed
+  // normal (the 16-bit octahedral code moves a unit normal by < 1e-4).
+  const F3 c = transformPoint(a.T_wc, pc);
+  const float c1 = fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+  const float slack = (c1 + 1.0f) * 0x1.0p-12f;
+  const float vp = dot3(n_new, c) - po_new;
+  uint32_t want = 1u << d;
+  if (!(fabsf(vp) > a.near_base + (c1 + 4.0f * a.voxel) * 0x1.0p-18f + slack)) {  // NaN sets both, as in staging
+    if (!(vp >= slack)) want |= 1u << (kSignBelow + d);
+    if (!(vp <= -slack)) want |= 1u << (kSignAbove + d);
+  }
+  if (want & ~dmask) atomicOr(&V.dirmask[p], want);
 }
 
 // Runs the expensive part on queue entries [Q.n - m, Q.n) with m = min(Q.n, 256)

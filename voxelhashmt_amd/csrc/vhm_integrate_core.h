@@ -337,8 +337,10 @@ VHM_HD void swapSlot(Slots& L, int i, int j) {
 
 // integrateDirectionalSample (voxel_hash_tsdf.h:479-529) on one direction's
 // three slots.  Returns true when a slot was written.
+// n_out / po_out (optional): the written slot's unit normal before packing
+// and its plane offset.
 VHM_HD bool integrateSample(Slots& L, float coord, F3 pw, F3 nw, float w, float merge_tol, float min_cos,
-                            float max_weight) {
+                            float max_weight, F3* n_out = nullptr, float* po_out = nullptr) {
   if (w <= 0.0f) return false;
   const F3 n = normalized3(nw);
   if (normAtMost1em6(n)) return false;
@@ -380,6 +382,8 @@ VHM_HD bool integrateSample(Slots& L, float coord, F3 pw, F3 nw, float w, float 
     if (normAtMost1em6(blended)) blended = n;
   }
   const uint32_t pn1 = packNormalOct(blended);
+  if (n_out) *n_out = blended;
+  if (po_out) *po_out = po1;
 #pragma unroll
   for (int j = 0; j < 3; ++j)
     if (j == k) {

@@ -63,7 +63,8 @@ struct VolumeView {
   uint32_t* pool;  // float fields are stored as their bit patterns
   uint32_t* obs;
   uint32_t* dirty;
-  uint32_t* dirmask;  // per block: bit d set once any voxel has an observed slot in direction d
+  uint32_t* dirmask;  // per block: bit d set once any voxel has an observed slot in direction d;
+                      // bits 8 + d / 16 + d: kSignBelow / kSignAbove (below)
   uint32_t max_blocks;
   uint32_t* active;
   DevHeader* hdr;
@@ -100,6 +101,24 @@ struct AllocArgs {
   uint32_t frame;
 };
 
+// Sticky per-block hints for marching cubes at iso 0, in dirmask: bit
+// kSignBelow + d (kSignAbove + d) is set once a slot of direction d was
+// written whose plane distance at its voxel's centre may lie within the
+// staging's near window of 0 and below 0 (at or above 0).  An item none of
+// whose 8 source blocks has both bits of its direction can have no cube
+// with corners on both sides, so no triangle.  Writers that do not compute
+// the distance set both bits.
+constexpr uint32_t kSignBelow = 8u;
+constexpr uint32_t kSignAbove = 16u;
+VHM_HD uint32_t signBitsBoth(int d) { return (1u << (kSignBelow + d)) | (1u << (kSignAbove + d)); }
+// IntegrateArgs::near_base for a volume's voxel size and layer_extract_scale
+// (the extraction tolerance as marching cubes forms it).
+inline float meshNearBase(float voxel, float extract_scale) {
+  float tol = voxel * extract_scale;
+  tol = tol < 1e-6f ? 1e-6f : tol;
+  return 2.0f * tol + 1.875f * voxel;
+}
+
 struct IntegrateArgs {
   Intrinsics K;
   Pose34 T_wc, T_cw;
@@ -110,6 +129,7 @@ struct IntegrateArgs {
   float inv_r, inv_a;
   float inv_fx, inv_fy;  // RN(1 / fx), RN(1 / fy)
   int exact_fx, exact_fy;  // divByConst by fx / fy is exact (host-checked)
+  float near_base;  // marching cubes' near window without its rounding term: 2 extract_tol + 1.875 voxel
 };
 
 // RN(1 / f) and whether divByConst by f equals the division for every

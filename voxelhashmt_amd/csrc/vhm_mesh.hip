@@ -755,9 +755,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // kInvalid when absent) into nbr[8 i + k], and the OR of the 8 sources'
 // observed-direction masks into observed_dirs[i].  Called by 8 consecutive
 // lanes of a wavefront per block (all lanes of the wavefront take part).
+//
+// signs (iso 0 only): keep a direction only if the sources also hold both
+// near-iso sign hints for it (kSignBelow / kSignAbove).
 __device__ __forceinline__ void neighbourLane(const VolumeView& V, const uint32_t* __restrict__ list, bool active,
                                               uint32_t i, int k, uint32_t* __restrict__ nbr,
-                                              uint8_t* __restrict__ observed_dirs) {
+                                              uint8_t* __restrict__ observed_dirs, bool signs) {
   uint32_t dm = 0;
   if (active) {
     const uint32_t self = list[i];
@@ -771,15 +774,16 @@ __device__ __forceinline__ void neighbourLane(const VolumeView& V, const uint32_
   dm |= __shfl_xor(dm, 1);
   dm |= __shfl_xor(dm, 2);
   dm |= __shfl_xor(dm, 4);
-  if (active && k == 0) observed_dirs[i] = static_cast<uint8_t>(dm);
+  if (signs) dm &= (dm >> kSignBelow) & (dm >> kSignAbove);
+  if (active && k == 0) observed_dirs[i] = static_cast<uint8_t>(dm & 0x3Fu);
 }
 
 // The sources of listed blocks 0..n-1: eight lanes per block.
 __global__ __launch_bounds__(256) void meshNeighboursKernel(VolumeView V, const uint32_t* __restrict__ list,
                                                             uint32_t n, uint32_t* __restrict__ nbr,
-                                                            uint8_t* __restrict__ observed_dirs) {
+                                                            uint8_t* __restrict__ observed_dirs, bool signs) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  neighbourLane(V, list, (g >> 3) < n, g >> 3, static_cast<int>(g & 7u), nbr, observed_dirs);
+  neighbourLane(V, list, (g >> 3) < n, g >> 3, static_cast<int>(g & 7u), nbr, observed_dirs, signs);
 }
 
 // Persistent over the 6 n items: item dir * n + i meshes block list[i] (or
@@ -1044,7 +1048,7 @@ __global__ __launch_bounds__(256) void meshNeighboursBatchKernel(const BatchScen
     const int s = findSegment(prefix, S, 6u * bg);
     const BatchScene& sc = scenes[s];
     neighbourLane(sc.V, sc.remesh_list, active, bg - prefix[s] / 6u, static_cast<int>(g & 7u), sc.mesh_nbr,
-                  sc.mesh_obs_dirs);
+                  sc.mesh_obs_dirs, sc.ma.iso == 0.0f);
   }
 }
 
@@ -1076,7 +1080,7 @@ hipError_t launchMeshFused(const VolumeView& V, const uint32_t* list, uint32_t n
                            const uint8_t* dirs, uint32_t* nbr, uint8_t* observed_dirs, uint32_t* total, float* tris,
                            uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  meshNeighboursKernel<<<(8ull * n + 255) / 256, 256, 0, s>>>(V, list, n, nbr, observed_dirs);
+  meshNeighboursKernel<<<(8ull * n + 255) / 256, 256, 0, s>>>(V, list, n, nbr, observed_dirs, a.iso == 0.0f);
   meshFusedKernel<<<meshGrid(kDirs * n), 256, 0, s>>>(V, list, n, a, dirs, nbr, observed_dirs, total, tris, tri_owner,
                                                        capacity, cand);
   return hipGetLastError();

@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void pcApplyKernel(const uint64_t* __restrict_
           // returned word decides which lane counts the new observation
           if (!(atomicOr(&V.obs[static_cast<size_t>(p) * 16u + (vi >> 5)], bit) & bit)) ++new_obs;
           atomicOr(&V.dirty[p], 1u << d);
-          atomicOr(&V.dirmask[p], 1u << d);
+          atomicOr(&V.dirmask[p], (1u << d) | signBitsBoth(d));  // distances not computed here
         }
       }
     }
