@@ -114,6 +114,20 @@ uint64_t vcheck_integrate_blocks(void* blocks_aos, const int32_t* keys, std::siz
   return updates;
 }
 
+// Largest |unpackNormalOct(packNormalOct(n)) - n| (Euclidean) over the given
+// normals (each normalised first, as integrateSample's blended normal is):
+// bounds the error of the integrator's near-iso sign hints.
+float vic_oct_roundtrip_max_err(const float* n3, size_t count) {
+  float worst = 0.0f;
+  for (size_t i = 0; i < count; ++i) {
+    const F3 n = normalized3(F3{n3[3 * i], n3[3 * i + 1], n3[3 * i + 2]});
+    const F3 m = unpackNormalOct(packNormalOct(n));
+    const float e = norm3(sub3(m, n));
+    if (e > worst) worst = e;
+  }
+  return worst;
+}
+
 // The slot search's normal test: for each (packed, n, min_cos), the filtered
 // decision octNormalCosBelowWith (with an rsqrt moved `rsqrt_ulps` ulp off
 // the host's 1/sqrtf, as the GPU's may be) and the reference computation's

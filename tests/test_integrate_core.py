@@ -176,3 +176,23 @@ def test_slot_normal_filter_equals_reference_decision(chk):
         bad = np.flatnonzero(fast != exact)
         assert len(bad) == 0, (ulps, bad[:5], pk[bad[:5]], mc[bad[:5]])
     assert len(np.unique(exact)) == 2
+
+
+def test_octahedral_roundtrip_error_bound(chk):
+    """The integrator's near-iso sign hints form plane distances from the
+    normal before packing and allow (|c|_1 + 1) 2^-12 for the 16-bit
+    octahedral code; that rests on the code moving a unit normal by less
+    than 1e-4.  Random normals, and normals on and near the axes and the
+    octahedron's fold (|x| + |y| = 1 at z = 0) where the mapping bends."""
+    f = chk.vic_oct_roundtrip_max_err
+    f.argtypes = [C.c_void_p, C.c_size_t]
+    f.restype = C.c_float
+    rng = np.random.default_rng(11)
+    n = rng.normal(size=(200000, 3))
+    axes = np.eye(3)[rng.integers(0, 3, 20000)] * rng.choice([-1.0, 1.0], (20000, 1))
+    axes += rng.normal(scale=1e-3, size=axes.shape)
+    t = rng.uniform(0, 2 * np.pi, 20000)
+    fold = np.stack([np.cos(t), np.sin(t), rng.normal(scale=1e-4, size=t.shape)], 1)
+    pts = np.ascontiguousarray(np.concatenate([n, axes, fold]).astype(np.float32))
+    err = f(pts.ctypes.data, len(pts))
+    assert 0.0 < err < 1e-4, err
