@@ -11,7 +11,9 @@ order).  Within one output each item's triangles must be one contiguous run;
 the host output has its items in item order, the device output in the order
 the items finished.  With a device buffer smaller
 than the output, only whole cubes' triangles are written, nothing past the
-capacity, and every written triangle is its item's.
+capacity, and every written triangle is its item's.  After point-cloud integration
+(whose blocks carry both near-iso sign hints, so the fused pass's item
+filter keeps them) and frames on top, the same per-item equality holds.
 """
 import ctypes as C
 
@@ -114,5 +116,39 @@ def test_fused_device_output_past_capacity(gpu):
     part = _per_item(ka, got[written], got_items[written].astype(np.uint32))
     for k, t in part.items():  # each written item's triangles are a prefix of its full list
         assert np.array_equal(t.view(np.uint32), full[k][: len(t)].view(np.uint32)), k
+    a.close()
+    b.close()
+
+
+def test_fused_after_point_cloud_and_frames(gpu):
+    """Blocks written by point-cloud integration carry both near-iso sign
+    hints (the fused pass's item filter must never drop them), then frames
+    update them through the integrator: the fused device output still equals
+    the two-pass host output per item."""
+    import torch
+    import test_point_cloud as tpc
+    w = synth.tiny(n_frames=3, seed=9, width=96, height=72)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    a, b = _volumes(w)
+    cloud = tpc.depth_cloud(frames[0].astype(np.float32) * np.float32(0.001), w.K)
+    dev_tris = torch.empty(1_000_000 * 9, dtype=torch.float32, device="cuda:0")
+    dev_items = torch.empty(1_000_000, dtype=torch.int32, device="cuda:0")
+    for v in (a, b):
+        v.integrate_point_cloud(cloud, w.poses[0])
+    for step in range(2):
+        if step == 1:
+            for v in (a, b):
+                v.integrate(frames[1], w.K, w.poses[1])
+                v.integrate(frames[2], w.K, w.poses[2])
+        ka, da, m, n = _device_call(a, dev_tris, dev_items)
+        kb, db, tris, items = b.extract_mesh_incremental()
+        print(f"[fused] point cloud step {step}: {m} blocks, {n} triangles", flush=True)
+        assert (m, n) == (len(kb), len(tris)) and n > 0
+        got = dev_tris[: 9 * n].cpu().numpy().reshape(-1, 3, 3)
+        ga = _per_item(ka, got, dev_items[:n].cpu().numpy().view(np.uint32))
+        gb = _per_item(kb, tris, items, in_order=True)
+        assert ga.keys() == gb.keys()
+        for k in ga:
+            assert np.array_equal(ga[k].view(np.uint32), gb[k].view(np.uint32)), k
     a.close()
     b.close()
