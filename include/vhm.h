@@ -155,7 +155,10 @@ int vhm_observed_voxel_count(vhm_ctx* ctx, size_t* out);
 int vhm_stats(vhm_ctx* ctx, uint64_t* updates, uint64_t* visited, uint64_t* frames);
 
 /* gpuVolumeCacheBlockKeys (cuda_tsdf_integrator.cu:1054-1057): all
- * allocated keys in pool order. */
+ * allocated keys in pool order.  Pool order is creation order: frame by
+ * frame (point-cloud call by call), and within one frame in increasing
+ * packed key (x + 2^20) | (y + 2^20) << 21 | (z + 2^20) << 42, so it depends
+ * only on the inputs, not on thread scheduling. */
 int vhm_block_keys(vhm_ctx* ctx, int32_t* keys_out, size_t capacity, size_t* n_out);
 
 /* Keys of the blocks holding at least one observed voxel (the blocks
@@ -170,9 +173,18 @@ int vhm_observed_block_keys(vhm_ctx* ctx, int32_t* keys_out, size_t capacity, si
  * returns per record.  Unallocated keys yield zeros.  Synchronous. */
 int vhm_download_blocks(vhm_ctx* ctx, const int32_t* keys, size_t n, void* out_host);
 
+/* Per key, a digest of the bytes vhm_download_blocks would return for it:
+ * sum_i w_i * 0x100000001B3^i (mod 2^64) over the block's 36,864 32-bit
+ * words w_i in that layout (0 for an unallocated key), computed on the
+ * device.  No reference counterpart: it compares whole volumes (parity
+ * tests, batched vs one-volume runs) without copying pools to the host.
+ * Synchronous. */
+int vhm_block_digests(vhm_ctx* ctx, const int32_t* keys, size_t n, uint64_t* out_host);
+
 /* extractMesh / extractMeshCuda (reconstruction_backend.cpp:279-291,
  * cuda_mesh_extractor.cu:636-720).  Two passes on the GPU (count, scan,
- * emit) in a deterministic order: direction, block pool order, cube, cluster.
+ * emit) in a deterministic order: direction, block pool order (see
+ * vhm_block_keys), cube, cluster; the same inputs give the same bytes.
  * Writes min(capacity, n) triangles; *n_out = n.  tris_out == NULL with
  * capacity 0 only counts.  Synchronous. */
 int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capacity, size_t* n_out);
@@ -193,8 +205,11 @@ int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capa
  * items in item order.  VHM_DEVICE: one pass writes in place on the GPU,
  * nothing copied to the host; each item's triangles are one contiguous run
  * in the same order, the runs in the order the items finished (as the
- * reference's atomic appends), and a cube whose triangles would cross
- * capacity is not written.  Synchronous. */
+ * reference's atomic appends).  *n_tris_out > tri_capacity reports an
+ * overflow: an item whose run would cross capacity writes no triangle, its
+ * entries below capacity get item 0xFFFFFFFF and NaN coordinates, and it is
+ * marked dirty again, so the next incremental call lists it; every entry
+ * with a valid item holds that item's triangle.  Synchronous. */
 int vhm_extract_mesh_incremental(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out, uint8_t* block_dirs_out,
                                  size_t block_capacity, size_t* n_blocks_out, float* tris_out,
                                  uint32_t* tri_item_out, size_t tri_capacity, int out_memkind, size_t* n_tris_out);
@@ -281,9 +296,20 @@ int vhm_batch_steps_done(const vhm_batch* b);
  * (reservations, captured graph and grid sizes are kept). */
 int vhm_batch_rewind(vhm_batch* b);
 /* Last step's incremental mesh of a scene: blocks re-meshed, triangle total
- * (may exceed tri_capacity), and up to capacity triangles (9 floats, host). */
+ * (may exceed tri_capacity), and up to capacity triangles (9 floats, host).
+ * Only the last step's output is kept: each step overwrites it. */
 int vhm_batch_mesh_output(vhm_batch* b, int scene, uint32_t* blocks, uint32_t* triangles, float* tris_host,
                           size_t capacity);
+/* The items of that output, as vhm_extract_mesh_incremental returns them:
+ * the listed blocks (keys in pool order, per block the bit mask of
+ * directions re-meshed; min(block_capacity, n) written, *n_blocks_out = n)
+ * and per triangle its item d * n + i (min(tri_capacity, triangles)
+ * written).  Items land in completion order, each one contiguous run; an
+ * item that did not fit tri_capacity has item 0xFFFFFFFF and is listed
+ * again by the next step.  A caller keeping one mesh per (block, direction)
+ * replaces exactly the items in the masks after every step. */
+int vhm_batch_mesh_items(vhm_batch* b, int scene, int32_t* block_keys_out, uint8_t* block_dirs_out,
+                         size_t block_capacity, size_t* n_blocks_out, uint32_t* tri_item_out, size_t tri_capacity);
 /* One step with direct launches, timed per stage: ms_out[0] filter +
  * allocation, [1] integration, [2] incremental meshing; the step's voxel
  * updates and re-meshed blocks over all scenes.  Synchronous. */

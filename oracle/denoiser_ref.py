@@ -182,6 +182,87 @@ def block_features(blocks: dict, key, voxel: float, trunc: float, max_w: float =
     return out
 
 
+def bf16_to_float(bits: np.ndarray) -> np.ndarray:
+    """bf16 bit patterns (uint16) -> float32."""
+    return (np.asarray(bits, np.uint32) << 16).view(np.float32)
+
+
+def _dot3(n, c):
+    """(n.x c.x + n.y c.y) + n.z c.z, each operation rounded to float32."""
+    f = np.float32
+    return f(f(f(n[..., 0] * c[..., 0]) + f(n[..., 1] * c[..., 1])) + f(n[..., 2] * c[..., 2]))
+
+
+def apply_blocks(blocks: np.ndarray, keys: np.ndarray, Y: np.ndarray, X: np.ndarray, voxel: float, trunc: float,
+                 max_w: float, min_w: float, max_delta: float, block_max_delta: float):
+    """Write-back of denoised tsdf with the per-block confidence fallback, in
+    float32 in the kernel's operation order (dnApplyKernel).
+
+    blocks: (n, 512, 6, 3) Voxel records of the blocks (AoS, as
+    vhm_download_blocks returns them); Y: (n, 512, 6) predictions; X:
+    (n, 10, 10, 10, 16) the inputs as bf16 bits.  Per block, the mean of
+    |y - x| over the cells (voxel, direction) whose input weight channel is
+    > 0 is formed as 256 partial sums (partial t takes voxels t and t + 256,
+    directions in order) added pairwise in a tree; a block with no such cell
+    or a mean above block_max_delta keeps plain fusion.  Otherwise every cell
+    whose heaviest slot (first on ties) has weight / max_w >= min_w and
+    |y - x| <= max_delta gets that slot's plane_offset = n . centre -
+    clamp(y, -1, 1) * trunc, so that the plane's signed distance at the voxel
+    centre is the prediction in truncation units (planeSignedDistance,
+    voxel_hash_tsdf.h:336-339).  Returns (new blocks, cells replaced,
+    blocks kept plain)."""
+    f = np.float32
+    out = blocks.copy()
+    xin = bf16_to_float(X).reshape(len(keys), 10, 10, 10, 16)[:, 1:9, 1:9, 1:9].reshape(len(keys), 512, 16)
+    inv_w = f(1.0) / f(max_w)
+    vi = np.arange(512)
+    lx, ly, lz = vi & 7, (vi >> 3) & 7, vi >> 6
+    replaced = kept = 0
+    for b, key in enumerate(np.asarray(keys, np.int64)):
+        y = np.asarray(Y[b], f)
+        x = xin[b, :, :6]
+        delta = np.abs(f(1) * (y - x)).astype(f)
+        valid = xin[b, :, 6:12] > 0
+        part = np.zeros(256, f)
+        cnt = np.zeros(256, np.int64)
+        for half in (0, 256):
+            for d in range(6):
+                m = valid[half:half + 256, d]
+                part = np.where(m, (part + delta[half:half + 256, d]).astype(f), part)
+                cnt += m
+        off = 128
+        while off > 0:
+            part[:off] = (part[:off] + part[off:2 * off]).astype(f)
+            cnt[:off] += cnt[off:2 * off]
+            off >>= 1
+        if cnt[0] == 0 or f(part[0] / f(cnt[0])) > f(block_max_delta):
+            kept += 1
+            continue
+        blk = out[b]
+        centre = np.stack([(f(1) * (key[0] * 8 + lx).astype(f) + f(0.5)) * f(voxel),
+                           (f(1) * (key[1] * 8 + ly).astype(f) + f(0.5)) * f(voxel),
+                           (f(1) * (key[2] * 8 + lz).astype(f) + f(0.5)) * f(voxel)], -1).astype(f)
+        for d in range(6):
+            w = blk["weight"][:, d, :]
+            bw = np.zeros(512, f)
+            bk = np.full(512, -1)
+            for k in range(3):  # heaviest slot, the first on ties
+                better = w[:, k] > bw
+                bw = np.where(better, w[:, k], bw)
+                bk = np.where(better, k, bk)
+            take = (bk >= 0) & ~((bw * inv_w).astype(f) < f(min_w)) & ~(delta[:, d] > f(max_delta))
+            if not take.any():
+                continue
+            idx = np.flatnonzero(take)
+            slot = bk[idx]
+            n = _unpack(blk["packed_normal"][idx, d, slot])
+            yc = np.clip(y[idx, d], f(-1), f(1)).astype(f)
+            po = (_dot3(n, centre[idx]) - (yc * f(trunc)).astype(f)).astype(f)
+            blk["plane_offset"][idx, d, slot] = po
+            replaced += len(idx)
+    return out, replaced, kept
+
+
 def params_to_flat(p: dict) -> np.ndarray:
     return np.concatenate([p[k].reshape(-1) for k in ORDER]).astype(np.float64)
 

@@ -2,18 +2,22 @@
 
 Every batched step (a replay of the captured hipGraph, or direct launches)
 must leave each scene exactly where vhm_integrate_u16 + the incremental mesh
-leave a separate volume fed the same frames: the same re-meshed block count
-and triangle multiset after every step, then the same block keys and
-bit-identical voxel layers.  The one-volume path is itself bit-exact against
-the oracle (test_gpu_parity.py), so this pins the batched path to the oracle.
+leave a separate volume fed the same frames: the same listed (block,
+direction) items with the same triangles per item after every step, then
+the same block keys (in pool order) and bit-identical voxel layers.  The
+one-volume path is itself bit-exact against the oracle (test_gpu_parity.py,
+test_gpu_config_golden.py), so this pins the batched path to the oracle.
 Scenes differ in resolution and sequence length; steps continue past the
-shortest sequence.
+shortest sequence.  The batched workload itself (48 rooms of 1k-60k
+blocks at 320x240 to 1280x720) runs its first steps the same way, scene
+volumes compared through device digests of every block.
 """
 import numpy as np
 import pytest
 
 import oracle_lib as O
 import voxelhashmt_amd as vhm
+from meshitems import item_digest, per_item
 from voxelhashmt_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -66,15 +70,21 @@ def _run(torch, use_graph, autotune):
             for i, (w, frames) in enumerate(wl):
                 if step < len(frames):
                     singles[i].integrate(frames[step], w.K, w.poses[step])
-                keys, _, tris, _ = singles[i].extract_mesh_incremental()
+                keys, dirs, tris, items = singles[i].extract_mesh_incremental()
                 nb, nt, btris = b.mesh_output(i)
                 assert nb == len(keys), f"scene {i} step {step}: re-meshed blocks {nb} vs {len(keys)}"
                 assert nt == len(tris), f"scene {i} step {step}: triangles {nt} vs {len(tris)}"
-                assert np.array_equal(O.canonical_mesh(btris), O.canonical_mesh(tris))
+                bkeys, bdirs, bitems = b.mesh_items(i)
+                assert np.array_equal(bkeys, keys) and np.array_equal(bdirs, dirs)
+                got, want = per_item(bkeys, btris, bitems), per_item(keys, tris, items, in_order=True)
+                assert got.keys() == want.keys(), f"scene {i} step {step}: items differ"
+                for k in want:
+                    assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), (i, step, k)
             step += 1
         assert b.steps_done == steps
         for i, (w, frames) in enumerate(wl):
             bv, sv = b.volumes[i], singles[i]
+            assert np.array_equal(bv.block_keys(), sv.block_keys()), f"scene {i}: pool order differs"
             kb, ks = O.sorted_keys(bv.block_keys()), O.sorted_keys(sv.block_keys())
             assert np.array_equal(kb, ks), f"scene {i}: block keys differ"
             assert np.array_equal(_blocks(bv, kb).view(np.uint32), _blocks(sv, ks).view(np.uint32))
@@ -101,3 +111,53 @@ def test_batch_equals_single_volumes(gpu, use_graph):
 def test_batch_autotune_keeps_results(gpu):
     import torch
     _run(torch, True, autotune=True)
+
+
+def test_batched_48_rooms_first_steps(gpu):
+    """The batched workload's 48 rooms (block targets 1k-60k, 320x240 to
+    1280x720), first 3 steps through the captured graph: every scene's
+    per-step incremental mesh (items and their triangles) and final volume
+    (keys in pool order, counters, the device digest of every block) equal a
+    one-volume run of the same frames, made one scene at a time."""
+    import torch
+    steps = 3
+    ws = synth.batched_scenes()
+    params = [_params(w) for w in ws]
+    caps = [200 * w.target_blocks + 65536 for w in ws]
+    frames = []
+    for w in ws:
+        t = torch.empty((steps, w.K.height, w.K.width), dtype=torch.int16, device="cuda:0")
+        synth.render(w.scene, w.K, w.poses[:steps], w.noise, 0, t)
+        frames.append(t)
+    digests = [[] for _ in ws]
+    with vhm.Batch(params) as b:
+        for i, w in enumerate(ws):
+            b.set_sequence(i, frames[i], w.K, w.poses[:steps], tri_capacity=caps[i])
+        b.prepare(use_graph=True)
+        for step in range(steps):
+            print(f"[batch48] step {step}", flush=True)
+            b.step(1)
+            b.synchronize()
+            for i in range(len(ws)):
+                nb, nt, tris = b.mesh_output(i)
+                assert nt <= caps[i], f"scene {i}: triangle buffer overflow"
+                keys, dirs, items = b.mesh_items(i)
+                digests[i].append(item_digest(keys, dirs, tris, items))
+        final = []
+        for i, v in enumerate(b.volumes):
+            keys = v.block_keys()
+            final.append((keys, v.block_digests(keys), v.stats()["updates"], v.observed_voxel_count()))
+    total_blocks = 0
+    for i, w in enumerate(ws):
+        with vhm.Volume(params[i]) as sv:
+            for step in range(steps):
+                sv.integrate(frames[i][step], w.K, w.poses[step])
+                keys, dirs, tris, items = sv.extract_mesh_incremental()
+                assert item_digest(keys, dirs, tris, items) == digests[i][step], f"scene {i} step {step}: mesh"
+            keys = sv.block_keys()
+            assert np.array_equal(keys, final[i][0]), f"scene {i}: block keys / pool order"
+            assert np.array_equal(sv.block_digests(keys), final[i][1]), f"scene {i}: block contents"
+            assert (sv.stats()["updates"], sv.observed_voxel_count()) == final[i][2:], f"scene {i}: counters"
+            total_blocks += len(keys)
+        print(f"[batch48] scene {i} ({w.K.width}x{w.K.height}, {len(keys)} blocks) equal", flush=True)
+    print(f"[batch48] {len(ws)} scenes, {total_blocks} blocks", flush=True)

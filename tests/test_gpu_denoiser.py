@@ -190,3 +190,71 @@ def test_trained_network_beats_plain_few_frame_fusion(gpu):
               f"untrained {net0:.5f}, trained {net:.5f}", flush=True)
         assert abs(ident - ident2) <= 1e-4 * max(ident, 1e-12) + 1e-9  # the identity ignores W1, W2
         assert net < 0.9 * ident
+
+
+def test_adam_kernel_matches_oracle(gpu):
+    """dnAdamKernel (FP32 master weights and moments) against the oracle's
+    adam_step (FP64; itself pinned to torch.optim.Adam) over two steps fed
+    the GPU's own gradients; the gradient reduction is deterministic."""
+    w, vol = _scene_volume(seed=9, frames=4)
+    keys = O.sorted_keys(vol.block_keys())
+    obs = (vol.download_blocks(keys)["weight"] > 0).reshape(len(keys), -1).sum(1)
+    pick = keys[np.argsort(-obs)[:8]]
+    p = D.init_params(5)
+    p["W3"] *= 10.0
+    p0 = D.params_to_flat(p).astype(np.float32)
+    with vhm.Denoiser(64, 16) as dn:
+        dn.set_params(p0)
+        dn.append(vol, pick, vol)
+        _, g1 = dn.train_step(0, len(pick), 0.0, want_grad=True)
+        _, g1b = dn.train_step(0, len(pick), 1e-3, want_grad=True)
+        assert np.array_equal(g1, g1b)
+        p1 = dn.get_params()
+        _, g2 = dn.train_step(0, len(pick), 1e-3, want_grad=True)
+        p2 = dn.get_params()
+    ref = D.flat_to_params(p0.astype(np.float64))
+    m = {k: np.zeros(D.SHAPES[k]) for k in D.ORDER}
+    v = {k: np.zeros(D.SHAPES[k]) for k in D.ORDER}
+    D.adam_step(ref, D.flat_to_params(g1.astype(np.float64)), m, v, 1)
+    r1 = D.params_to_flat(ref)
+    err1 = np.abs(p1 - r1).max()
+    ref = D.flat_to_params(p1.astype(np.float64))
+    D.adam_step(ref, D.flat_to_params(g2.astype(np.float64)), m, v, 2)
+    r2 = D.params_to_flat(ref)
+    err2 = np.abs(p2 - r2).max()
+    print(f"[denoiser] adam max abs err vs fp64: step 1 {err1:.2e}, step 2 {err2:.2e}", flush=True)
+    assert np.allclose(p1, r1, rtol=1e-6, atol=1e-8) and np.allclose(p2, r2, rtol=1e-6, atol=1e-8)
+    assert np.abs(p1 - p0).max() > 5e-4  # the step moved the weights (lr 1e-3)
+    vol.close()
+
+
+def test_apply_kernel_matches_numpy_statement(gpu):
+    """dnApplyKernel against the oracle's apply_blocks, bit for bit: the
+    blocks after write-back, cells replaced and blocks kept plain, for a
+    threshold that keeps about half the blocks plain and then for a
+    weight / delta filter on the already written volume."""
+    w, vol = _scene_volume(seed=5, frames=3)
+    keys = O.sorted_keys(vol.block_keys())
+    obs = (vol.download_blocks(keys)["weight"] > 0).reshape(len(keys), -1).sum(1)
+    pick = keys[np.argsort(-obs)[:16]]
+    p = D.init_params(2)
+    p["W3"] *= 30.0
+    with vhm.Denoiser(64, 32) as dn:
+        dn.set_params(D.params_to_flat(p).astype(np.float32))
+        dn.append(vol, pick, vol)
+        y = dn.forward(0, len(pick))
+        x, _, _ = dn.samples(0, len(pick))
+        xin = D.bf16_to_float(x)[:, 1:9, 1:9, 1:9, :].reshape(len(pick), 512, 16)
+        means = [np.abs(y[b] - xin[b, :, :6])[xin[b, :, 6:12] > 0].mean() for b in range(len(pick))]
+        settings = [(0.0, 10.0, float(np.median(means))), (0.05, 0.02, 10.0)]
+        for min_w, max_delta, bmd in settings:
+            before = vol.download_blocks(pick)
+            want, rep, kept = D.apply_blocks(before, pick, y, x, w.voxel_size, w.truncation, 100.0, min_w, max_delta,
+                                             bmd)
+            r = dn.apply(vol, min_weight=min_w, max_delta=max_delta, block_max_delta=bmd)
+            got = vol.download_blocks(pick)
+            print(f"[denoiser] apply min_w={min_w} max_delta={max_delta} block_max={bmd:.4f}: {r}", flush=True)
+            assert (r["cells_replaced"], r["blocks_kept_plain"]) == (rep, kept)
+            assert rep > 0
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    vol.close()

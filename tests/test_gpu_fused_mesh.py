@@ -3,15 +3,14 @@ pass, one atomic output range per item) against the same call with host
 output (the two-pass count, scan, emit path), on two volumes fed the same
 frames.
 
-Pool order (and so item numbering and output order) differs between two
-volumes -- blocks take pool slots in the order the allocator's atomics run --
-so the outputs are compared per (block key, direction) item: the same items,
+The device output has its items in completion order, so the outputs are
+compared per (block key, direction) item: the same items,
 and for each the same triangles in the same order (cube order, then cluster
 order).  Within one output each item's triangles must be one contiguous run;
 the host output has its items in item order, the device output in the order
-the items finished.  With a device buffer smaller
-than the output, only whole cubes' triangles are written, nothing past the
-capacity, and every written triangle is its item's.  After point-cloud integration
+the items finished.  With a device buffer smaller than the output, only
+whole items are written, an item that does not fit is marked invalid and is
+listed again by the next call.  After point-cloud integration
 (whose blocks carry both near-iso sign hints, so the fused pass's item
 filter keeps them) and frames on top, the same per-item equality holds.
 """
@@ -21,6 +20,7 @@ import numpy as np
 import pytest
 
 import voxelhashmt_amd as vhm
+from meshitems import per_item as _per_item
 from voxelhashmt_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -43,24 +43,6 @@ def _device_call(vol, dev_tris, dev_items):
                                                 vhm.VHM_DEVICE, C.byref(nt))
     assert rc == 0, vhm.lib().vhm_last_error(vol._h)
     return keys[: nb.value], dirs[: nb.value], nb.value, nt.value
-
-
-def _per_item(keys, tris, items, in_order=False):
-    """{(key, direction): triangles in output order}; each item must be one contiguous run."""
-    m = len(keys)
-    items = items.astype(np.int64)
-    if in_order and len(items):
-        assert np.all(np.diff(items) >= 0), "items not in item order"
-    run_starts = np.flatnonzero(np.r_[True, items[1:] != items[:-1]]) if len(items) else np.zeros(0, np.int64)
-    run_ends = np.append(run_starts[1:], len(items))
-    out = {}
-    for s, e in zip(run_starts, run_ends):
-        t = int(items[s])
-        d, i = t // m, t % m
-        k = (tuple(keys[i]), d)
-        assert k not in out, f"item {t} split into several runs"
-        out[k] = tris[s:e]
-    return out
 
 
 @pytest.mark.parametrize("workload", ["tiny", "plain_small"])
@@ -93,6 +75,11 @@ def test_fused_device_output_equals_two_pass_host_output(gpu, workload):
 
 
 def test_fused_device_output_past_capacity(gpu):
+    """An item whose run would cross the buffer's capacity writes no
+    triangle: its entries below capacity carry item 0xFFFFFFFF and NaN
+    coordinates, nothing past capacity is written, every other item is
+    complete, and the next call (nothing integrated in between) lists the
+    dropped items again and returns their triangles."""
     import torch
     w = synth.tiny(n_frames=3, seed=4)
     frames = synth.render(w.scene, w.K, w.poses, w.noise)
@@ -104,18 +91,29 @@ def test_fused_device_output_past_capacity(gpu):
     full = _per_item(kb, tris, items, in_order=True)
     cap = len(tris) // 3
     dev_tris = torch.full((cap * 9 + 9,), -7.0, dtype=torch.float32, device="cuda:0")
-    dev_items = torch.full((cap,), -1, dtype=torch.int32, device="cuda:0")
+    dev_items = torch.full((cap,), -2, dtype=torch.int32, device="cuda:0")
     ka, da, m, n = _device_call(a, dev_tris[: cap * 9], dev_items)
     assert (m, n) == (len(kb), len(tris)) and n > cap
     got = dev_tris[: 9 * cap].cpu().numpy().reshape(-1, 3, 3)
-    got_items = dev_items.cpu().numpy()
-    written = got_items >= 0
-    # a cube whose triangles do not all fit is not written (at most 24 x 5 triangles)
-    assert written[: cap - 120].all()
-    assert np.all(got[~written] == -7.0) and np.all(dev_tris[cap * 9:].cpu().numpy() == -7.0)
-    part = _per_item(ka, got[written], got_items[written].astype(np.uint32))
-    for k, t in part.items():  # each written item's triangles are a prefix of its full list
-        assert np.array_equal(t.view(np.uint32), full[k][: len(t)].view(np.uint32)), k
+    got_items = dev_items.cpu().numpy().view(np.uint32)
+    assert not np.any(got_items == 0xFFFFFFFE), "every entry below capacity belongs to an item"
+    dropped = got_items == 0xFFFFFFFF
+    assert dropped.any() and np.isnan(got[dropped]).all()
+    assert np.all(dev_tris[cap * 9:].cpu().numpy() == -7.0)
+    part = _per_item(ka, got[~dropped], got_items[~dropped])
+    for k, t in part.items():  # a written item is complete
+        assert np.array_equal(t.view(np.uint32), full[k].view(np.uint32)), k
+    missing = set(full) - set(part)
+    assert missing
+    big_tris = torch.empty(len(tris) * 9, dtype=torch.float32, device="cuda:0")
+    big_items = torch.empty(len(tris), dtype=torch.int32, device="cuda:0")
+    ka2, da2, m2, n2 = _device_call(a, big_tris, big_items)
+    again = _per_item(ka2, big_tris[: 9 * n2].cpu().numpy().reshape(-1, 3, 3), big_items[:n2].cpu().numpy().view(np.uint32))
+    print(f"[fused] past capacity: {len(part)} items written, {len(missing)} dropped, {len(again)} re-listed",
+          flush=True)
+    assert missing <= set(again)
+    for k, t in again.items():
+        assert np.array_equal(t.view(np.uint32), full[k].view(np.uint32)), k
     a.close()
     b.close()
 

@@ -187,19 +187,53 @@ def test_capacity_overflow_is_reported(gpu):
         assert e.value.code == 3
 
 
+def _packed(keys):
+    """Packed 21-bit block keys (bias 2^20; x lowest), the order blocks are numbered in."""
+    k = np.asarray(keys, np.int64).reshape(-1, 3) + (1 << 20)
+    return k[:, 0] | (k[:, 1] << 21) | (k[:, 2] << 42)
+
+
 def test_determinism(gpu):
-    w = synth.tiny(n_frames=3, seed=4)
+    """Two runs over the same frames return the same bytes: block keys in pool
+    order, the full extraction and the host-output incremental extraction in
+    their output order.  Pool order is creation order: each frame's new
+    blocks follow the earlier ones, in packed-key order."""
+    w = synth.plain_small(n_frames=3)
     frames = synth.render(w.scene, w.K, w.poses, w.noise)
     outs = []
-    for _ in range(2):
+    for run in range(2):
+        print(f"[determinism] run {run}", flush=True)
         with vhm.Volume(_params(w)) as vol:
+            counts = []
             for i in range(3):
                 vol.integrate(frames[i], w.K, w.poses[i])
-            keys = O.sorted_keys(vol.block_keys())
-            outs.append((keys, vol.download_blocks(keys), vol.extract_mesh()))
+                counts.append(vol.block_count())
+            pool_keys = vol.block_keys()
+            inc = vol.extract_mesh_incremental()
+            outs.append((pool_keys, vol.extract_mesh(), inc))
+            packed = _packed(pool_keys)
+            start = 0
+            for n in counts:
+                assert np.all(np.diff(packed[start:n]) > 0), "a frame's new blocks are numbered in key order"
+                start = n
     assert np.array_equal(outs[0][0], outs[1][0])
-    assert np.array_equal(outs[0][1].view(np.uint8), outs[1][1].view(np.uint8))
-    assert np.array_equal(O.canonical_mesh(outs[0][2]), O.canonical_mesh(outs[1][2]))
+    assert outs[0][1].tobytes() == outs[1][1].tobytes()
+    for a, b in zip(outs[0][2], outs[1][2]):
+        assert np.asarray(a).tobytes() == np.asarray(b).tobytes()
+
+
+def test_point_cloud_blocks_numbered_in_key_order(gpu):
+    """A point cloud's new blocks take pool indices in packed-key order."""
+    w = synth.tiny(n_frames=1, seed=4)
+    vp = _params(w)
+    rng = np.random.default_rng(5)
+    pts = np.stack([rng.uniform(-0.8, 0.8, 4000), rng.uniform(-0.6, 0.6, 4000), rng.uniform(0.5, 1.5, 4000)], 1)
+    pose = vhm.make_pose(np.eye(3, dtype=np.float32), np.zeros(3, np.float32))
+    with vhm.Volume(vp) as vol:
+        vol.integrate_point_cloud(pts.astype(np.float32), pose)
+        packed = _packed(vol.block_keys())
+        assert len(packed) > 100
+        assert np.all(np.diff(packed) > 0)
 
 
 def test_incremental_mesh_equals_full_extraction(gpu):
@@ -287,3 +321,22 @@ def test_incremental_mesh_copy_after_small_buffer(gpu):
     (t0, i0, k0, d0), (t1, i1, k1, d1) = results
     assert np.array_equal(O.canonical_mesh(t0), O.canonical_mesh(t1))
     assert sorted(map(tuple, k0)) == sorted(map(tuple, k1))
+
+
+def test_block_digests_equal_host_hash(gpu):
+    """vhm_block_digests equals the polynomial hash of the bytes
+    vhm_download_blocks returns, per block; an absent key hashes to 0."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from make_golden import block_hash
+    w = synth.tiny(n_frames=2, seed=6)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    with vhm.Volume(_params(w)) as vol:
+        for i in range(2):
+            vol.integrate(frames[i], w.K, w.poses[i])
+        keys = vol.block_keys()
+        probe = np.concatenate([keys, np.array([[900000, 900000, 900000]], np.int32)])
+        dig = vol.block_digests(probe)
+        assert np.array_equal(dig[:-1], block_hash(vol.download_blocks(keys)))
+        assert dig[-1] == 0 and len(set(dig[:-1].tolist())) > len(keys) // 2

@@ -134,6 +134,9 @@ _SIGS = {
                                        C.POINTER(C.c_uint64)]),
     "vhm_batch_mesh_output": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                         C.c_void_p, C.c_size_t]),
+    "vhm_block_digests": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "vhm_batch_mesh_items": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
+                                       C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t]),
     "vhm_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
     "vhm_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
@@ -349,6 +352,15 @@ class Volume:
         m = C.c_size_t()
         self._check(lib().vhm_block_keys(self._h, out.ctypes.data, n, C.byref(m)))
         return out[:n]
+
+    def block_digests(self, keys: np.ndarray) -> np.ndarray:
+        """Per key, the device digest of the block's AoS words (sum w_i P^i mod
+        2^64, P = 0x100000001B3; 0 for an unallocated key)."""
+        k = np.ascontiguousarray(keys, np.int32).reshape(-1, 3)
+        out = np.zeros(len(k), np.uint64)
+        if len(k):
+            self._check(lib().vhm_block_digests(self._h, k.ctypes.data, len(k), out.ctypes.data))
+        return out
 
     def observed_block_keys(self) -> np.ndarray:
         """Keys of blocks with any observed voxel (pool order)."""
@@ -586,6 +598,18 @@ class Batch:
             if tris.value:
                 self._check(lib().vhm_batch_mesh_output(self._h, scene, None, None, out.ctypes.data, tris.value))
         return blocks.value, tris.value, out
+
+    def mesh_items(self, scene: int):
+        """(block keys (m,3), direction masks (m,), triangle items (n,)) of the
+        last step's incremental mesh of one scene (as extract_mesh_incremental)."""
+        blocks, tris, _ = self.mesh_output(scene, fetch=False)
+        keys = np.zeros((max(blocks, 1), 3), np.int32)
+        dirs = np.zeros(max(blocks, 1), np.uint8)
+        items = np.zeros(max(tris, 1), np.uint32)
+        nb = C.c_size_t()
+        self._check(lib().vhm_batch_mesh_items(self._h, scene, keys.ctypes.data, dirs.ctypes.data, blocks,
+                                               C.byref(nb), items.ctypes.data, tris))
+        return keys[:blocks], dirs[:blocks], items[:tris]
 
     def timed_step(self) -> dict:
         """One step with direct launches, timed per stage (synchronous)."""

@@ -21,7 +21,8 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_obj"
 LIB = PKG / "libvhm.so"
 SOURCES = ["vhm_filter.hip", "vhm_alloc.hip", "vhm_integrate.hip", "vhm_mesh.hip", "vhm_synth.hip",
-           "vhm_capi.hip", "vhm_denoise.hip", "vhm_denoise_api.hip", "vhm_batch.hip", "vhm_pointcloud.hip"]
+           "vhm_capi.hip", "vhm_denoise.hip", "vhm_denoise_api.hip", "vhm_batch.hip", "vhm_pointcloud.hip",
+           "vhm_blocks.hip"]
 # Host-only C++ (no GPU code): built with g++ and the reference CPU build's
 # float flags (x86-64 baseline, -O2, no contraction).
 HOST_SOURCES = ["vhm_io.cpp"]

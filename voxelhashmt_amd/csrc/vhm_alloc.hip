@@ -15,7 +15,9 @@
 // stamp (atomic exchange) elects one finder per key per frame, which appends
 // the key's slot to the frame's active list; inserting lanes take a pool index
 // with one atomic add.  LDS overflow falls back to direct global inserts, so
-// the resulting key set equals the reference's for every input.
+// the resulting key set equals the reference's for every input.  A second
+// kernel (numberBlocksKernel) then renumbers the frame's new blocks in key
+// order, so pool order does not depend on the order lanes arrived in.
 #include "vhm_hash.h"
 #include "vhm_internal.h"
 
@@ -172,7 +174,99 @@ __global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) {
   hdr->defer_count = 0u;
   hdr->integrate_ticket = 0u;
   hdr->frame = frame;
+  hdr->number_base = hdr->num_blocks;
 }
+
+__global__ void numberBeginKernel(DevHeader* hdr) { hdr->number_base = hdr->num_blocks; }
+
+// ---- deterministic pool order ----
+//
+// insertBlock hands out pool indices in lane arrival order.  The blocks one
+// pass created hold pool indices [number_base, num_blocks) and nothing has
+// been written to them yet (zeroed pool, zeroed per-block words), so they can
+// be renumbered by rewriting only pool_keys and the table's pool index.  One
+// workgroup sorts the pass's keys (bitonic, in LDS) and gives the i-th
+// smallest key index number_base + i.  More keys than one LDS sort holds
+// (frames that open a scene) are sorted in runs into sort_tmp; a key's rank
+// is then its rank in its run plus, per other run, the number of that run's
+// keys below it (binary search; keys are distinct).
+
+constexpr int kNumberThreads = 1024;
+constexpr uint32_t kNumberRun = 8192;  // keys per LDS sort (64 KiB)
+
+__device__ void bitonicSortLds(uint64_t* s, uint32_t P) {
+  for (uint32_t k = 2; k <= P; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P; i += kNumberThreads) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = s[i], b = s[ixj];
+          if ((a > b) == ((i & k) == 0u)) {
+            s[i] = b;
+            s[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+// Sorts src[0..n) (n <= kNumberRun) into LDS s[0..n), padding with kEmptyKey.
+__device__ void sortRunLds(uint64_t* s, const uint64_t* src, uint32_t n) {
+  uint32_t P = 2;
+  while (P < n) P <<= 1;
+  for (uint32_t i = threadIdx.x; i < P; i += kNumberThreads) s[i] = i < n ? src[i] : kEmptyKey;
+  __syncthreads();
+  bitonicSortLds(s, P);
+}
+
+__device__ void assignNumber(const VolumeView& V, uint64_t key, uint32_t p) {
+  V.pool_keys[p] = key;
+  const uint32_t slot = findSlot(V, key);
+  if (slot != kInvalid) V.vals[slot] = p;
+}
+
+__device__ void numberBlocks(const VolumeView& V) {
+  __shared__ uint64_t s[kNumberRun];
+  const uint32_t base = V.hdr->number_base;
+  const uint32_t nb = V.hdr->num_blocks;
+  const uint32_t end = nb < V.max_blocks ? nb : V.max_blocks;
+  if (end <= base + 1u) return;  // zero or one new block: nothing to order
+  const uint32_t n = end - base;
+  const uint64_t* src = V.pool_keys + base;
+  if (n <= kNumberRun) {
+    sortRunLds(s, src, n);
+    for (uint32_t i = threadIdx.x; i < n; i += kNumberThreads) assignNumber(V, s[i], base + i);
+    return;
+  }
+  const uint32_t runs = (n + kNumberRun - 1u) / kNumberRun;
+  for (uint32_t r = 0; r < runs; ++r) {
+    const uint32_t first = r * kNumberRun, len = (n - first) < kNumberRun ? (n - first) : kNumberRun;
+    sortRunLds(s, src + first, len);
+    for (uint32_t i = threadIdx.x; i < len; i += kNumberThreads) V.sort_tmp[first + i] = s[i];
+    __syncthreads();  // s is reloaded by the next run
+  }
+  __threadfence();
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += kNumberThreads) {
+    const uint64_t key = V.sort_tmp[i];
+    const uint32_t own = i / kNumberRun;
+    uint32_t rank = i - own * kNumberRun;
+    for (uint32_t r = 0; r < runs; ++r) {
+      if (r == own) continue;
+      const uint64_t* run = V.sort_tmp + r * kNumberRun;
+      uint32_t lo = 0, hi = (n - r * kNumberRun) < kNumberRun ? (n - r * kNumberRun) : kNumberRun;
+      while (lo < hi) {  // first element >= key
+        const uint32_t mid = (lo + hi) >> 1;
+        if (run[mid] < key) lo = mid + 1u; else hi = mid;
+      }
+      rank += lo;
+    }
+    assignNumber(V, key, base + rank);
+  }
+}
+
+__global__ __launch_bounds__(kNumberThreads) void numberBlocksKernel(VolumeView V) { numberBlocks(V); }
 
 // ---- batched steps ----
 
@@ -183,6 +277,15 @@ __global__ void batchFrameBeginKernel(const BatchScene* __restrict__ scenes, int
   scenes[s].V.hdr->defer_count = 0u;
   scenes[s].V.hdr->integrate_ticket = 0u;
   scenes[s].V.hdr->frame = st->step + 1u;
+  scenes[s].V.hdr->number_base = scenes[s].V.hdr->num_blocks;
+}
+
+// One workgroup per scene.
+__global__ __launch_bounds__(kNumberThreads) void numberBlocksBatchKernel(const BatchScene* __restrict__ scenes,
+                                                                          const BatchState* __restrict__ st) {
+  const BatchScene& sc = scenes[blockIdx.x];
+  if (st->step >= sc.n_frames) return;
+  numberBlocks(sc.V);
 }
 
 __global__ __launch_bounds__(256) void allocateBatchKernel(const BatchScene* __restrict__ scenes, int S,
@@ -237,6 +340,22 @@ hipError_t launchAllocateBatch(const BatchScene* scenes, int S, const uint32_t* 
 hipError_t launchActivePrefix(const BatchScene* scenes, int S, const BatchState* st, uint32_t* prefix,
                               hipStream_t s) {
   activePrefixKernel<<<1, 64, 0, s>>>(scenes, S, st, prefix);
+  return hipGetLastError();
+}
+
+hipError_t launchNumberBlocksBatch(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s) {
+  if (S <= 0) return hipSuccess;
+  numberBlocksBatchKernel<<<S, kNumberThreads, 0, s>>>(scenes, st);
+  return hipGetLastError();
+}
+
+hipError_t launchNumberBegin(DevHeader* hdr, hipStream_t s) {
+  numberBeginKernel<<<1, 1, 0, s>>>(hdr);
+  return hipGetLastError();
+}
+
+hipError_t launchNumberBlocks(const VolumeView& V, hipStream_t s) {
+  numberBlocksKernel<<<1, kNumberThreads, 0, s>>>(V);
   return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 //   batchFrameBegin     reset every scene's active list, frame id = step + 1
 //   filterBatch         all scenes' filter tiles in one grid
 //   allocateBatch       all scenes' allocation tiles in one grid
+//   numberBlocksBatch   each scene's new blocks renumbered in key order
 //   activePrefix        prefix sums of the scenes' active-block counts
 //   integrateBatch      persistent grid over all scenes' active blocks
 //   incremental mesh    all scenes' dirty blocks (+ -x/-y/-z neighbours):
@@ -81,15 +82,22 @@ void noteStep(vhm_batch* b) {
   ++b->steps_done;
 }
 
+// A step's stages before integration: frame begin, filter, allocation, the
+// ordered numbering of new blocks, the active-list prefix.
+hipError_t enqueueFrontEnd(vhm_batch* b) {
+  const int S = b->S;
+  hipError_t e = launchBatchFrameBegin(b->scenes, S, b->state, b->stream);
+  if (e == hipSuccess) e = launchFilterBatch(b->scenes, S, b->filter_prefix, b->filter_total, b->state, b->stream);
+  if (e == hipSuccess) e = launchAllocateBatch(b->scenes, S, b->alloc_prefix, b->alloc_total, b->state, b->stream);
+  if (e == hipSuccess) e = launchNumberBlocksBatch(b->scenes, S, b->state, b->stream);
+  if (e == hipSuccess) e = launchActivePrefix(b->scenes, S, b->state, b->active_prefix, b->stream);
+  return e;
+}
+
 // Launches one step's kernels on b->stream (also what the graph captures).
 int enqueueStep(vhm_batch* b) {
   const int S = b->S;
-  B_TRY(b, launchBatchFrameBegin(b->scenes, S, b->state, b->stream), "batch frame begin launch");
-  B_TRY(b, launchFilterBatch(b->scenes, S, b->filter_prefix, b->filter_total, b->state, b->stream),
-        "batch filter launch");
-  B_TRY(b, launchAllocateBatch(b->scenes, S, b->alloc_prefix, b->alloc_total, b->state, b->stream),
-        "batch allocation launch");
-  B_TRY(b, launchActivePrefix(b->scenes, S, b->state, b->active_prefix, b->stream), "batch prefix launch");
+  B_TRY(b, enqueueFrontEnd(b), "batch front-end launch");
   B_TRY(b, launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, b->integrate_grid, b->stream),
         "batch integrate launch");
   B_TRY(b, launchMeshBatch(b->scenes, S, b->mesh_prefix, b->mesh_grid, b->mesh_cand, b->stream), "batch mesh launch");
@@ -237,6 +245,7 @@ int vhm_batch_set_sequence(vhm_batch* b, int scene, const uint16_t* frames_devic
   B_TRY(b, bmalloc(b, &sc.mesh_obs_dirs, mb), "hipMalloc mesh source directions");
   sc.tri_capacity = static_cast<uint32_t>(tri_capacity);
   B_TRY(b, bmalloc(b, &sc.tris, 36 * (tri_capacity ? tri_capacity : 1)), "hipMalloc triangles");
+  B_TRY(b, bmalloc(b, &sc.tri_owner, 4 * (tri_capacity ? tri_capacity : 1)), "hipMalloc triangle items");
   B_TRY(b, bmalloc(b, &sc.out, 16), "hipMalloc mesh outputs");
   B_TRY(b, hipMemset(sc.out, 0, 16), "hipMemset mesh outputs");
   b->ready[scene] = 1;
@@ -322,7 +331,8 @@ int vhm_batch_synchronize(vhm_batch* b) {
 
 // Last step's incremental mesh of a scene: blocks re-meshed, triangles
 // (the total, which may exceed the capacity) and optionally the triangles
-// (min(total, capacity) x 9 floats, host).
+// (min(total, capacity) x 9 floats, host).  Each step overwrites the
+// previous step's output.
 int vhm_batch_mesh_output(vhm_batch* b, int scene, uint32_t* blocks, uint32_t* triangles, float* tris_host,
                           size_t capacity) {
   if (!b || scene < 0 || scene >= b->S) return VHM_ERR_INVALID_ARGUMENT;
@@ -337,6 +347,35 @@ int vhm_batch_mesh_output(vhm_batch* b, int scene, uint32_t* blocks, uint32_t* t
     n = n < capacity ? n : capacity;
     if (n) B_TRY(b, hipMemcpy(tris_host, sc.tris, n * 36, hipMemcpyDeviceToHost), "hipMemcpy triangles");
   }
+  return VHM_OK;
+}
+
+// The items of the last step's incremental mesh of a scene: the listed
+// blocks (keys in pool order, direction masks) and each triangle's item.
+int vhm_batch_mesh_items(vhm_batch* b, int scene, int32_t* block_keys_out, uint8_t* block_dirs_out,
+                         size_t block_capacity, size_t* n_blocks_out, uint32_t* tri_item_out, size_t tri_capacity) {
+  if (!b || scene < 0 || scene >= b->S || !n_blocks_out) return VHM_ERR_INVALID_ARGUMENT;
+  B_TRY(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+  uint32_t o[4] = {0, 0, 0, 0};
+  const BatchScene& sc = b->host_scenes[scene];
+  B_TRY(b, hipMemcpy(o, sc.out, 16, hipMemcpyDeviceToHost), "hipMemcpy mesh outputs");
+  *n_blocks_out = o[0];
+  const size_t kb = block_capacity < o[0] ? block_capacity : o[0];
+  if (kb && block_keys_out) {
+    int32_t* dkeys = nullptr;
+    B_TRY(b, hipMalloc(&dkeys, 12 * kb), "hipMalloc key buffer");
+    hipError_t e = launchListKeys(sc.V, sc.remesh_list, static_cast<uint32_t>(kb), dkeys, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(block_keys_out, dkeys, 12 * kb, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dkeys);
+    B_TRY(b, e, "batch mesh keys");
+  }
+  if (kb && block_dirs_out)
+    B_TRY(b, hipMemcpy(block_dirs_out, sc.remesh_dirs, kb, hipMemcpyDeviceToHost), "hipMemcpy remesh directions");
+  size_t n = o[1] < sc.tri_capacity ? o[1] : sc.tri_capacity;
+  n = n < tri_capacity ? n : tri_capacity;
+  if (n && tri_item_out)
+    B_TRY(b, hipMemcpy(tri_item_out, sc.tri_owner, 4 * n, hipMemcpyDeviceToHost), "hipMemcpy triangle items");
   return VHM_OK;
 }
 
@@ -360,10 +399,7 @@ int timedStep(vhm_batch* b, int gi, int gm, float ms[3], uint64_t* updates, uint
   hipEvent_t ev[4] = {};
   for (auto& e : ev) B_TRY(b, hipEventCreate(&e), "hipEventCreate");
   hipError_t e = hipEventRecord(ev[0], b->stream);
-  if (e == hipSuccess) e = launchBatchFrameBegin(b->scenes, S, b->state, b->stream);
-  if (e == hipSuccess) e = launchFilterBatch(b->scenes, S, b->filter_prefix, b->filter_total, b->state, b->stream);
-  if (e == hipSuccess) e = launchAllocateBatch(b->scenes, S, b->alloc_prefix, b->alloc_total, b->state, b->stream);
-  if (e == hipSuccess) e = launchActivePrefix(b->scenes, S, b->state, b->active_prefix, b->stream);
+  if (e == hipSuccess) e = enqueueFrontEnd(b);
   if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
   if (e == hipSuccess) e = launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, gi, b->stream);
   if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);

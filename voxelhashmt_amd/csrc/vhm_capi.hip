@@ -80,6 +80,7 @@ struct vhm_ctx {
   // integrator's deferred border blocks (max_blocks each; flags kept zero between frames)
   uint32_t* defer_flags = nullptr;
   uint32_t* defer_list = nullptr;
+  uint64_t* sort_tmp = nullptr;  // block numbering: sorted runs of a pass's new keys (max_blocks)
   // point clouds (grown on demand)
   float* pc_points = nullptr;
   size_t pc_points_cap = 0;
@@ -146,6 +147,7 @@ VolumeView view(vhm_ctx* c) {
   V.hdr = c->hdr;
   V.defer_flags = c->defer_flags;
   V.defer_list = c->defer_list;
+  V.sort_tmp = c->sort_tmp;
   return V;
 }
 
@@ -251,7 +253,8 @@ int allocStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, cons
   aa.frame = c->frame;
   hipEvent_t ev{};
   beginTimed(c, 1, &ev);
-  const hipError_t e = launchAllocate(dev_depth, view(c), aa, c->stream);
+  hipError_t e = launchAllocate(dev_depth, view(c), aa, c->stream);
+  if (e == hipSuccess) e = launchNumberBlocks(view(c), c->stream);
   endTimed(c, ev);
   VHM_TRY(c, e, "block allocation kernel launch");
   return VHM_OK;
@@ -461,6 +464,7 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->remesh_dirs, size_t(p->max_blocks));
   if (e == hipSuccess) e = hipMalloc(&c->defer_flags, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->defer_list, size_t(p->max_blocks) * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->sort_tmp, size_t(p->max_blocks) * 8u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_partials, scanPartialsSize(p->max_blocks) * 4u);
   if (e != hipSuccess) {
     vhm_destroy(c);
@@ -485,7 +489,7 @@ int vhm_destroy(vhm_ctx* c) {
                   c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
                   c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->mesh_counters, c->mesh_nbr, c->mesh_obs_dirs,
-                  c->defer_flags, c->defer_list, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
+                  c->defer_flags, c->defer_list, c->sort_tmp, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
                   c->pc_vals, c->pc_vals_tmp, c->pc_count, c->pc_hist, c->pc_offsets, c->pc_partials};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -630,8 +634,10 @@ int vhm_integrate_point_cloud(vhm_ctx* c, const float* points, size_t n, int mem
       pts = c->pc_points;
     }
     VHM_TRY(c, hipMemsetAsync(c->pc_count, 0, sizeof(uint32_t), c->stream), "hipMemset record count");
+    VHM_TRY(c, launchNumberBegin(c->hdr, c->stream), "block numbering kernel launch");
     VHM_TRY(c, launchPointCloudSample(pts, m, V, a, c->pc_info, c->pc_keys, c->pc_vals, c->pc_count, c->stream),
             "point cloud sample kernel launch");
+    VHM_TRY(c, launchNumberBlocks(V, c->stream), "block numbering kernel launch");
     uint32_t nrec = 0;
     VHM_TRY(c, hipMemcpyAsync(&nrec, c->pc_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
             "hipMemcpy D2H record count");
@@ -764,6 +770,31 @@ int vhm_download_blocks(vhm_ctx* c, const int32_t* keys, size_t n, void* out_hos
     VHM_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(out_host) + off * kBlockWords * 4u, c->gather_buf,
                               m * kBlockWords * 4u, hipMemcpyDeviceToHost, c->stream),
             "hipMemcpy D2H blocks");
+    VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  }
+  return VHM_OK;
+}
+
+int vhm_block_digests(vhm_ctx* c, const int32_t* keys, size_t n, uint64_t* out_host) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  if (n == 0) return VHM_OK;
+  if (!keys || !out_host) return fail(c, VHM_ERR_INVALID_ARGUMENT, "block digest failed: null argument");
+  DevHeader h{};
+  int rc = readHeader(c, &h);
+  if (rc) return rc;
+  const size_t chunk = size_t(1) << 20;
+  const size_t m0 = n < chunk ? n : chunk;
+  if ((rc = ensure(c, &c->key_buf, &c->key_buf_cap, 3 * m0, "hipMalloc key buffer"))) return rc;
+  if ((rc = ensure(c, &c->gather_buf, &c->gather_cap, 2 * m0, "hipMalloc digest buffer"))) return rc;
+  for (size_t off = 0; off < n; off += chunk) {
+    const size_t m = (n - off) < chunk ? (n - off) : chunk;
+    VHM_TRY(c, hipMemcpyAsync(c->key_buf, keys + 3 * off, 3 * m * sizeof(int32_t), hipMemcpyHostToDevice, c->stream),
+            "hipMemcpy H2D keys");
+    uint64_t* dig = reinterpret_cast<uint64_t*>(c->gather_buf);
+    VHM_TRY(c, launchBlockDigests(c->key_buf, static_cast<uint32_t>(m), view(c), dig, c->stream),
+            "block digest launch");
+    VHM_TRY(c, hipMemcpyAsync(out_host + off, dig, m * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H digests");
     VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   }
   return VHM_OK;

@@ -421,38 +421,6 @@ __global__ __launch_bounds__(256) void integrateDeferredBatchKernel(const BatchS
   flushTally(sc.V, t);
 }
 
-// Copies blocks into the reference's AoS Voxel layout (voxel_hash_tsdf.h:18-31):
-// out[block][voxel][dir][slot][field].  One workgroup per requested key.
-__global__ __launch_bounds__(256) void gatherBlocksAoSKernel(const int32_t* __restrict__ keys, VolumeView V,
-                                                             uint32_t* __restrict__ out) {
-  const uint32_t b = blockIdx.x;
-  const int x = keys[3 * b], y = keys[3 * b + 1], z = keys[3 * b + 2];
-  uint32_t p = kInvalid;
-  if (keyInRange(x, y, z)) {
-    const uint64_t key = packKey(x, y, z);
-    uint32_t slot = hashSlot(key, V.log2cap);
-    const int bound = (V.mask + 1u) < static_cast<uint32_t>(kMaxProbe) ? static_cast<int>(V.mask + 1u) : kMaxProbe;
-    for (int probe = 0; probe < bound; ++probe) {
-      const uint64_t cur = V.keys[slot];
-      if (cur == key) {
-        p = V.vals[slot];
-        break;
-      }
-      if (cur == kEmptyKey) break;
-      slot = (slot + 1u) & V.mask;
-    }
-  }
-  uint32_t* dst = out + static_cast<size_t>(b) * kBlockWords;
-  const uint32_t* src = (p < V.max_blocks) ? V.pool + static_cast<size_t>(p) * kBlockWords : nullptr;
-  for (uint32_t i = threadIdx.x; i < kBlockWords; i += 256) {
-    // destination index i = ((voxel * 6 + dir) * 3 + slot) * 4 + field
-    const uint32_t field = i & 3u, rest = i >> 2;
-    const uint32_t slotk = rest % 3u, rest2 = rest / 3u;
-    const uint32_t dir = rest2 % 6u, voxel = rest2 / 6u;
-    dst[i] = src ? src[poolWord(dir, field, slotk, voxel)] : 0u;
-  }
-}
-
 }  // namespace
 
 hipError_t launchIntegrate(const float* depth, const VolumeView& V, const IntegrateArgs& a, int grid,
@@ -466,13 +434,6 @@ hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t*
                                 int grid, hipStream_t s) {
   integrateBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix, st);
   integrateDeferredBatchKernel<<<dim3(64, S), 256, 0, s>>>(scenes, st);
-  return hipGetLastError();
-}
-
-hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeView& V, uint32_t* out,
-                                 hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  gatherBlocksAoSKernel<<<n, 256, 0, s>>>(keys, V, out);
   return hipGetLastError();
 }
 

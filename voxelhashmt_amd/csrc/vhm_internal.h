@@ -5,14 +5,17 @@
 //   hash table   keys[2^k] u64 packed block key (empty = ~0)
 //                vals[2^k] u32 pool index of the block (~0 = none)
 //                stamp[2^k] u32 last frame id that touched the key
-//   block pool   pool[max_blocks][6 dir][4 field][3 slot][512 voxel] (f32/u32)
-//                field 0 weight, 1 surface_coord, 2 plane_offset,
-//                3 packed_normal.  One (dir, field, slot) row is 2 KiB of
-//                contiguous voxels, so a wavefront's 64 lanes read 256 B.
-//                147,456 B per block, as the reference's Voxel array.
+//   block pool   pool[max_blocks][6 dir][512 voxel][3 slot][4 field]
+//                (f32/u32; poolWord below): field 0 weight, 1 surface_coord,
+//                2 plane_offset, 3 packed_normal.  A voxel's 3 slots of one
+//                direction are 48 contiguous bytes.  147,456 B per block, as
+//                the reference's Voxel array.
 //                pool_keys[max_blocks] u64, obs[max_blocks][16] u32 observed
 //                bitmask, dirty[max_blocks] u32, dirmask[max_blocks] u32
-//                (directions with observations).
+//                (directions with observations, sign hints).  Pool indices
+//                are handed out in creation order: pass by pass (frame,
+//                point-cloud chunk), and within a pass in packed-key order,
+//                so the pool order of a volume depends only on its inputs.
 //   per frame    active[2^k] u32 hash slots touched this frame,
 //                depth staging, filtered depth (W x H f32).
 #pragma once
@@ -51,6 +54,7 @@ struct DevHeader {
   unsigned long long mesh_total;
   uint32_t defer_count;  // blocks appended to the integrator's deferred list this frame
   uint32_t integrate_ticket;  // integrator: next active-list entry to hand out this frame
+  uint32_t number_base;       // num_blocks when the current insertion pass began
 };
 
 struct VolumeView {
@@ -74,6 +78,9 @@ struct VolumeView {
   // Null in volumes driven only by the batched step.
   uint32_t* defer_flags;
   uint32_t* defer_list;
+  // launchNumberBlocks: the sorted runs of a pass's new keys when there are
+  // more than one LDS sort holds (max_blocks entries)
+  uint64_t* sort_tmp;
 };
 
 // Pool word of (direction, field, slot) of voxel vi within a block.  Layout
@@ -104,9 +111,12 @@ struct AllocArgs {
 // Sticky per-block hints for marching cubes at iso 0, in dirmask: bit
 // kSignBelow + d (kSignAbove + d) is set once a slot of direction d was
 // written whose plane distance at its voxel's centre may lie within the
-// staging's near window of 0 and below 0 (at or above 0).  An item none of
-// whose 8 source blocks has both bits of its direction can have no cube
-// with corners on both sides, so no triangle.  Writers that do not compute
+// staging's near window of 0 and below 0 (at or above 0).  The rule is over
+// the item's 8 source blocks together: when the OR of their direction masks
+// lacks the below bit or the above bit of the item's direction, no cube of
+// the item can have corners on both sides, so no triangle, and the item is
+// skipped.  (A per-block test would be wrong: the corner below 0 and the one
+// above may sit in different source blocks.)  Writers that do not compute
 // the distance set both bits.
 constexpr uint32_t kSignBelow = 8u;
 constexpr uint32_t kSignAbove = 16u;
@@ -163,10 +173,20 @@ hipError_t launchFilterF32(const float* in, float* out, const FilterArgs& a, hip
 hipError_t launchFilterU16(const uint16_t* in, float* out, const FilterArgs& a, hipStream_t s);
 hipError_t launchFrameBegin(DevHeader* hdr, uint32_t frame, hipStream_t s);
 hipError_t launchAllocate(const float* depth, const VolumeView& V, const AllocArgs& a, hipStream_t s);
+// Deterministic pool order: launchNumberBegin records num_blocks before an
+// insertion pass (frameBegin does it for frames); launchNumberBlocks then
+// renumbers the blocks the pass created, in packed-key order, and must run
+// before anything reads their pool indices.
+hipError_t launchNumberBegin(DevHeader* hdr, hipStream_t s);
+hipError_t launchNumberBlocks(const VolumeView& V, hipStream_t s);
 hipError_t launchIntegrate(const float* depth, const VolumeView& V, const IntegrateArgs& a, int grid,
                            hipStream_t s);
 hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeView& V, uint32_t* out,
                                  hipStream_t s);
+// out[b]: polynomial digest (sum w_i 0x100000001B3^i mod 2^64) of the AoS
+// words of the block of keys[b] (vhm_blocks.hip).
+hipError_t launchBlockDigests(const int32_t* keys, uint32_t n, const VolumeView& V, uint64_t* out,
+                              hipStream_t s);
 // Marching cubes over block `list` (pool indices; null = all of 0..n-1):
 // counts[dir * n + i] and per-cube counts cube_counts[512 * (dir * n + i) ..]
 // (6 * n * 512 bytes), then triangles at the scanned offsets.
@@ -220,6 +240,7 @@ struct BatchScene {
   uint32_t* mesh_nbr;         // [max_blocks][8] source blocks per listed block
   uint8_t* mesh_obs_dirs;     // [max_blocks] observed directions over those sources
   float* tris;                // [tri_capacity][9]
+  uint32_t* tri_owner;        // [tri_capacity] item d * listed + i of each triangle
   uint32_t tri_capacity;
   uint32_t* out;              // [4]: blocks re-meshed, triangles (total, may exceed capacity)
   MeshArgs ma;
@@ -242,6 +263,7 @@ hipError_t launchFilterBatch(const BatchScene* scenes, int S, const uint32_t* ti
 hipError_t launchBatchFrameBegin(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s);
 hipError_t launchAllocateBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
                                const BatchState* st, hipStream_t s);
+hipError_t launchNumberBlocksBatch(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s);
 hipError_t launchActivePrefix(const BatchScene* scenes, int S, const BatchState* st, uint32_t* prefix,
                               hipStream_t s);
 hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t* prefix, const BatchState* st,

@@ -269,9 +269,9 @@ VHM_HD float approxRsqrt(float x) {
 // rsqrt(dot3(m, m)) of the unnormalised direction m both lie within 8 ulp of
 // 1 of the exact m.n / |m| (R: 2.5 ulp in the length, 1 in each quotient, 3
 // in the dot product; A: 3 in each dot product, 2 in the rsqrt, 1 in the
-// product), so |A - R| < 2^-20.  Outside a 2^-17 band around min_cos the
-// estimate decides; inside it the reference computation does.  `rsqrt` is
-// any 1/sqrt within 2 ulp.
+// product): each is within 2^-20 of it, so |A - R| < 2^-19.  Outside a
+// 2^-17 band around min_cos (a 4x margin) the estimate decides; inside it the
+// reference computation does.  `rsqrt` is any 1/sqrt within 2 ulp.
 template <class Rsqrt>
 VHM_HD bool octNormalCosBelowWith(uint32_t packed, F3 n, float min_cos, Rsqrt rsqrt) {
   const F3 m = octDirection(packed);

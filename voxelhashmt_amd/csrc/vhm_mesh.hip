@@ -708,7 +708,8 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
   if (tid == 0) S.base = total ? atomicAdd(total_out, total) : 0u;
   __syncthreads();
   const uint32_t base = S.base;
-  if (total > 0 && base < capacity) {  // workgroup-uniform
+  const bool fits = static_cast<uint64_t>(base) + total <= capacity;  // workgroup-uniform
+  if (total > 0 && fits) {
     const uint32_t m = S.nemit;
     for (uint32_t e = tid; e < m; e += 256) {
       const uint32_t j = S.list_cube[e];
@@ -717,12 +718,22 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
       const int cube = static_cast<int>(meta & 0xffffu), ncl = static_cast<int>(meta >> 16);
       const uint32_t start = base + S.list_start[cube];
       const uint32_t c = S.cnt[cube];
-      if (start + c <= capacity) {
-        emitFromCentres(S, key, cube, a, set + 256 + (j & 255u), ncl, tris + 9ull * start);
-        if (tri_owner)
-          for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = item;
-      }
+      emitFromCentres(S, key, cube, a, set + 256 + (j & 255u), ncl, tris + 9ull * start);
+      if (tri_owner)
+        for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = item;
     }
+  } else if (total > 0) {
+    // The item does not fit the buffer: none of its triangles is written.
+    // Its entries below capacity are marked invalid (item kInvalid, NaN
+    // coordinates), and its direction is marked dirty again so that the next
+    // incremental call lists it.
+    const uint32_t end = static_cast<uint64_t>(base) + total < capacity ? base + total : capacity;
+    for (uint32_t t = base + tid; t < end; t += 256) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) tris[9ull * t + k] = __uint_as_float(0x7fc00000u);
+      if (tri_owner) tri_owner[t] = kInvalid;
+    }
+    if (tid == 0) atomicOr(&V.dirty[b], 1u << dir);
   }
   __syncthreads();  // the item's LDS is reused by the next item
   return total;
@@ -1068,7 +1079,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const uint32_t dir = j / m, idx = j % m;
     if (sc.remesh_dirs[idx] & sc.mesh_obs_dirs[idx] & (1u << dir))  // workgroup-uniform
       fusedItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, dir * m + idx, sc.out + 1, sc.tris,
-                nullptr, sc.tri_capacity, sc.mesh_nbr + 8ull * idx);
+                sc.tri_owner, sc.tri_capacity, sc.mesh_nbr + 8ull * idx);
   }
 }
 
