@@ -246,7 +246,9 @@ def test_apply_kernel_matches_numpy_statement(gpu):
         x, _, _ = dn.samples(0, len(pick))
         xin = D.bf16_to_float(x)[:, 1:9, 1:9, 1:9, :].reshape(len(pick), 512, 16)
         means = [np.abs(y[b] - xin[b, :, :6])[xin[b, :, 6:12] > 0].mean() for b in range(len(pick))]
-        settings = [(0.0, 10.0, float(np.median(means))), (0.05, 0.02, 10.0)]
+        valid = xin[..., 6:12] > 0
+        med_delta = float(np.median(np.abs(y - xin[..., :6])[valid]))
+        settings = [(0.0, 10.0, float(np.median(means))), (0.05, med_delta, 10.0)]
         for min_w, max_delta, bmd in settings:
             before = vol.download_blocks(pick)
             want, rep, kept = D.apply_blocks(before, pick, y, x, w.voxel_size, w.truncation, 100.0, min_w, max_delta,
