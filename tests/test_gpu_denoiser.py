@@ -248,7 +248,8 @@ def test_apply_kernel_matches_numpy_statement(gpu):
         means = [np.abs(y[b] - xin[b, :, :6])[xin[b, :, 6:12] > 0].mean() for b in range(len(pick))]
         valid = xin[..., 6:12] > 0
         med_delta = float(np.median(np.abs(y - xin[..., :6])[valid]))
-        settings = [(0.0, 10.0, float(np.median(means))), (0.05, med_delta, 10.0)]
+        med_w = float(np.median(xin[..., 6:12][valid]))  # weight / max_weight of observed cells
+        settings = [(0.0, 10.0, float(np.median(means))), (med_w, med_delta, 10.0)]
         for min_w, max_delta, bmd in settings:
             before = vol.download_blocks(pick)
             want, rep, kept = D.apply_blocks(before, pick, y, x, w.voxel_size, w.truncation, 100.0, min_w, max_delta,
