@@ -213,6 +213,21 @@ int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capa
 int vhm_extract_mesh_incremental(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out, uint8_t* block_dirs_out,
                                  size_t block_capacity, size_t* n_blocks_out, float* tris_out,
                                  uint32_t* tri_item_out, size_t tri_capacity, int out_memkind, size_t* n_tris_out);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  vhm_extract_mesh_incremental with device output
+ * (VHM_DEVICE: tris_out / tri_item_out device memory, tri_item_out
+ * optional; block_keys_out / block_dirs_out host memory), with the same
+ * results, but it allocates nothing: its scratch is a fixed 58 MB device
+ * array in libvhm.so, one per GPU, kept until the process ends, shared by
+ * every call on that GPU (calls that overlap on one GPU race).  At most
+ * 131,072 blocks re-meshed per call: a larger list returns VHM_ERR_CAPACITY
+ * and leaves the blocks dirty for a later call.  tri_capacity 0 only
+ * counts (every item stays dirty).  Synchronous. */
+int vhm_extract_mesh_incremental_noalloc(vhm_ctx* ctx, float iso_level, int32_t* block_keys_out,
+                                         uint8_t* block_dirs_out, size_t block_capacity, size_t* n_blocks_out,
+                                         float* tris_out, uint32_t* tri_item_out, size_t tri_capacity,
+                                         size_t* n_tris_out);
 /* Copies again the triangles (and items) of the last incremental extraction
  * that used VHM_HOST output -- all of them, min(capacity, n), for a caller
  * whose first buffer was too small; *n_out = n.  Synchronous. */

@@ -134,6 +134,9 @@ _SIGS = {
                                        C.POINTER(C.c_uint64)]),
     "vhm_batch_mesh_output": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                         C.c_void_p, C.c_size_t]),
+    "vhm_extract_mesh_incremental_noalloc": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_size_t,
+                                                       C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t,
+                                                       C.POINTER(C.c_size_t)]),
     "vhm_block_digests": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "vhm_batch_mesh_items": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
                                        C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t]),

@@ -30,6 +30,63 @@ hipError_t launchActiveKeys(const VolumeView& V, int32_t* out, hipStream_t s, ui
 
 using namespace vhm;
 
+namespace {
+
+// ---- no-allocation incremental meshing: fixed device scratch ----
+//
+// vhm_extract_mesh_incremental_noalloc keeps everything its kernels need
+// beyond the context's create-time buffers and the caller's outputs in one
+// file-scope device array: the triangle counter, the 8 source blocks and
+// the observed-direction mask of every listed block, the listed keys
+// staged for the copy to the host, and the cluster-centre columns of the
+// fused pass (kMeshFusedCandWords floats per workgroup, at most
+// kNoAllocGrid workgroups).  Each GPU holds its own copy for the life of
+// the process; calls on one GPU share it, so calls that overlap on one GPU
+// race.
+constexpr uint32_t kNoAllocMaxListed = 131072;
+constexpr uint32_t kNoAllocGrid = 1024;
+constexpr size_t kNoAllocCounterOff = 0;
+constexpr size_t kNoAllocNbrOff = 256;
+constexpr size_t kNoAllocObsOff = kNoAllocNbrOff + size_t(8) * kNoAllocMaxListed * 4;
+constexpr size_t kNoAllocKeysOff = kNoAllocObsOff + kNoAllocMaxListed;
+constexpr size_t kNoAllocCandOff = kNoAllocKeysOff + size_t(3) * kNoAllocMaxListed * 4;
+constexpr size_t kNoAllocBytes = kNoAllocCandOff + size_t(kNoAllocGrid) * kMeshFusedCandWords * sizeof(float);
+static_assert(kNoAllocObsOff % 256 == 0 && kNoAllocKeysOff % 256 == 0 && kNoAllocCandOff % 256 == 0,
+              "scratch regions stay 256-byte aligned");
+
+}  // namespace
+
+// Host code must not pass `scratch` itself to a kernel or a copy: its name
+// is a host-side stand-in.  Its device address on each GPU comes from
+// hipGetSymbolAddress (noAllocScratch).
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked
+// up once per device; nullptr with *e set on failure.
+uint8_t* noAllocScratch(int device, hipError_t* e) {
+  *e = hipSuccess;
+  if (device < 0 || device >= kMaxDevices) {
+    *e = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    *e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (*e != hipSuccess) return nullptr;
+    g_scratch_ptr[device] = static_cast<uint8_t*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+
+}  // namespace
+
 struct vhm_ctx {
   vhm_params P{};
   int device = 0;
@@ -471,6 +528,11 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
     return VHM_ERR_HIP;
   }
   if (initVolume(c) != VHM_OK) {
+    vhm_destroy(c);
+    return VHM_ERR_HIP;
+  }
+  hipError_t se = hipSuccess;
+  if (!noAllocScratch(c->device, &se)) {
     vhm_destroy(c);
     return VHM_ERR_HIP;
   }
@@ -956,6 +1018,79 @@ int vhm_extract_mesh_incremental(vhm_ctx* c, float iso, int32_t* block_keys_out,
   endTimed(c, ev);
   VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   if (c->timing) drainEvents(c);
+  return VHM_OK;
+}
+
+int vhm_extract_mesh_incremental_noalloc(vhm_ctx* c, float iso, int32_t* block_keys_out, uint8_t* block_dirs_out,
+                                         size_t block_capacity, size_t* n_blocks_out, float* tris_out,
+                                         uint32_t* tri_item_out, size_t tri_capacity, size_t* n_tris_out) {
+  if (!c || !n_blocks_out || !n_tris_out) return VHM_ERR_INVALID_ARGUMENT;
+  *n_blocks_out = 0;
+  *n_tris_out = 0;
+  if (tri_capacity > 0 && !tris_out)
+    return fail(c, VHM_ERR_INVALID_ARGUMENT, "incremental mesh extraction failed: null triangle buffer");
+  hipError_t se = hipSuccess;
+  uint8_t* const base = noAllocScratch(c->device, &se);
+  if (!base) return hipFail(c, se, "hipGetSymbolAddress scratch");
+  DevHeader h{};
+  int rc = readHeader(c, &h);
+  if (rc) return rc;
+  const uint32_t nb = h.num_blocks < c->P.max_blocks ? h.num_blocks : c->P.max_blocks;
+  if (nb == 0) return VHM_OK;
+  const VolumeView V = view(c);
+  VHM_TRY(c, launchMarkRemesh(V, nb, c->remesh_flags, c->stream), "remesh mark kernel launch");
+  VHM_TRY(c, launchScan(c->remesh_flags, c->remesh_pos, nb, c->remesh_partials, c->stream, true),
+          "remesh scan launch");
+  uint32_t m = 0;
+  VHM_TRY(c, hipMemcpyAsync(&m, c->remesh_partials + (scanPartialsSize(nb) - 1), sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H remesh count");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (m > kNoAllocMaxListed) {
+    // not meshed: the marked blocks go back into the dirty masks
+    VHM_TRY(c, launchRestoreDirty(V, nb, c->remesh_flags, c->stream), "remesh restore kernel launch");
+    VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return fail(c, VHM_ERR_CAPACITY,
+                "incremental mesh extraction failed: " + std::to_string(m) +
+                    " blocks to re-mesh exceed the no-allocation limit of " + std::to_string(kNoAllocMaxListed) +
+                    "; the blocks stay dirty (use vhm_extract_mesh_incremental)");
+  }
+  VHM_TRY(c, launchCompactFlags(c->remesh_flags, c->remesh_pos, nb, c->remesh_list, c->remesh_dirs, c->stream),
+          "remesh compact launch");
+  *n_blocks_out = m;
+  if (m == 0) {
+    VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return VHM_OK;
+  }
+  uint32_t* const counter = reinterpret_cast<uint32_t*>(base + kNoAllocCounterOff);
+  uint32_t* const nbr = reinterpret_cast<uint32_t*>(base + kNoAllocNbrOff);
+  uint8_t* const obs_dirs = base + kNoAllocObsOff;
+  int32_t* const keys = reinterpret_cast<int32_t*>(base + kNoAllocKeysOff);
+  float* const cand = reinterpret_cast<float*>(base + kNoAllocCandOff);
+  const size_t kb = block_capacity < m ? block_capacity : m;
+  if (block_keys_out && kb) {
+    VHM_TRY(c, launchListKeys(V, c->remesh_list, m, keys, c->stream), "remesh key launch");
+    VHM_TRY(c, hipMemcpyAsync(block_keys_out, keys, 3 * kb * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H remesh keys");
+  }
+  if (block_dirs_out && kb)
+    VHM_TRY(c, hipMemcpyAsync(block_dirs_out, c->remesh_dirs, kb, hipMemcpyDeviceToHost, c->stream),
+            "hipMemcpy D2H remesh directions");
+  MeshArgs ma{};
+  ma.voxel = c->P.voxel_size;
+  ma.iso = iso;
+  ma.extract_tol = c->P.voxel_size * c->P.layer_extract_scale;
+  ma.extract_tol = ma.extract_tol < 1e-6f ? 1e-6f : ma.extract_tol;
+  VHM_TRY(c, launchZeroWords(counter, 1, c->stream), "mesh counter clear launch");
+  const uint32_t cap = static_cast<uint32_t>(tri_capacity < 0xFFFFFFFFu ? tri_capacity : 0xFFFFFFFFu);
+  VHM_TRY(c, launchMeshFused(V, c->remesh_list, m, ma, c->remesh_dirs, nbr, obs_dirs, counter, tris_out, tri_item_out,
+                             cap, cand, c->stream, kNoAllocGrid),
+          "fused mesh kernel launch");
+  uint32_t total = 0;
+  VHM_TRY(c, hipMemcpyAsync(&total, counter, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+          "hipMemcpy D2H triangle count");
+  VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *n_tris_out = total;
   return VHM_OK;
 }
 

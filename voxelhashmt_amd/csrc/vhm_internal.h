@@ -202,9 +202,16 @@ size_t scanPartialsSize(uint32_t n);
 // items in completion order; total: one zeroed word, receives the count.
 // nbr (8 n words) and observed_dirs (n bytes): scratch for the items'
 // source blocks.
+// max_grid: at most this many workgroups (cand holds kMeshFusedCandWords
+// floats per workgroup).
 hipError_t launchMeshFused(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                            const uint8_t* dirs, uint32_t* nbr, uint8_t* observed_dirs, uint32_t* total, float* tris,
-                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s);
+                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s,
+                           uint32_t max_grid = kMeshGridMax);
+// p[0..n) = 0 with a kernel (no memset node when captured).
+hipError_t launchZeroWords(uint32_t* p, uint32_t n, hipStream_t s);
+// Puts a marked remesh list back into the dirty masks (flags cleared).
+hipError_t launchRestoreDirty(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s);
 hipError_t launchMeshEmit(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                           const uint32_t* counts, const uint8_t* cube_counts, const uint32_t* offsets, float* tris,
                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s);

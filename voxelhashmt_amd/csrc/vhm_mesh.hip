@@ -1089,11 +1089,47 @@ static uint32_t meshGrid(uint32_t items) { return items < kMeshGridMax ? items :
 
 hipError_t launchMeshFused(const VolumeView& V, const uint32_t* list, uint32_t n, const MeshArgs& a,
                            const uint8_t* dirs, uint32_t* nbr, uint8_t* observed_dirs, uint32_t* total, float* tris,
-                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s) {
+                           uint32_t* tri_owner, uint32_t capacity, float* cand, hipStream_t s, uint32_t max_grid) {
   if (n == 0) return hipSuccess;
   meshNeighboursKernel<<<(8ull * n + 255) / 256, 256, 0, s>>>(V, list, n, nbr, observed_dirs, a.iso == 0.0f);
-  meshFusedKernel<<<meshGrid(kDirs * n), 256, 0, s>>>(V, list, n, a, dirs, nbr, observed_dirs, total, tris, tri_owner,
-                                                       capacity, cand);
+  uint32_t grid = meshGrid(kDirs * n);
+  grid = grid < max_grid ? grid : max_grid;
+  meshFusedKernel<<<grid, 256, 0, s>>>(V, list, n, a, dirs, nbr, observed_dirs, total, tris, tri_owner, capacity,
+                                       cand);
+  return hipGetLastError();
+}
+
+namespace {
+__global__ void zeroWordsKernel(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// A remesh list that is not meshed: every listed block's flags go back into
+// its dirty mask (a superset of what was cleared: the next call lists these
+// blocks and their -x/-y/-z neighbours) and the flags are cleared.
+__global__ __launch_bounds__(256) void restoreDirtyKernel(VolumeView V, uint32_t num_blocks,
+                                                          uint32_t* __restrict__ flags) {
+  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < num_blocks; p += gridDim.x * 256u) {
+    const uint32_t f = flags[p];
+    if (f) {
+      V.dirty[p] |= f;
+      flags[p] = 0u;
+    }
+  }
+}
+}  // namespace
+
+static uint32_t gridFor(uint32_t n);
+
+hipError_t launchZeroWords(uint32_t* p, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  zeroWordsKernel<<<(n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024, 256, 0, s>>>(p, n);
+  return hipGetLastError();
+}
+
+hipError_t launchRestoreDirty(const VolumeView& V, uint32_t num_blocks, uint32_t* flags, hipStream_t s) {
+  if (num_blocks == 0) return hipSuccess;
+  restoreDirtyKernel<<<gridFor(num_blocks), 256, 0, s>>>(V, num_blocks, flags);
   return hipGetLastError();
 }
 
