@@ -51,7 +51,8 @@ enum vhm_memkind { VHM_HOST = 0, VHM_DEVICE = 1 };
 
 /* Constant presets: the reference CPU constants (voxel_hash_tsdf.h:194-204,
  * reconstruction_backend.cpp:26-32) or its CUDA constants
- * (cuda_tsdf_integrator.cu:18-35). */
+ * (cuda_tsdf_integrator.cu:18-35).  The CUDA preset changes constants only:
+ * the algorithm is always the CPU path's. */
 enum vhm_preset { VHM_PRESET_REFERENCE_CPU = 0, VHM_PRESET_REFERENCE_CUDA = 1 };
 
 /* Pinhole intrinsics, as Intrinsics (math_types.h:200-207). */

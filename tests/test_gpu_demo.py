@@ -16,6 +16,7 @@ tsdf_voxel_hash_demo command line over libvhm) on the GPU.
   the reference's in tests/test_hostio.py).
 """
 import json
+import os
 import re
 import subprocess
 import sys
@@ -132,3 +133,24 @@ def test_dataset_mode_matches_reference_cpu_path(gpu, tmp_path):
     H.write_ply(tmp_path / "ref.ply", mesh)
     assert G.ply_digest(tmp_path / "m.ply") == G.ply_digest(tmp_path / "ref.ply")
     print(f"[demo] dataset: {final.group(0)}", flush=True)
+
+
+REF_VHM_DEMO = ROOT / "oracle" / "_ref" / "tsdf_voxel_hash_demo_vhm"
+
+
+@pytest.mark.skipif(not REF_VHM_DEMO.exists(), reason="oracle/_ref/tsdf_voxel_hash_demo_vhm not built")
+@pytest.mark.parametrize("case", sorted(GOLDEN))
+def test_reference_demo_on_libvhm_matches_reference_cpu_run(gpu, tmp_path, case):
+    """The drop-in: the reference's own main.cpp with its backend patched to
+    call libvhm (oracle/vhm_backend.sed), --backend gpu, gives the reference
+    CPU demo's per-frame counts, final counts and PLY faces."""
+    g = GOLDEN[case]
+    args = [str(REF_VHM_DEMO), "--synthetic", *g["args"], "--backend", "gpu", "--output", str(tmp_path / "m.ply")]
+    print(f"[dropin] {' '.join(args[1:])}", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300, env={**os.environ, "VHM_MAX_BLOCKS": "200000"})
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Integration backend: GPU (HIP, gfx950)" in r.stdout
+    got = G.parse_stdout(r.stdout)
+    assert got["frames"] == g["frames"]
+    assert got["final"] == g["final"]
+    assert G.ply_digest(tmp_path / "m.ply") == (g["ply_faces"], g["ply_digest"])
