@@ -21,9 +21,10 @@ the latter taking the voxels that project within 2 px of the top/left image
 border), algorithmic bytes = 96 B per update (3 slots x 16 B read +
 written), HIP events on the library's stream around the pair; avg_launch_ms
 is per frame.
-cpu_baseline: the reference's CPU path (oracle/_ref, compiled from its
-sources) or, if absent, the restated oracle, on frame 0 of the same workload,
-one core.
+cpu_baseline: frame 0 of the same workload on one host core: allocation and
+integration by the reference's own CPU code (oracle/_ref, compiled from its
+sources) after the restated filter, or, if that build is absent, the
+restated CPU path throughout.
 """
 from __future__ import annotations
 
@@ -46,7 +47,14 @@ def log(*a):
 
 
 def cpu_baseline(w, frame0_u16, gpu_updates_frame0: int) -> dict | None:
-    """Times the CPU path on frame 0 (filter + allocation + integration)."""
+    """Times the CPU path on frame 0 (filter + allocation + integration), one
+    core.  Where the reference's own CPU path is built (oracle/_ref, from its
+    sources), allocation and integration are its compiled code
+    (VoxelHashTSDF::integrate, voxel_hash_tsdf.h:125-131) on the frame
+    filtered by the restated filter: the reference build has its filter
+    constants compiled in (reconstruction_backend.cpp:26-32) and cannot run
+    the workload's (sigma_s 4.5 px, sigma_r 30 mm).  Otherwise the restated
+    path (the oracle) does all three."""
     import numpy as np
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O
@@ -57,25 +65,29 @@ def cpu_baseline(w, frame0_u16, gpu_updates_frame0: int) -> dict | None:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except (AttributeError, OSError):
         pass
-    if O.ref_available() and w.filter_sigma_spatial == 1.5 and w.filter_sigma_range == 0.0012:
-        kind = "reference"
-    else:
-        kind = "port"
     vp = vhm.default_params(w.voxel_size, w.truncation, filter_sigma_spatial=w.filter_sigma_spatial,
                             filter_sigma_range=w.filter_sigma_range)
-    ov = O.OracleVolume(O.oracle_params(vp))
+    op = O.oracle_params(vp)
     t0 = time.perf_counter()
-    if kind == "reference":
+    if O.ref_available():
+        filtered = O.oracle_filter(op, depth, w.K)
+        t1 = time.perf_counter()
         rv = O.RefVolume(w.voxel_size, w.truncation)
-        rv.integrate(depth, w.K, pose)
-        updates = gpu_updates_frame0
+        rv.integrate(filtered, w.K, pose, apply_filter=False)
+        t2 = time.perf_counter()
+        updates, kind = gpu_updates_frame0, "reference"
+        how = (f"filter (restated, config sigmas) {t1 - t0:.1f} s + allocation and integration by the reference's "
+               f"compiled CPU code {t2 - t1:.1f} s")
     else:
+        ov = O.OracleVolume(op)
         ov.integrate(depth, w.K, pose)
-        updates = ov.updates()
-    dt = time.perf_counter() - t0
+        t2 = time.perf_counter()
+        updates, kind = ov.updates(), "port"
+        how = "filter + allocation + integration, restated CPU path"
+    dt = t2 - t0
     return {"value": updates / dt, "unit": "voxel updates/s", "cores": 1, "kind": kind,
-            "sample": f"frame 0 of {w.name} ({w.K.width}x{w.K.height}): filter + allocation + integration, "
-                      f"{updates} updates in {dt:.1f} s on 1 host core (of {os.cpu_count()})",
+            "sample": f"frame 0 of {w.name} ({w.K.width}x{w.K.height}): {updates} updates in {dt:.1f} s on 1 host "
+                      f"core (of {os.cpu_count()}); {how}",
             "seconds": dt}
 
 
@@ -228,6 +240,38 @@ def bench_batched(args):
             if b.mesh_output(i, fetch=False)[1] > caps[i]:
                 overflow += 1
     achieved = BYTES_PER_UPDATE * tot["updates"] / max(tot["integrate_ms"] * 1e-3, 1e-12) / 1e9
+
+    # the same scenes and steps without batching: one scene after another
+    # through the one-volume path (vhm_integrate_u16 + the incremental mesh
+    # into a device buffer after every frame), frames resident in HBM
+    b.rewind()
+    b.synchronize()
+    seq_buf = torch.empty(max(caps) * 9, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, w in enumerate(ws):
+        v = b.volumes[i]
+        for step in range(K):
+            v.integrate(frames[i][step], w.K, w.poses[step])
+            v.extract_mesh_incremental(device_tris=seq_buf)
+        v.synchronize()
+    torch.cuda.synchronize()
+    seq_elapsed = time.perf_counter() - t0
+    seq_updates = sum(v.stats()["updates"] for v in b.volumes)
+    log(f"[bench-batched] sequential: {K} steps of {len(ws)} scenes one after another in {seq_elapsed * 1e3:.1f} ms")
+    assert seq_updates == updates, "the sequential run integrates the same voxel updates"
+    del seq_buf
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        i0 = min(range(len(ws)), key=lambda i: ws[i].K.width * ws[i].K.height)
+        f0 = frames[i0][0].cpu().numpy().view(np.uint16)
+        with vhm.Volume(params[i0]) as v0:
+            v0.integrate(frames[i0][0], ws[i0].K, ws[i0].poses[0])
+            u0 = v0.stats()["updates"]
+        log("[bench-batched] timing the CPU baseline ...")
+        cpu = cpu_baseline(ws[i0], f0, u0)
+        cpu.pop("seconds", None)
     out = {
         "metric": "voxel updates/s (% HBM peak)",
         "value": updates / elapsed,
@@ -252,15 +296,84 @@ def bench_batched(args):
         "stage_ms_direct_launches": {k: round(v, 3) for k, v in tot.items() if k.endswith("_ms")},
         "blocks_remeshed": tot["blocks_remeshed"],
         "mesh_capacity_overflows": overflow,
+        "sequential_one_volume": {"ms_per_step": round(seq_elapsed * 1e3 / K, 3),
+                                  "speedup_of_batching": round(seq_elapsed / elapsed, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "integrateBatchKernel+integrateDeferredBatchKernel",
                      "avg_launch_ms": round(tot["integrate_ms"] / K, 4),
                      "updates_per_launch": tot["updates"] / K},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     b.close()
+
+
+# Algorithmic FLOP of the denoiser per block (three 3x3x3 convolutions on the
+# 8^3 interior, 12->32->32->6): forward = 2 * 512 * 27 * (12*32 + 32*32 +
+# 32*6); a training step adds the three weight gradients (the forward's
+# FLOP again) and the data gradients of layers 3 and 2 (32*6 + 32*32 per
+# voxel and tap).
+DN_FWD_FLOP = 2 * 512 * 27 * (12 * 32 + 32 * 32 + 32 * 6)
+DN_TRAIN_FLOP = 2 * DN_FWD_FLOP + 2 * 512 * 27 * (32 * 6 + 32 * 32)
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def bench_denoiser(args):
+    """Denoiser training (the learned configuration's step): samples from
+    seeded small rooms in the plain-small style (input: 10 noisy frames,
+    target: 300 frames with the noise sigma halved), then K training steps
+    at batch 4096 (forward, backward, gradient reduction, Adam) over a
+    resident sample set, timed with the device idle before and after."""
+    import numpy as np
+    import torch
+    import voxelhashmt_amd as vhm
+    import voxelhashmt_amd.learned as L
+
+    ok, msg = vhm.device_available()
+    if not ok:
+        raise SystemExit(f"no GPU: {msg}")
+    batch = 4096
+    K = max(1, min(args.steps, 200))
+    W = max(1, min(args.warmup, 20))
+    dn = vhm.Denoiser(capacity=3 * batch, batch_capacity=batch)
+    dn.set_params(L.init_params(0))
+    L.build_training_set(dn, range(1000, 1020), blocks_per_scene=1000)
+    n = dn.count
+    log(f"[bench-denoiser] {msg}; {n} samples, batch {batch}, warmup {W}, steps {K}")
+    offs = np.random.default_rng(0).integers(0, n - batch + 1, size=W + K)
+    for i in range(W):
+        dn.train_step(int(offs[i]), batch, 1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [dn.train_step(int(offs[W + i]), batch, 1e-3) for i in range(K)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    flop = DN_TRAIN_FLOP * batch
+    tflops = flop * K / dt / 1e12
+    t1 = time.perf_counter()
+    dn.forward(0, batch, fetch=False)
+    torch.cuda.synchronize()
+    fwd_dt = time.perf_counter() - t1
+    out = {
+        "metric": "denoiser training blocks/s (% MFMA peak)",
+        "value": batch * K / dt,
+        "unit": "blocks/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": dt * 1e3 / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic (denoiser samples from 20 seeded plain-small rooms fused on the GPU)",
+        "config": {"workload": "denoiser_train_batch4096", "batch": batch, "samples": n,
+                   "network": "3x3x3 conv 12->32->32->6, ReLU", "optimizer": "Adam lr 1e-3"},
+        "loss_first": losses[0], "loss_last": losses[-1],
+        "forward_ms_per_batch": fwd_dt * 1e3,
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": BF16_DENSE_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / BF16_DENSE_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "training step (forward, backward, reduce, Adam)",
+                     "flop_per_step": flop},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    dn.close()
 
 
 def main():
@@ -269,7 +382,7 @@ def main():
     ap.add_argument("--steps", type=int, default=480)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="plain_scale",
-                    choices=["plain_scale", "plain_small", "batched", "point_cloud"])
+                    choices=["plain_scale", "plain_small", "batched", "point_cloud", "denoiser_train"])
     ap.add_argument("--max-blocks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
@@ -291,6 +404,9 @@ def main():
         return
     if args.workload == "point_cloud":
         bench_point_cloud(args)
+        return
+    if args.workload == "denoiser_train":
+        bench_denoiser(args)
         return
 
     import numpy as np
