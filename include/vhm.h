@@ -279,6 +279,10 @@ int vhm_denoiser_samples(vhm_denoiser* d, size_t offset, size_t n, uint16_t* x_o
 /* Device check of the v_mfma_f32_16x16x16_bf16 operand layout the kernels
  * assume: out[4 * lane + r] = D element held by (lane, r). */
 int vhm_mfma_layout_probe(float* out256);
+/* The same for v_mfma_f32_16x16x32_bf16 (gfx950): a (16 x 32) and b (32 x 16)
+ * row-major (values exact in bf16) placed as the kernels place operands;
+ * out[4 * lane + r] = D element held by (lane, r). */
+int vhm_mfma_layout_probe_k32(const float* a16x32, const float* b32x16, float* out256);
 
 /* --- batched multi-scene fusion (no reference counterpart) -------------- */
 

@@ -25,6 +25,20 @@ def test_mfma_operand_layout(gpu):
             assert out[lane, r] == i + n / 16.0, (lane, r, out[lane, r])
 
 
+def test_mfma_k32_operand_layout(gpu):
+    """v_mfma_f32_16x16x32_bf16 with operands placed as the kernels place
+    them equals A B on small integers (exact in bf16 and fp32)."""
+    rng = np.random.default_rng(1)
+    a = rng.integers(-3, 4, size=(16, 32)).astype(np.float32)
+    b = rng.integers(-3, 4, size=(32, 16)).astype(np.float32)
+    out = np.zeros(256, np.float32)
+    assert vhm.lib().vhm_mfma_layout_probe_k32(a.ctypes.data, b.ctypes.data, out.ctypes.data) == 0
+    d = a @ b
+    for lane in range(64):
+        for r in range(4):
+            assert out[4 * lane + r] == d[4 * (lane // 16) + r, lane % 16], (lane, r)
+
+
 def _scene_volume(seed=5, frames=3):
     w = synth.tiny(n_frames=frames, seed=seed)
     f = synth.render(w.scene, w.K, w.poses, w.noise)

@@ -137,6 +137,7 @@ _SIGS = {
     "vhm_extract_mesh_incremental_noalloc": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_size_t,
                                                        C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t,
                                                        C.POINTER(C.c_size_t)]),
+    "vhm_mfma_layout_probe_k32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "vhm_block_digests": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "vhm_batch_mesh_items": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
                                        C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t]),

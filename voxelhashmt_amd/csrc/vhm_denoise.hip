@@ -10,21 +10,26 @@
 //   h2 = relu(conv3x3x3(h1) same,  32->32)     [8][8][8][32]
 //   y  = X.tsdf[centre] + conv3x3x3(h2) same, 32->6
 // A convolution is an implicit GEMM: M = 512 output voxels, N = output
-// channels, K = 27 taps x input channels, on v_mfma_f32_16x16x16_bf16
-// (16x16 output tile per wavefront, K = 16 per instruction: 16 input
-// channels of one tap).  Operand layout of the instruction, per lane l:
-//   A[i][k]: i = l % 16, k = 4 (l / 16) + j   (4 bf16 per lane, j = 0..3)
-//   B[k][n]: n = l % 16, k = 4 (l / 16) + j
+// channels, K = 27 taps x input channels, on gfx950's
+// v_mfma_f32_16x16x32_bf16 (mfma32: a 16x16 output tile per instruction and
+// wavefront, K = 32: one tap of 32 input channels, or two taps of conv1's
+// 16).  Operand layout of the instruction, per lane l:
+//   A[i][k]: i = l % 16, k = 8 (l / 16) + j   (8 bf16 per lane, j = 0..7)
+//   B[k][n]: n = l % 16, k = 8 (l / 16) + j
 //   D[i][n]: n = l % 16, i = 4 (l / 16) + r   (4 fp32 per lane, r = 0..3)
-// (vhm_mfma_layout_probe checks this on the device).
+// (vhm_mfma_layout_probe_k32 checks this on the device).
 //
-// Forward: one workgroup (4 wavefronts) per block.  LDS holds the padded
-// input grid and the padded h1 grid (ping-pong, 128 KiB); weights stream
-// from L2 ([tap][n][c] bf16, 8 bytes per lane per instruction).
-// Backward (training): per-layer kernels over blocks with the bf16
-// activations in HBM; weight gradients accumulate in each wavefront's
-// registers across the blocks it visits and are reduced in a fixed order
-// (deterministic); the data gradient is the same convolution with the
+// Forward: one workgroup of 16 wavefronts per block (4 per SIMD).  LDS holds
+// the padded input grid and the padded h1 grid (ping-pong, 128 KiB); a lane
+// reads 8 consecutive channels (16 bytes) of A from LDS and of B (weights,
+// [tap][n][c] bf16) from L2 per instruction.
+// Backward (training): per-layer kernels of 8 wavefronts, persistent over
+// blocks, with the bf16 activations in HBM.  Weight gradients are GEMMs with
+// K = voxels: A from a channel-major copy of the layer input (a lane's 8
+// voxels are one x-row, one 16-byte LDS read), B gathered once per 32
+// voxels and reused for every tap; they accumulate in each wavefront's
+// registers across its blocks and are reduced in a fixed order
+// (deterministic); the data gradient is the forward convolution with the
 // flipped, transposed weights.
 #include <cstring>
 
@@ -35,7 +40,17 @@
 namespace vhm {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x32_bf16: D (16 x 16 fp32) += A (16 x 32) B (32 x 16), bf16
+// operands given as their bit patterns; per lane l, A[l % 16][8 (l / 16) + j]
+// and B[8 (l / 16) + j][l % 16] (j = 0..7), D[4 (l / 16) + r][l % 16].
+__device__ __forceinline__ v4f mfma32(v8s a, v8s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
+                                                 0);
+}
 
 constexpr int kDnIn = 16;      // input channels (12 used)
 constexpr int kDnHid = 32;
@@ -68,23 +83,37 @@ __device__ __forceinline__ int padIndex(int m, int t) {
 
 // acc[i] (i < MT) += conv over the padded LDS grid `in` ([1000][CIN] bf16)
 // for this wavefront's tiles: n-tile `nt` (16 output channels, weights
-// W [27][NTOT][CIN]) and m-tiles mt = mt0 + i * mstride.
+// W [27][NTOT][CIN]) and m-tiles mt = mt0 + i * mstride, on
+// v_mfma_f32_16x16x32_bf16 (mfma32): K = 32 per instruction, one tap of 32
+// input channels, or for CIN = 16 two taps (k-groups 0-1: tap
+// 2 tp, 2-3: tap 2 tp + 1; the 28th tap is a zero operand).  A lane loads
+// 8 consecutive channels (16 bytes) of A and of B.
 template <int CIN, int NTOT, int MT>
-__device__ __forceinline__ void convTiles(const uint16_t* in, const uint16_t* __restrict__ W, int nt, int mt0,
-                                          int mstride, v4f (&acc)[MT]) {
+__device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* __restrict__ W, int nt, int mt0,
+                                            int mstride, v4f (&acc)[MT]) {
+  static_assert(CIN == 16 || CIN == 32, "16 or 32 input channels");
   const int lane = threadIdx.x & 63;
-  const int li = lane & 15, lk = (lane >> 4) * 4;
-  for (int t = 0; t < 27; ++t) {
+  const int li = lane & 15, kg = lane >> 4;
+  constexpr int kSteps = CIN == 32 ? 27 : 14;
+  for (int st = 0; st < kSteps; ++st) {
+    int t, c0;
+    bool live = true;
+    if constexpr (CIN == 32) {
+      t = st;
+      c0 = kg * 8;
+    } else {
+      t = 2 * st + (kg >> 1);
+      c0 = (kg & 1) * 8;
+      live = t < 27;
+      t = live ? t : 26;
+    }
+    v8s b = *reinterpret_cast<const v8s*>(W + (static_cast<size_t>(t) * NTOT + nt * 16 + li) * CIN + c0);
+    if (!live) b = v8s{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int cb = 0; cb < CIN / 16; ++cb) {
-      const v4s b = *reinterpret_cast<const v4s*>(W + (static_cast<size_t>(t) * NTOT + nt * 16 + li) * CIN +
-                                                  cb * 16 + lk);
-#pragma unroll
-      for (int i = 0; i < MT; ++i) {
-        const int m = (mt0 + i * mstride) * 16 + li;
-        const v4s a = *reinterpret_cast<const v4s*>(in + padIndex(m, t) * CIN + cb * 16 + lk);
-        acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
-      }
+    for (int i = 0; i < MT; ++i) {
+      const int m = (mt0 + i * mstride) * 16 + li;
+      const v8s av = *reinterpret_cast<const v8s*>(in + padIndex(m, t) * CIN + c0);
+      acc[i] = mfma32(av, b, acc[i]);
     }
   }
 }
@@ -92,7 +121,7 @@ __device__ __forceinline__ void convTiles(const uint16_t* in, const uint16_t* __
 // Zero a padded grid's 488 halo voxels (the 8^3 interior is overwritten).
 template <int C>
 __device__ void zeroHalo(uint16_t* g) {
-  for (int r = threadIdx.x; r < kGridVox; r += 256) {
+  for (int r = threadIdx.x; r < kGridVox; r += blockDim.x) {
     const int x = r % kGrid, y = (r / kGrid) % kGrid, z = r / 100;
     if (x == 0 || y == 0 || z == 0 || x == 9 || y == 9 || z == 9)
       for (int c = 0; c < C; ++c) g[r * C + c] = 0;
@@ -279,56 +308,62 @@ struct DnFwdArgs {
   float inv_count;   // 1 / max(sum M, 1)
 };
 
-__global__ __launch_bounds__(256) void dnForwardKernel(DnFwdArgs a) {
+// One workgroup of 16 wavefronts per block (4 per SIMD: LDS holds two padded
+// grids, 128 KiB, so one workgroup per CU).  conv1 and conv2: 64 output
+// tiles (2 n-tiles x 32 m-tiles), wavefront w takes n-tile w & 1 and m-tiles
+// (w >> 1) + 8 i; conv3: 32 tiles, m-tiles w and w + 16.
+constexpr int kFwdThreads = 1024;
+constexpr int kFwdWaves = kFwdThreads / 64;
+
+__global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
   __shared__ __align__(16) uint16_t gridA[kGridVox * kDnHid];  // X (16 ch) then h2
   __shared__ __align__(16) uint16_t gridB[kGridVox * kDnHid];  // h1
-  __shared__ float red[4];
+  __shared__ float red[kFwdWaves];
   const int b = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint16_t* Xb = a.X + static_cast<size_t>(b) * kGridVox * kDnIn;
-  {  // X -> gridA (16 channels, 8-byte copies)
-    const uint2* s = reinterpret_cast<const uint2*>(Xb);
-    uint2* d = reinterpret_cast<uint2*>(gridA);
-    for (int i = threadIdx.x; i < kGridVox * kDnIn / 4; i += 256) d[i] = s[i];
+  {  // X -> gridA (16 channels, 16-byte copies)
+    const uint4* src = reinterpret_cast<const uint4*>(Xb);
+    uint4* dst = reinterpret_cast<uint4*>(gridA);
+    for (int i = threadIdx.x; i < kGridVox * kDnIn / 8; i += kFwdThreads) dst[i] = src[i];
   }
   zeroHalo<kDnHid>(gridB);
   __syncthreads();
-  // conv1 (valid on X): 2 n-tiles x 32 m-tiles; wave: n-tile wave % 2, m-tiles wave / 2 + 2 i
-  {
-    v4f acc[16];
+  {  // conv1 (valid on X)
+    v4f acc[4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles<kDnIn, 32, 16>(gridA, a.w.W1t, wave & 1, wave >> 1, 2, acc);
-    storeHidden<16>(acc, a.w.b + kOffB1, wave & 1, wave >> 1, 2, gridB,
-                    a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+    for (int i = 0; i < 4; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles32<kDnIn, 32, 4>(gridA, a.w.W1t, wave & 1, wave >> 1, 8, acc);
+    storeHidden<4>(acc, a.w.b + kOffB1, wave & 1, wave >> 1, 8, gridB,
+                   a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
   __syncthreads();
   zeroHalo<kDnHid>(gridA);  // X is dead; gridA becomes h2
   __syncthreads();
-  {
-    v4f acc[16];
+  {  // conv2 (same on h1)
+    v4f acc[4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles<kDnHid, 32, 16>(gridB, a.w.W2t, wave & 1, wave >> 1, 2, acc);
-    storeHidden<16>(acc, a.w.b + kOffB2, wave & 1, wave >> 1, 2, gridA,
-                    a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+    for (int i = 0; i < 4; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles32<kDnHid, 32, 4>(gridB, a.w.W2t, wave & 1, wave >> 1, 8, acc);
+    storeHidden<4>(acc, a.w.b + kOffB2, wave & 1, wave >> 1, 8, gridA,
+                   a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
   __syncthreads();
-  // conv3: 1 n-tile (6 of 16 used) x 32 m-tiles; wave: m-tiles wave + 4 i
+  // conv3: 1 n-tile (6 of 16 used) x 32 m-tiles
   float lsum = 0.0f;
   {
-    v4f acc[8];
+    v4f acc[2];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles<kDnHid, 16, 8>(gridA, a.w.W3t, 0, wave, 4, acc);
+    for (int i = 0; i < 2; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    convTiles32<kDnHid, 16, 2>(gridA, a.w.W3t, 0, wave, kFwdWaves, acc);
     const int n = lane & 15;
     if (n < kDnOut) {
       const float bb = a.w.b[kOffB3 + n];
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = (wave + 4 * i) * 16 + (lane >> 4) * 4 + r;
+          const int m = (wave + kFwdWaves * i) * 16 + (lane >> 4) * 4 + r;
           const float x0 = bf2f(Xb[interiorIndex(m) * kDnIn + n]);
           const float y = x0 + (acc[i][r] + bb);
           const size_t o = (static_cast<size_t>(b) * 512 + m) * kDnOut + n;
@@ -342,45 +377,125 @@ __global__ __launch_bounds__(256) void dnForwardKernel(DnFwdArgs a) {
         }
     }
   }
-  if (a.loss) {  // workgroup partial of the loss numerator
+  if (a.loss) {  // workgroup partial of the loss numerator, in a fixed order
     for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
     if (lane == 0) red[wave] = lsum;
     __syncthreads();
-    if (threadIdx.x == 0) a.loss[b] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (threadIdx.x == 0) {
+      float r8[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) r8[i] = red[2 * i] + red[2 * i + 1];
+      a.loss[b] = ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+    }
   }
 }
 
 // ------------------------------------------------------------- backward --
 
-// Weight-gradient tiles over one block: dW[t][c][o] += sum_v in[pad(v, t)][c]
-// * dout[v][o] for this wavefront's taps t = wave + 4 j.  A = input^T
-// (i = c), B = dout (n = o), K = voxels (16 per instruction).
-template <int CIN, int COUTP, int NTAPS>
-__device__ __forceinline__ void accumulateDW(const uint16_t* in /*padded [1000][CIN] or X*/,
-                                             const uint16_t* dout /*[512][COUTP] bf16 LDS*/, int wave,
-                                             v4f (&acc)[NTAPS][CIN / 16][COUTP / 16]) {
+// ------------------------------------------------------------- backward --
+//
+// Per layer one kernel of kBwdThreads lanes per workgroup, persistent over
+// blocks (workgroup g takes blocks g, g + groups, ...), accumulating its
+// share of the weight gradient in registers and writing one partial row;
+// dnReduceKernel sums the rows in a fixed order (deterministic gradients).
+
+constexpr int kBwdThreads = 512;
+constexpr int kBwdWaves = kBwdThreads / 64;
+constexpr int kBwdTaps = (27 + kBwdWaves - 1) / kBwdWaves;  // taps per wavefront: t = wave + kBwdWaves j
+
+// Weight-gradient tiles over one block, on mfma32: dW[t][c][o] += sum over
+// the 512 voxels v of in[pad(v, t)][c] * dout[v][o] for this wavefront's taps.
+// A (i = c, k = v) comes from the channel-major padded grid inT ([CIN][1000]):
+// the 8 voxels of a lane's k-group are one x-row, so its 8 values are one
+// (unaligned) 16-byte LDS read.  B (k = v, n = o) is gathered once per 32
+// voxels from dout (channels-last, [512][COUTP], or its padded grid's interior
+// when kPadded) and reused for every tap and input-channel tile.
+template <int CIN, int COUTP, bool kPadded>
+__device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16_t* dout, int wave,
+                                               v4f (&acc)[kBwdTaps][CIN / 16][COUTP / 16]) {
   const int lane = threadIdx.x & 63;
-  const int li = lane & 15, lk = (lane >> 4) * 4;
-  for (int kb = 0; kb < 32; ++kb) {  // 16 voxels per step
+  const int li = lane & 15, kg = lane >> 4;
+  for (int kb = 0; kb < 16; ++kb) {
+    const int v0 = kb * 32 + kg * 8;  // voxels v0 .. v0 + 7: x = 0..7 of row (y, z)
+    const int y = (v0 >> 3) & 7, z = v0 >> 6;
+    v8s bv[COUTP / 16];
 #pragma unroll
-    for (int j = 0; j < NTAPS; ++j) {
-      const int t = wave + 4 * j;
+    for (int ot = 0; ot < COUTP / 16; ++ot)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int v = v0 + e;
+        bv[ot][e] = static_cast<short>(dout[(kPadded ? interiorIndex(v) : v) * COUTP + ot * 16 + li]);
+      }
+#pragma unroll
+    for (int j = 0; j < kBwdTaps; ++j) {
+      const int t = wave + kBwdWaves * j;
       if (t >= 27) break;
+      const int kx = t % 3, ky = (t / 3) % 3, kz = t / 9;
+      const int p0 = ((z + kz) * kGrid + (y + ky)) * kGrid + kx;
 #pragma unroll
       for (int ct = 0; ct < CIN / 16; ++ct) {
-        v4s av;
+        v8s av;
+        __builtin_memcpy(&av, inT + (ct * 16 + li) * kGridVox + p0, sizeof(av));
 #pragma unroll
-        for (int e = 0; e < 4; ++e) av[e] = static_cast<short>(in[padIndex(kb * 16 + lk + e, t) * CIN + ct * 16 + li]);
-#pragma unroll
-        for (int ot = 0; ot < COUTP / 16; ++ot) {
-          v4s bv;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) bv[e] = static_cast<short>(dout[(kb * 16 + lk + e) * COUTP + ot * 16 + li]);
-          acc[j][ct][ot] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j][ct][ot], 0, 0, 0);
-        }
+        for (int ot = 0; ot < COUTP / 16; ++ot) acc[j][ct][ot] = mfma32(av, bv[ot], acc[j][ct][ot]);
       }
     }
   }
+}
+
+template <int CIN, int COUTP>
+__device__ __forceinline__ void zeroDW(v4f (&acc)[kBwdTaps][CIN / 16][COUTP / 16]) {
+#pragma unroll
+  for (int j = 0; j < kBwdTaps; ++j)
+#pragma unroll
+    for (int c = 0; c < CIN / 16; ++c)
+#pragma unroll
+      for (int o = 0; o < COUTP / 16; ++o) acc[j][c][o] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+// Writes the wavefront's weight-gradient tiles into partial row P at offset
+// off ([27][cin][cout] of the master layout): input channels c < cin and
+// output channels o < cout are real.
+template <int CIN, int COUTP>
+__device__ __forceinline__ void storeDW(const v4f (&acc)[kBwdTaps][CIN / 16][COUTP / 16], float* P, int off, int cin,
+                                        int cout, int wave) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < kBwdTaps; ++j) {
+    const int t = wave + kBwdWaves * j;
+    if (t >= 27) break;
+#pragma unroll
+    for (int ct = 0; ct < CIN / 16; ++ct)
+#pragma unroll
+      for (int ot = 0; ot < COUTP / 16; ++ot)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int c = ct * 16 + (lane >> 4) * 4 + r, o = ot * 16 + (lane & 15);
+          if (c < cin && o < cout) P[off + (t * cin + c) * cout + o] = acc[j][ct][ot][r];
+        }
+  }
+}
+
+// Bias gradient of one block: for channel ch < C, the sum over the 512
+// voxels of get(v, ch), as kBwdThreads / CS partial sums (CS = channel slots,
+// a power of two >= C: lane t takes channel t % CS and voxels
+// (t / CS) * (512 CS / kBwdThreads) ...) combined in order by lane ch, which
+// adds it to db.  Workgroup-wide; part: kBwdThreads floats of LDS.
+template <int C, int CS, class Get>
+__device__ __forceinline__ void biasGrad(Get get, float* part, float& db) {
+  constexpr int kParts = kBwdThreads / CS, kSpan = 512 / kParts;
+  const int ch = threadIdx.x % CS, pt = threadIdx.x / CS;
+  float s = 0.0f;
+  if (ch < C)
+    for (int q = 0; q < kSpan; ++q) s += get(pt * kSpan + q, ch);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float t = 0.0f;
+    for (int i = 0; i < kParts; ++i) t += part[i * CS + threadIdx.x];
+    db += t;
+  }
+  __syncthreads();  // part is reused
 }
 
 struct DnBwdArgs {
@@ -395,205 +510,170 @@ struct DnBwdArgs {
   float* partial;        // [gridDim.x][kNumParams]
 };
 
-// Loads a [512][C] bf16 activation into the interior of a padded grid.
+// Loads a [512][C] bf16 activation into the interior of a channel-major
+// padded grid ([C][1000]).
+template <int C>
+__device__ void loadInteriorT(const uint16_t* __restrict__ src, uint16_t* gridT) {
+  for (int i = threadIdx.x; i < 512 * C / 4; i += kBwdThreads) {
+    const int m = (i * 4) / C, c = (i * 4) % C;
+    const uint2 q = *reinterpret_cast<const uint2*>(src + m * C + c);
+    const int r = interiorIndex(m);
+    gridT[(c + 0) * kGridVox + r] = static_cast<uint16_t>(q.x & 0xffffu);
+    gridT[(c + 1) * kGridVox + r] = static_cast<uint16_t>(q.x >> 16);
+    gridT[(c + 2) * kGridVox + r] = static_cast<uint16_t>(q.y & 0xffffu);
+    gridT[(c + 3) * kGridVox + r] = static_cast<uint16_t>(q.y >> 16);
+  }
+}
+
+// Zeroes the halo of a channel-major padded grid ([C][1000]).
+template <int C>
+__device__ void zeroHaloT(uint16_t* gridT) {
+  for (int r = threadIdx.x; r < kGridVox; r += kBwdThreads) {
+    const int x = r % kGrid, y = (r / kGrid) % kGrid, z = r / 100;
+    if (x == 0 || y == 0 || z == 0 || x == 9 || y == 9 || z == 9)
+      for (int c = 0; c < C; ++c) gridT[c * kGridVox + r] = 0;
+  }
+}
+
+// Loads a [512][C] bf16 activation into the interior of a channels-last
+// padded grid ([1000][C]).
 template <int C>
 __device__ void loadInterior(const uint16_t* __restrict__ src, uint16_t* grid) {
-  for (int i = threadIdx.x; i < 512 * C / 4; i += 256) {
+  for (int i = threadIdx.x; i < 512 * C / 4; i += kBwdThreads) {
     const int m = (i * 4) / C, c = (i * 4) % C;
     *reinterpret_cast<uint2*>(grid + interiorIndex(m) * C + c) = *reinterpret_cast<const uint2*>(src + m * C + c);
   }
 }
 
-// Layer 3: dW3, db3 and dA2 = (conv(dY_pad, W3 flipped^T)) * (h2 > 0).
-__global__ __launch_bounds__(256) void dnBackward3Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t h2p[kGridVox * kDnHid];
+// Data gradient through a layer: g = conv(dout padded, flipped weights),
+// masked by the layer input's ReLU (in > 0, read from the channel-major
+// grid inT), written as bf16 [512][32].  16 m-tiles x 2 n-tiles: wavefront w
+// takes n-tile w & 1 and m-tiles (w >> 1) + 4 i.
+template <int CIN>
+__device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t* __restrict__ Wb,
+                                         const uint16_t* inT, uint16_t* __restrict__ out, int wave) {
+  const int lane = threadIdx.x & 63;
+  v4f g[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) g[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  convTiles32<CIN, 32, 8>(doutPad, Wb, wave & 1, wave >> 1, 4, g);
+  const int n = (wave & 1) * 16 + (lane & 15);
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = ((wave >> 1) + 4 * i) * 16 + (lane >> 4) * 4 + r;
+      const bool on = inT[n * kGridVox + interiorIndex(m)] != 0;
+      out[m * kDnHid + n] = on ? f2bf(g[i][r]) : 0;
+    }
+}
+
+// Layer 3: dW3, db3 and dA2 = conv(dY_pad, W3 flipped^T) * (h2 > 0).
+__global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
+  __shared__ __align__(16) uint16_t h2T[kDnHid * kGridVox];       // h2, channel-major padded
   __shared__ __align__(16) uint16_t dyp[kGridVox * kDnOutPad];   // padded dY (bf16, 16 ch)
   __shared__ __align__(16) uint16_t dyc[512 * kDnOutPad];        // dY [512][16]
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  v4f acc[7][2][1];
-#pragma unroll
-  for (int j = 0; j < 7; ++j)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) acc[j][c][0] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  __shared__ float part[kBwdThreads];
+  const int wave = threadIdx.x >> 6;
+  v4f acc[kBwdTaps][2][1];
+  zeroDW<kDnHid, kDnOutPad>(acc);
   float db = 0.0f;
-  zeroHalo<kDnHid>(h2p);
+  zeroHaloT<kDnHid>(h2T);
   zeroHalo<kDnOutPad>(dyp);
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
-    loadInterior<kDnHid>(a.H2 + static_cast<size_t>(b) * 512 * kDnHid, h2p);
-    for (int i = threadIdx.x; i < 512 * kDnOutPad; i += 256) {
+    loadInteriorT<kDnHid>(a.H2 + static_cast<size_t>(b) * 512 * kDnHid, h2T);
+    const float* dYb = a.dY + static_cast<size_t>(b) * 512 * kDnOut;
+    for (int i = threadIdx.x; i < 512 * kDnOutPad; i += kBwdThreads) {
       const int m = i / kDnOutPad, o = i % kDnOutPad;
-      const float g = o < kDnOut ? a.dY[(static_cast<size_t>(b) * 512 + m) * kDnOut + o] : 0.0f;
-      const uint16_t h = f2bf(g);
+      const uint16_t h = f2bf(o < kDnOut ? dYb[m * kDnOut + o] : 0.0f);
       dyc[i] = h;
       dyp[interiorIndex(m) * kDnOutPad + o] = h;
     }
     __syncthreads();
-    // db3 (fp32 from dY directly): lanes 0..5 of wave 0
-    if (wave == 0 && lane < kDnOut)
-      for (int m = 0; m < 512; ++m) db += a.dY[(static_cast<size_t>(b) * 512 + m) * kDnOut + lane];
-    accumulateDW<kDnHid, kDnOutPad, 7>(h2p, dyc, wave, acc);
-    // dA2 = conv(dyp, W3b) masked by h2 > 0: 2 n-tiles x 32 m-tiles
-    v4f g[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) g[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles<kDnOutPad, 32, 16>(dyp, a.w.W3b, wave & 1, wave >> 1, 2, g);
-    const int n = (wave & 1) * 16 + (lane & 15);
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = ((wave >> 1) + 2 * i) * 16 + (lane >> 4) * 4 + r;
-        const bool on = h2p[interiorIndex(m) * kDnHid + n] != 0;
-        a.dA2[(static_cast<size_t>(b) * 512 + m) * kDnHid + n] = on ? f2bf(g[i][r]) : 0;
-      }
+    // db3 in fp32 from dY
+    biasGrad<kDnOut, 8>([&](int v, int ch) { return dYb[v * kDnOut + ch]; }, part, db);
+    accumulateDW32<kDnHid, kDnOutPad, false>(h2T, dyc, wave, acc);
+    dataGrad<kDnOutPad>(dyp, a.w.W3b, h2T, a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, wave);
   }
-  // per-workgroup partial gradient: dW3[t][c][o] at kOffW3 + (t*32 + c)*6 + o
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
-#pragma unroll
-  for (int j = 0; j < 7; ++j) {
-    const int t = wave + 4 * j;
-    if (t >= 27) break;
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int c = ct * 16 + (lane >> 4) * 4 + r, o = lane & 15;
-        if (o < kDnOut) P[kOffW3 + (t * 32 + c) * 6 + o] = acc[j][ct][0][r];
-      }
-  }
-  if (wave == 0 && lane < kDnOut) P[kOffB3 + lane] = db;
+  storeDW<kDnHid, kDnOutPad>(acc, P, kOffW3, kDnHid, kDnOut, wave);
+  if (threadIdx.x < kDnOut) P[kOffB3 + threadIdx.x] = db;
 }
 
 // Layer 2: dW2, db2 and dA1 = conv(dA2_pad, W2 flipped^T) * (h1 > 0).
-__global__ __launch_bounds__(256) void dnBackward2Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t h1p[kGridVox * kDnHid];
-  __shared__ __align__(16) uint16_t dap[kGridVox * kDnHid];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  v4f acc[7][2][2];
-#pragma unroll
-  for (int j = 0; j < 7; ++j)
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int o = 0; o < 2; ++o) acc[j][c][o] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+__global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
+  __shared__ __align__(16) uint16_t h1T[kDnHid * kGridVox];  // h1, channel-major padded
+  __shared__ __align__(16) uint16_t dap[kGridVox * kDnHid];  // dA2, channels-last padded
+  __shared__ float part[kBwdThreads];
+  const int wave = threadIdx.x >> 6;
+  v4f acc[kBwdTaps][2][2];
+  zeroDW<kDnHid, kDnHid>(acc);
   float db = 0.0f;
-  zeroHalo<kDnHid>(h1p);
+  zeroHaloT<kDnHid>(h1T);
   zeroHalo<kDnHid>(dap);
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
-    loadInterior<kDnHid>(a.H1 + static_cast<size_t>(b) * 512 * kDnHid, h1p);
+    loadInteriorT<kDnHid>(a.H1 + static_cast<size_t>(b) * 512 * kDnHid, h1T);
     loadInterior<kDnHid>(a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, dap);
     __syncthreads();
-    if (wave == 0 && lane < kDnHid)
-      for (int m = 0; m < 512; ++m) db += bf2f(dap[interiorIndex(m) * kDnHid + lane]);
-    // dW2 over voxels: A = h1p patches (i = c), B = dA2 (n = o) read from the padded grid interior
-    {
-      const int li = lane & 15, lk = (lane >> 4) * 4;
-      for (int kb = 0; kb < 32; ++kb) {
-#pragma unroll
-        for (int j = 0; j < 7; ++j) {
-          const int t = wave + 4 * j;
-          if (t >= 27) break;
-#pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
-            v4s av;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) av[e] = static_cast<short>(h1p[padIndex(kb * 16 + lk + e, t) * kDnHid + ct * 16 + li]);
-#pragma unroll
-            for (int ot = 0; ot < 2; ++ot) {
-              v4s bv;
-#pragma unroll
-              for (int e = 0; e < 4; ++e)
-                bv[e] = static_cast<short>(dap[interiorIndex(kb * 16 + lk + e) * kDnHid + ot * 16 + li]);
-              acc[j][ct][ot] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j][ct][ot], 0, 0, 0);
-            }
-          }
-        }
-      }
-    }
-    v4f g[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) g[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles<kDnHid, 32, 16>(dap, a.w.W2b, wave & 1, wave >> 1, 2, g);
-    const int n = (wave & 1) * 16 + (lane & 15);
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = ((wave >> 1) + 2 * i) * 16 + (lane >> 4) * 4 + r;
-        const bool on = h1p[interiorIndex(m) * kDnHid + n] != 0;
-        a.dA1[(static_cast<size_t>(b) * 512 + m) * kDnHid + n] = on ? f2bf(g[i][r]) : 0;
-      }
+    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(dap[interiorIndex(v) * kDnHid + ch]); }, part, db);
+    accumulateDW32<kDnHid, kDnHid, true>(h1T, dap, wave, acc);
+    dataGrad<kDnHid>(dap, a.w.W2b, h1T, a.dA1 + static_cast<size_t>(b) * 512 * kDnHid, wave);
   }
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
-#pragma unroll
-  for (int j = 0; j < 7; ++j) {
-    const int t = wave + 4 * j;
-    if (t >= 27) break;
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-      for (int ot = 0; ot < 2; ++ot)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int c = ct * 16 + (lane >> 4) * 4 + r, o = ot * 16 + (lane & 15);
-          P[kOffW2 + (t * 32 + c) * 32 + o] = acc[j][ct][ot][r];
-        }
-  }
-  if (wave == 0 && lane < kDnHid) P[kOffB2 + lane] = db;
+  storeDW<kDnHid, kDnHid>(acc, P, kOffW2, kDnHid, kDnHid, wave);
+  if (threadIdx.x < kDnHid) P[kOffB2 + threadIdx.x] = db;
 }
 
 // Layer 1: dW1, db1 from X and dA1.
-__global__ __launch_bounds__(256) void dnBackward1Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t xg[kGridVox * kDnIn];
-  __shared__ __align__(16) uint16_t da[512 * kDnHid];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  v4f acc[7][1][2];
-#pragma unroll
-  for (int j = 0; j < 7; ++j)
-#pragma unroll
-    for (int o = 0; o < 2; ++o) acc[j][0][o] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+__global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
+  __shared__ __align__(16) uint16_t xT[kDnIn * kGridVox];  // X, channel-major (halo included)
+  __shared__ __align__(16) uint16_t da[512 * kDnHid];      // dA1 [512][32]
+  __shared__ float part[kBwdThreads];
+  const int wave = threadIdx.x >> 6;
+  v4f acc[kBwdTaps][1][2];
+  zeroDW<kDnIn, kDnHid>(acc);
   float db = 0.0f;
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
     {
-      const uint2* s = reinterpret_cast<const uint2*>(a.X + static_cast<size_t>(b) * kGridVox * kDnIn);
-      uint2* d = reinterpret_cast<uint2*>(xg);
-      for (int i = threadIdx.x; i < kGridVox * kDnIn / 4; i += 256) d[i] = s[i];
-      const uint2* s2 = reinterpret_cast<const uint2*>(a.dA1 + static_cast<size_t>(b) * 512 * kDnHid);
-      uint2* d2 = reinterpret_cast<uint2*>(da);
-      for (int i = threadIdx.x; i < 512 * kDnHid / 4; i += 256) d2[i] = s2[i];
+      const uint16_t* xs = a.X + static_cast<size_t>(b) * kGridVox * kDnIn;
+      for (int i = threadIdx.x; i < kGridVox * kDnIn / 4; i += kBwdThreads) {
+        const int r = (i * 4) / kDnIn, c = (i * 4) % kDnIn;
+        const uint2 q = *reinterpret_cast<const uint2*>(xs + r * kDnIn + c);
+        xT[(c + 0) * kGridVox + r] = static_cast<uint16_t>(q.x & 0xffffu);
+        xT[(c + 1) * kGridVox + r] = static_cast<uint16_t>(q.x >> 16);
+        xT[(c + 2) * kGridVox + r] = static_cast<uint16_t>(q.y & 0xffffu);
+        xT[(c + 3) * kGridVox + r] = static_cast<uint16_t>(q.y >> 16);
+      }
+      const uint4* s2 = reinterpret_cast<const uint4*>(a.dA1 + static_cast<size_t>(b) * 512 * kDnHid);
+      uint4* d2 = reinterpret_cast<uint4*>(da);
+      for (int i = threadIdx.x; i < 512 * kDnHid / 8; i += kBwdThreads) d2[i] = s2[i];
     }
     __syncthreads();
-    if (wave == 0 && lane < kDnHid)
-      for (int m = 0; m < 512; ++m) db += bf2f(da[m * kDnHid + lane]);
-    accumulateDW<kDnIn, kDnHid, 7>(xg, da, wave, acc);
+    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(da[v * kDnHid + ch]); }, part, db);
+    accumulateDW32<kDnIn, kDnHid, false>(xT, da, wave, acc);
   }
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
-#pragma unroll
-  for (int j = 0; j < 7; ++j) {
-    const int t = wave + 4 * j;
-    if (t >= 27) break;
-#pragma unroll
-    for (int ot = 0; ot < 2; ++ot)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int c = (lane >> 4) * 4 + r, o = ot * 16 + (lane & 15);
-        if (c < 12) P[kOffW1 + (t * 12 + c) * 32 + o] = acc[j][0][ot][r];
-      }
-  }
-  if (wave == 0 && lane < kDnHid) P[kOffB1 + lane] = db;
+  storeDW<kDnIn, kDnHid>(acc, P, kOffW1, 12, kDnHid, wave);
+  if (threadIdx.x < kDnHid) P[kOffB1 + threadIdx.x] = db;
 }
 
-// grad[i] = sum over workgroups g (in order) of partial[g][i].
+// grad[i] = sum over workgroups g (in order) of partial[g][i]: 4 slices per
+// parameter (slice s takes rows s, s + 4, ...), combined in slice order.
+constexpr int kReduceSlices = 4;
 __global__ __launch_bounds__(256) void dnReduceKernel(const float* __restrict__ partial, int groups,
                                                       float* __restrict__ grad) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= kNumParams) return;
+  __shared__ float part[kReduceSlices][64];
+  const int pi = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + pi;
   float s = 0.0f;
-  for (int g = 0; g < groups; ++g) s += partial[static_cast<size_t>(g) * kNumParams + i];
-  grad[i] = s;
+  if (i < kNumParams)
+    for (int g = sl; g < groups; g += kReduceSlices) s += partial[static_cast<size_t>(g) * kNumParams + i];
+  part[sl][pi] = s;
+  __syncthreads();
+  if (sl == 0 && i < kNumParams) grad[i] = (part[0][pi] + part[1][pi]) + (part[2][pi] + part[3][pi]);
 }
 
 // Adam (Kingma & Ba) with bias correction; fp32 master parameters.
@@ -700,6 +780,23 @@ __global__ void dnMfmaProbeKernel(float* out) {
   for (int r = 0; r < 4; ++r) out[lane * 4 + r] = c[r];
 }
 
+// 16x16x32 bf16 MFMA (gfx950) layout probe: operands placed by the layout
+// the kernels assume -- per lane l, A[i][k] and B[k][n] with i = n = l % 16
+// and k = 8 (l / 16) + j for j = 0..7 -- from a (16 x 32) and b (32 x 16)
+// row-major; out[4 l + r] = the D element (lane l, register r).
+__global__ void dnMfma32ProbeKernel(const float* a, const float* b, float* out) {
+  const int lane = threadIdx.x;
+  v8s av, bv;
+  for (int j = 0; j < 8; ++j) {
+    const int i = lane & 15, k = (lane >> 4) * 8 + j;
+    av[j] = static_cast<short>(f2bf(a[i * 32 + k]));
+    bv[j] = static_cast<short>(f2bf(b[k * 16 + (lane & 15)]));
+  }
+  v4f c = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  c = mfma32(av, bv, c);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = c[r];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers --
@@ -725,7 +822,7 @@ hipError_t dnLaunchForward(const uint16_t* X, const DnWeights& w, uint32_t n, fl
                            const float* T, const float* M, float* dY, float* loss, float inv_count, hipStream_t s) {
   if (n == 0) return hipSuccess;
   DnFwdArgs a{X, w, Y, H1, H2, T, M, dY, loss, inv_count};
-  dnForwardKernel<<<n, 256, 0, s>>>(a);
+  dnForwardKernel<<<n, kFwdThreads, 0, s>>>(a);
   return hipGetLastError();
 }
 hipError_t dnLaunchBackward(const uint16_t* X, const uint16_t* H1, const uint16_t* H2, const float* dY,
@@ -735,10 +832,10 @@ hipError_t dnLaunchBackward(const uint16_t* X, const uint16_t* H1, const uint16_
   DnBwdArgs a{static_cast<int>(n), X, H1, H2, dY, dA2, dA1, w, partial};
   // each layer kernel writes every partial entry of its layer, so the
   // reduction never reads stale values
-  dnBackward3Kernel<<<groups, 256, 0, s>>>(a);
-  dnBackward2Kernel<<<groups, 256, 0, s>>>(a);
-  dnBackward1Kernel<<<groups, 256, 0, s>>>(a);
-  dnReduceKernel<<<(kNumParams + 255) / 256, 256, 0, s>>>(partial, groups, grad);
+  dnBackward3Kernel<<<groups, kBwdThreads, 0, s>>>(a);
+  dnBackward2Kernel<<<groups, kBwdThreads, 0, s>>>(a);
+  dnBackward1Kernel<<<groups, kBwdThreads, 0, s>>>(a);
+  dnReduceKernel<<<(kNumParams + 63) / 64, 256, 0, s>>>(partial, groups, grad);
   return hipGetLastError();
 }
 hipError_t dnLaunchAdam(float* P, const float* G, float* M, float* V, float lr, float b1, float b2, float eps,
@@ -758,6 +855,10 @@ hipError_t dnLaunchApply(const VolumeView& V, const int32_t* keys, uint32_t n, c
 }
 hipError_t dnLaunchProbe(float* out, hipStream_t s) {
   dnMfmaProbeKernel<<<1, 64, 0, s>>>(out);
+  return hipGetLastError();
+}
+hipError_t dnLaunchProbe32(const float* a, const float* b, float* out, hipStream_t s) {
+  dnMfma32ProbeKernel<<<1, 64, 0, s>>>(a, b, out);
   return hipGetLastError();
 }
 int dnNumParams() { return kNumParams; }

@@ -17,7 +17,7 @@ struct vhm_denoiser {
   int batch_capacity = 0;  // blocks per training step
   int count = 0;
   hipStream_t stream = nullptr;
-  int groups = 512;  // backward workgroups (partial gradient rows)
+  int groups = 256;  // backward workgroups (partial gradient rows): one per CU
   int step = 0;
   // parameters (fp32 master + Adam moments + gradient) and packed bf16 weights
   float* P = nullptr;
@@ -293,6 +293,18 @@ int vhm_mfma_layout_probe(float* out256) {
   if (hipMalloc(&dev, 256 * 4) != hipSuccess) return VHM_ERR_HIP;
   hipError_t e = dnLaunchProbe(dev, nullptr);
   if (e == hipSuccess) e = hipMemcpy(out256, dev, 256 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dev);
+  return e == hipSuccess ? VHM_OK : VHM_ERR_HIP;
+}
+
+int vhm_mfma_layout_probe_k32(const float* a16x32, const float* b32x16, float* out256) {
+  if (!a16x32 || !b32x16 || !out256) return VHM_ERR_INVALID_ARGUMENT;
+  float* dev = nullptr;
+  if (hipMalloc(&dev, (512 + 512 + 256) * 4) != hipSuccess) return VHM_ERR_HIP;
+  hipError_t e = hipMemcpy(dev, a16x32, 512 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dev + 512, b32x16, 512 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = dnLaunchProbe32(dev, dev + 512, dev + 1024, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(out256, dev + 1024, 256 * 4, hipMemcpyDeviceToHost);
   (void)hipFree(dev);
   return e == hipSuccess ? VHM_OK : VHM_ERR_HIP;
 }

@@ -335,6 +335,7 @@ hipError_t dnLaunchApply(const VolumeView& V, const int32_t* keys, uint32_t n, c
                          float voxel, float trunc, float max_weight, float min_w, float max_delta,
                          float block_max_delta, unsigned long long* counters, hipStream_t s);
 hipError_t dnLaunchProbe(float* out, hipStream_t s);
+hipError_t dnLaunchProbe32(const float* a, const float* b, float* out, hipStream_t s);
 int dnNumParams();
 }  // namespace vhm
 
