@@ -40,6 +40,13 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = 96
+# Marching cubes' algorithmic bytes: a staged (block, direction) item reads
+# its 9x9x9 corner region's 3 slots (48 B per voxel); a triangle writes 36 B.
+MC_BYTES_PER_ITEM = 48 * 729
+MC_BYTES_PER_TRIANGLE = 36
+# VALU issue peak: one wave64 instruction per CU per cycle (4 SIMDs of 16
+# lanes), 256 CUs at 2.4 GHz.
+VALU_PEAK_GINST = 256 * 2.4
 
 
 def log(*a):
@@ -448,6 +455,7 @@ def main():
     if mesh_every:
         vol.extract_mesh_incremental(device_tris=tri_buf)  # consume the warm-up's dirty blocks
     inc_blocks = inc_tris = inc_calls = inc_overflows = inc_max = 0
+    items0 = vol.mesh_items_staged()
     vol.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -464,6 +472,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     s1 = vol.stats()
+    inc_items = vol.mesh_items_staged() - items0
     kt = vol.kernel_times()
     vol.set_timing(False)
     updates = s1["updates"] - s0["updates"]
@@ -500,6 +509,28 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "integrateKernel+integrateDeferredKernel", "avg_launch_ms": round(avg_ms, 4),
                 "updates_per_launch": updates / max(integ["launches"], 1)}
+    # VALU issue of the same launches: VALU wavefront instructions per voxel
+    # update measured with rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES (the
+    # profile's figure), times this run's updates per launch, over the
+    # launch time, against one wave64 instruction per CU per cycle
+    vfile = ROOT / "profiles" / "integrateKernel_plain_scale_1280x720_valu.json"
+    if vfile.exists():
+        try:
+            ipu = json.loads(vfile.read_text())["valu_insts_per_update"]
+            ginst = ipu * updates / max(integ["ms"] * 1e-3, 1e-12) / 1e9
+            roofline["valu"] = {"achieved": round(ginst, 1), "peak": VALU_PEAK_GINST, "unit": "G wave-instr/s",
+                                "frac": round(ginst / VALU_PEAK_GINST, 4), "insts_per_update": ipu}
+        except (KeyError, ValueError):
+            pass
+    mesh_ms = kt["mesh"]["ms"]
+    mc_bytes = MC_BYTES_PER_ITEM * inc_items + MC_BYTES_PER_TRIANGLE * inc_tris
+    mc_achieved = mc_bytes / max(mesh_ms * 1e-3, 1e-12) / 1e9
+    mc_roofline = {"bound": "hbm", "achieved": round(mc_achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(mc_achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                   "kernel": "incremental marching cubes (meshNeighboursKernel+meshFusedKernel and the listing)",
+                   "avg_call_ms": round(mesh_ms / max(inc_calls, 1), 4),
+                   "items_staged_per_call": inc_items / max(inc_calls, 1),
+                   "triangles_per_call": inc_tris / max(inc_calls, 1)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -533,9 +564,11 @@ def main():
         "kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items()},
         "incremental_mesh": {"every_frames": args.mesh_every, "calls": inc_calls, "blocks_remeshed": inc_blocks,
                              "triangles": inc_tris, "ms": round(kt["mesh"]["ms"], 3),
+                             "items_staged": inc_items,
                              "tri_capacity": args.mesh_tri_capacity, "max_triangles_per_call": inc_max,
                              "capacity_overflows": inc_overflows},
         "roofline": roofline,
+        "roofline_mesh": mc_roofline,
         "cpu_baseline": cpu,
         **mesh,
     }

@@ -155,6 +155,12 @@ int vhm_observed_voxel_count(vhm_ctx* ctx, size_t* out);
  * update with weight > 0), voxels visited (512 per active block), frames. */
 int vhm_stats(vhm_ctx* ctx, uint64_t* updates, uint64_t* visited, uint64_t* frames);
 
+/* Marching-cubes work since creation/reset (no reference counterpart): the
+ * (block, direction) items whose 9x9x9 corner region was staged (read from
+ * the pool: 3 slots x 16 B per region voxel), by the full (count pass only)
+ * and the incremental extractions.  Synchronous. */
+int vhm_mesh_stats(vhm_ctx* ctx, uint64_t* items_staged);
+
 /* gpuVolumeCacheBlockKeys (cuda_tsdf_integrator.cu:1054-1057): all
  * allocated keys in pool order.  Pool order is creation order: frame by
  * frame (point-cloud call by call), and within one frame in increasing

@@ -138,6 +138,7 @@ _SIGS = {
                                                        C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t,
                                                        C.POINTER(C.c_size_t)]),
     "vhm_mfma_layout_probe_k32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vhm_mesh_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "vhm_block_digests": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "vhm_batch_mesh_items": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
                                        C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t]),
@@ -343,6 +344,12 @@ class Volume:
     def observed_voxel_count(self) -> int:
         n = C.c_size_t()
         self._check(lib().vhm_observed_voxel_count(self._h, C.byref(n)))
+        return n.value
+
+    def mesh_items_staged(self) -> int:
+        """(block, direction) items marching cubes has staged since creation/reset."""
+        n = C.c_uint64()
+        self._check(lib().vhm_mesh_stats(self._h, C.byref(n)))
         return n.value
 
     def stats(self) -> dict:

@@ -32,7 +32,7 @@ DEMO_SOURCE = PKG / "host" / "vhm_demo.cpp"
 ARCH = os.environ.get("VHM_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    f"--offload-arch={ARCH}", "-O3", "-gline-tables-only", "-std=c++17", "-fPIC",
     "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",

@@ -750,6 +750,14 @@ int vhm_stats(vhm_ctx* c, uint64_t* updates, uint64_t* visited, uint64_t* frames
   return rc;
 }
 
+int vhm_mesh_stats(vhm_ctx* c, uint64_t* items_staged) {
+  if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  DevHeader h{};
+  const int rc = readHeader(c, &h);
+  if (items_staged) *items_staged = h.mesh_items;
+  return rc;
+}
+
 int vhm_block_keys(vhm_ctx* c, int32_t* keys_out, size_t capacity, size_t* n_out) {
   size_t n = 0;
   int rc = vhm_block_count(c, &n);

@@ -51,7 +51,7 @@ struct DevHeader {
   unsigned long long observed;  // voxels with any layer weight > 0
   unsigned long long updates;   // voxel updates (weight > 0, valid direction)
   unsigned long long visited;   // voxels visited (512 per integrated block)
-  unsigned long long mesh_total;
+  unsigned long long mesh_items;  // (block, direction) items marching cubes staged (each 9^3 x 3 slots)
   uint32_t defer_count;  // blocks appended to the integrator's deferred list this frame
   uint32_t integrate_ticket;  // integrator: next active-list entry to hand out this frame
   uint32_t number_base;       // num_blocks when the current insertion pass began

@@ -557,6 +557,7 @@ __device__ uint32_t countItem(MeshShared& S, float* __restrict__ cand, const Vol
     __syncthreads();  // S.any_dir is rewritten by the next item
     return 0u;
   }
+  if (threadIdx.x == 0) atomicAdd(&V.hdr->mesh_items, 1ull);
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int cube = threadIdx.x + 256 * half;
@@ -641,6 +642,7 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
     __syncthreads();  // the item's LDS is reused by the next item
     return 0u;
   }
+  if (tid == 0) atomicAdd(&V.hdr->mesh_items, 1ull);
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int cube = tid + 256 * half;
