@@ -78,15 +78,21 @@ def fuse(vol: Volume, w, frames, idx) -> float:
 
 
 def build_training_set(dn: Denoiser, seeds, frames_in=10, frames_target=300, blocks_per_scene=1000,
-                       target_noise="halved"):
-    """Appends training samples from each seed's small room to dn.  The
-    target frames carry the noise sigma halved ("halved", the training
-    configuration) or no noise at all ("none")."""
+                       target_noise="halved", style="small"):
+    """Appends training samples from each seed's room to dn: a small room
+    (style "small": 640x480, voxel 1 cm, the training configuration) or a
+    room in the plain-at-scale style ("scale": 8x8x3 m, 1280x720 random walk,
+    voxel 5 mm, the evaluation's style).  The target frames carry the noise
+    sigma halved ("halved", the training configuration) or no noise at all
+    ("none")."""
     vin = vt = None
     for s in seeds:
-        w = synth.plain_small(n_frames=frames_target, seed=s)
+        if style == "scale":
+            w = synth.plain_scale(n_frames=frames_target, seed=s)
+        else:
+            w = synth.plain_small(n_frames=frames_target, seed=s)
         if vin is None:
-            vp = volume_params(w, max_blocks=80_000)
+            vp = volume_params(w, max_blocks=80_000 if style == "small" else 400_000)
             vin, vt = Volume(vp), Volume(vp)
         else:
             vin.reset()
@@ -94,7 +100,7 @@ def build_training_set(dn: Denoiser, seeds, frames_in=10, frames_target=300, blo
         frames = render_gpu(w, w.noise)
         fuse(vin, w, frames, spread(frames_target, frames_in))
         del frames
-        frames = render_gpu(w, halved_noise(s) if target_noise == "halved" else None)
+        frames = render_gpu(w, halved_noise(s) if target_noise == "halved" else Noise(enabled=0))
         fuse(vt, w, frames, range(frames_target))
         del frames
         k_in = vin.observed_block_keys()
@@ -231,6 +237,8 @@ def compare_on_scene(dn: Denoiser, w, frames_plain=300, frames_learned=30, apply
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--train-scenes", type=int, default=200)
+    ap.add_argument("--train-style", choices=["small", "scale"], default="small",
+                    help="training rooms: the training configuration's small rooms, or the evaluation's style")
     ap.add_argument("--train-seed", type=int, default=1000)
     ap.add_argument("--blocks-per-scene", type=int, default=1000)
     ap.add_argument("--frames-in", type=int, default=10)
@@ -262,7 +270,8 @@ def main(argv=None):
         dn.set_params(init_params(0))
         t0 = time.perf_counter()
         seeds = list(range(args.train_seed, args.train_seed + args.train_scenes))
-        build_training_set(dn, seeds, args.frames_in, args.frames_target, args.blocks_per_scene, args.target_noise)
+        build_training_set(dn, seeds, args.frames_in, args.frames_target, args.blocks_per_scene, args.target_noise,
+                           args.train_style)
         res["train_set_blocks"] = dn.count
         res["train_set_seconds"] = time.perf_counter() - t0
         # the last scenes' samples are held out of training
