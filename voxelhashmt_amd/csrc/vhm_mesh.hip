@@ -291,15 +291,23 @@ __device__ __forceinline__ void emitCube(int ox, int oy, int oz, float voxel, co
     }
 }
 
-// Whether any corner of `cube` may lie below iso and any at or above it
-// (see stageRegion); a cube failing this yields no triangle.
+// Whether `cube` may yield a triangle (see stageRegion): some corner may lie
+// below iso and some at or above it, and at least 6 corners hold a slot that
+// a cluster within reach of iso can select (a corner without one is never
+// valid, and completeCubeValuesWithFallback drops a cube with fewer than 6
+// valid corners).  A cube failing this yields no triangle.
 __device__ __forceinline__ bool cubeMayCross(const MeshShared& S, int cube) {
   const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
   const int r0 = lx + kR * (ly + kR * lz);
-  const uint32_t span = S.bits[r0] | S.bits[r0 + 1] | S.bits[r0 + kR] | S.bits[r0 + kR + 1] |
-                        S.bits[r0 + kR * kR] | S.bits[r0 + kR * kR + 1] | S.bits[r0 + kR * kR + kR] |
-                        S.bits[r0 + kR * kR + kR + 1];
-  return (span & (kBitBelow | kBitAbove)) == (kBitBelow | kBitAbove);
+  constexpr uint32_t kNear = kBitBelow | kBitAbove;
+  uint32_t span = 0, usable = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t b = S.bits[r0 + (c & 1) + kR * ((c >> 1) & 1) + kR * kR * (c >> 2)];
+    span |= b;
+    usable += (b & kNear) != 0u ? 1u : 0u;
+  }
+  return (span & kNear) == kNear && usable >= 6u;
 }
 
 // Sorting network for the 24 candidate slots (8 corners x 3 slots): Batcher's
