@@ -219,6 +219,9 @@ def bench_batched(args):
     b.prepare(use_graph=True)
     gi, gm = b.autotune((2, 4, 8, 16))
     log(f"[bench-batched] autotuned grids: integrate {gi}, mesh {gm}")
+    ftw, atw, chunk = b.autotune_tiles((32, 16, 64, 32, 16, 64), (16, 32, 64, 16, 32, 64), (0, 16, 64, 0, 16, 64))
+    log(f"[bench-batched] autotuned tiles: filter {ftw}x{256 // ftw}, allocation {atw}x{256 // atw}, "
+        f"integrator chunk {chunk}")
     b.rewind()
     if args.warmup:
         b.step(n_frames)
@@ -298,6 +301,8 @@ def bench_batched(args):
                    "blocks_max": int(blocks.max()), "blocks_total": int(blocks.sum()),
                    "target_blocks_min": min(w.target_blocks for w in ws),
                    "target_blocks_max": max(w.target_blocks for w in ws),
+                   "filter_tile": f"{ftw}x{256 // ftw}", "alloc_tile": f"{atw}x{256 // atw}",
+                   "integrate_chunk": chunk,
                    "integrate_grid": gi, "mesh_grid": gm, "hipgraph": True},
         "frames_per_s": len(ws) * K / elapsed,
         "stage_ms_direct_launches": {k: round(v, 3) for k, v in tot.items() if k.endswith("_ms")},

@@ -345,6 +345,18 @@ int vhm_batch_timed_step(vhm_batch* b, float ms_out[3], uint64_t* updates, uint6
  * per unit of work (a captured graph is re-captured with them). */
 int vhm_batch_autotune(vhm_batch* b, const int* candidates, int n_candidates, int* integrate_grid_out,
                        int* mesh_grid_out);
+/* Tile shapes and the integrator's work split: step k runs with candidate k
+ * of each list (the last one once a list is exhausted) and the fastest of
+ * each is kept -- the filter tile width and the allocation tile width (16,
+ * 32 or 64 pixels wide, 256 / width tall; default 32 and 16), each timed
+ * by its own events, and the integrator's chunk (0, the default: one
+ * contiguous range of the concatenated active lists per workgroup; c > 0:
+ * workgroups take c entries at a time by atomic ticket), by time per voxel
+ * update.  Consumes max(n_filter, n_alloc, n_chunk) steps of the sequences;
+ * results do not depend on the choices.  A captured graph is re-captured. */
+int vhm_batch_autotune_tiles(vhm_batch* b, const int* filter_widths, int n_filter, const int* alloc_widths,
+                             int n_alloc, const int* chunks, int n_chunk, int* filter_width_out, int* alloc_width_out,
+                             int* chunk_out);
 
 /* --- synthetic depth (no reference counterpart; benchmark/test input) ---- */
 

@@ -44,7 +44,7 @@ def _blocks(vol, keys):
     return vol.download_blocks(keys) if len(keys) else None
 
 
-def _run(torch, use_graph, autotune):
+def _run(torch, use_graph, autotune, tiles=None):
     wl = _workloads()
     singles = [vhm.Volume(_params(w)) for w, _ in wl]
     with vhm.Batch([_params(w) for w, _ in wl]) as b:
@@ -58,6 +58,16 @@ def _run(torch, use_graph, autotune):
             gi, gm = b.autotune((1, 4))
             print(f"[batch] autotuned grids integrate {gi} mesh {gm}", flush=True)
             for _ in range(2):  # autotune consumed two steps; feed the singles the same frames
+                for i, (w, frames) in enumerate(wl):
+                    if step < len(frames):
+                        singles[i].integrate(frames[step], w.K, w.poses[step])
+                        singles[i].extract_mesh_incremental()
+                step += 1
+        if tiles is not None:
+            got = b.autotune_tiles(*tiles)
+            print(f"[batch] tiles {tiles} -> filter {got[0]}, allocation {got[1]}, chunk {got[2]}", flush=True)
+            assert got == tuple(t[-1] for t in tiles) or len(tiles[0]) > 1
+            for _ in range(max(len(t) for t in tiles)):  # consumed steps
                 for i, (w, frames) in enumerate(wl):
                     if step < len(frames):
                         singles[i].integrate(frames[step], w.K, w.poses[step])
@@ -111,6 +121,15 @@ def test_batch_equals_single_volumes(gpu, use_graph):
 def test_batch_autotune_keeps_results(gpu):
     import torch
     _run(torch, True, autotune=True)
+
+
+@pytest.mark.parametrize("tiles", [((64,), (64,), (8,)), ((16,), (32,), (1,)), ((32, 16), (16, 64), (0, 32))])
+def test_batch_tile_shapes_keep_results(gpu, tiles):
+    """Filter / allocation tile widths and the integrator's chunk (forced
+    single candidates, then a timed choice between two): every later step
+    and the final volumes still equal the one-volume path."""
+    import torch
+    _run(torch, True, autotune=False, tiles=tiles)
 
 
 def test_batched_48_rooms_first_steps(gpu):

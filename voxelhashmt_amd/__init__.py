@@ -144,6 +144,9 @@ _SIGS = {
                                        C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t]),
     "vhm_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
+    "vhm_batch_autotune_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int,
+                                           C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]),
     "vhm_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "vhm_integrate_point_cloud": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.POINTER(Pose)]),
     # include/vhm_io.h (host only)
@@ -635,6 +638,18 @@ class Batch:
         gi, gm = C.c_int(), C.c_int()
         self._check(lib().vhm_batch_autotune(self._h, c, len(candidates), C.byref(gi), C.byref(gm)))
         return gi.value, gm.value
+
+    def autotune_tiles(self, filter_widths=(32, 16, 64), alloc_widths=(16, 32, 64),
+                       chunks=(0, 16, 64)) -> tuple[int, int, int]:
+        """Filter / allocation tile widths and the integrator's chunk
+        (vhm_batch_autotune_tiles); consumes max of the list lengths steps."""
+        f = (C.c_int * len(filter_widths))(*filter_widths)
+        a = (C.c_int * len(alloc_widths))(*alloc_widths)
+        c = (C.c_int * len(chunks))(*chunks)
+        fo, ao, co = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().vhm_batch_autotune_tiles(self._h, f, len(filter_widths), a, len(alloc_widths), c,
+                                                   len(chunks), C.byref(fo), C.byref(ao), C.byref(co)))
+        return fo.value, ao.value, co.value
 
 
 def mfma_layout_probe() -> np.ndarray:

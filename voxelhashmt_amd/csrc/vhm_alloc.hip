@@ -8,7 +8,8 @@
 // (cuda_block_allocator.cu:231-300) issues all 81 candidate inserts per pixel
 // as global CAS into a table rebuilt (and memset) every frame.
 //
-// Here one workgroup covers a 16x16 sample tile and deduplicates twice in LDS
+// Here one workgroup covers a 256-sample tile (16x16; 32x8 and 64x4 for the
+// batched step's autotuner) and deduplicates twice in LDS
 // before touching HBM: the tile's (<= 768) centre keys, then their 27-block
 // neighbourhoods.  Each surviving key is inserted into the persistent
 // open-addressing table with a 64-bit CAS (linear probing).  A per-key frame
@@ -26,7 +27,7 @@
 namespace vhm {
 namespace {
 
-constexpr int kTile = 16;
+constexpr int kTileW = 16;  // default tile: 16 x 16 samples
 constexpr int kCentreSlots = 1024;  // >= 3 * 256 centre keys
 constexpr int kNbrSlots = 4096;
 constexpr int kNbrProbe = 32;
@@ -59,9 +60,12 @@ __device__ __forceinline__ int worldToVoxel1(float x, float voxel) {
   return static_cast<int>(floorf(x / voxel));
 }
 
-// One 16x16 sample tile of the allocation rule for one frame.
+// One TW x (256 / TW) sample tile of the allocation rule for one frame.
+template <int TW>
 __device__ void allocTile(const float* __restrict__ depth, const VolumeView& V, const AllocArgs& a, int tile_x,
                           int tile_y) {
+  static_assert(TW == 16 || TW == 32 || TW == 64, "tile widths 16, 32, 64");
+  constexpr int kTH = 256 / TW;
   __shared__ uint64_t centres[kCentreSlots];
   __shared__ uint64_t centre_list[3 * 256];
   __shared__ uint64_t nbrs[kNbrSlots];
@@ -75,7 +79,7 @@ __device__ void allocTile(const float* __restrict__ depth, const VolumeView& V, 
   __syncthreads();
 
   // 1. centre keys of this lane's pixel (voxel_hash_tsdf.h:645-661).
-  const int sx = tile_x * kTile + tid % kTile, sy = tile_y * kTile + tid / kTile;
+  const int sx = tile_x * TW + tid % TW, sy = tile_y * kTH + tid / TW;
   const int u = sx * a.stride, v = sy * a.stride;
   if (u < a.K.width && v < a.K.height) {
     const float d = depth[static_cast<size_t>(v) * a.K.width + u];
@@ -166,7 +170,7 @@ __device__ void allocTile(const float* __restrict__ depth, const VolumeView& V, 
 }
 
 __global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ depth, VolumeView V, AllocArgs a) {
-  allocTile(depth, V, a, blockIdx.x, blockIdx.y);
+  allocTile<kTileW>(depth, V, a, blockIdx.x, blockIdx.y);
 }
 
 __global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) {
@@ -270,8 +274,9 @@ __global__ __launch_bounds__(kNumberThreads) void numberBlocksKernel(VolumeView 
 
 // ---- batched steps ----
 
-__global__ void batchFrameBeginKernel(const BatchScene* __restrict__ scenes, int S, const BatchState* __restrict__ st) {
+__global__ void batchFrameBeginKernel(const BatchScene* __restrict__ scenes, int S, BatchState* __restrict__ st) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) st->integrate_ticket = 0u;
   if (s >= S) return;
   scenes[s].V.hdr->active_count = 0u;
   scenes[s].V.hdr->defer_count = 0u;
@@ -288,6 +293,7 @@ __global__ __launch_bounds__(kNumberThreads) void numberBlocksBatchKernel(const 
   numberBlocks(sc.V);
 }
 
+template <int TW>
 __global__ __launch_bounds__(256) void allocateBatchKernel(const BatchScene* __restrict__ scenes, int S,
                                                            const uint32_t* __restrict__ tile_prefix,
                                                            const BatchState* __restrict__ st) {
@@ -300,8 +306,8 @@ __global__ __launch_bounds__(256) void allocateBatchKernel(const BatchScene* __r
   a.frame = step + 1u;
   const uint32_t t = blockIdx.x - tile_prefix[s];
   const int sw = (a.K.width + a.stride - 1) / a.stride;
-  const uint32_t tiles_x = (sw + kTile - 1) / kTile;
-  allocTile(sc.filtered, sc.V, a, t % tiles_x, t / tiles_x);
+  const uint32_t tiles_x = (sw + TW - 1) / TW;
+  allocTile<TW>(sc.filtered, sc.V, a, t % tiles_x, t / tiles_x);
 }
 
 // prefix[s] = number of active blocks of scenes < s this step (S + 1 entries).
@@ -321,19 +327,27 @@ __global__ void stepIncrementKernel(BatchState* st) { st->step += 1u; }
 
 }  // namespace
 
-uint32_t allocTiles(const Intrinsics& K, int stride) {
+uint32_t allocTiles(const Intrinsics& K, int stride, int tile_w) {
   const int sw = (K.width + stride - 1) / stride, sh = (K.height + stride - 1) / stride;
-  return static_cast<uint32_t>(((sw + kTile - 1) / kTile) * ((sh + kTile - 1) / kTile));
+  const int th = 256 / tile_w;
+  return static_cast<uint32_t>(((sw + tile_w - 1) / tile_w) * ((sh + th - 1) / th));
 }
 
-hipError_t launchBatchFrameBegin(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s) {
+hipError_t launchBatchFrameBegin(const BatchScene* scenes, int S, BatchState* st, hipStream_t s) {
   batchFrameBeginKernel<<<(S + 63) / 64, 64, 0, s>>>(scenes, S, st);
   return hipGetLastError();
 }
 
 hipError_t launchAllocateBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
-                               const BatchState* st, hipStream_t s) {
-  allocateBatchKernel<<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+                               const BatchState* st, int tile_w, hipStream_t s) {
+  if (tile_w == 16)
+    allocateBatchKernel<16><<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  else if (tile_w == 32)
+    allocateBatchKernel<32><<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  else if (tile_w == 64)
+    allocateBatchKernel<64><<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -371,7 +385,7 @@ hipError_t launchFrameBegin(DevHeader* hdr, uint32_t frame, hipStream_t s) {
 
 hipError_t launchAllocate(const float* depth, const VolumeView& V, const AllocArgs& a, hipStream_t s) {
   const int sw = (a.K.width + a.stride - 1) / a.stride, sh = (a.K.height + a.stride - 1) / a.stride;
-  const dim3 grid((sw + kTile - 1) / kTile, (sh + kTile - 1) / kTile);
+  const dim3 grid((sw + kTileW - 1) / kTileW, (sh + 256 / kTileW - 1) / (256 / kTileW));
   allocateKernel<<<grid, 256, 0, s>>>(depth, V, a);
   return hipGetLastError();
 }

@@ -17,7 +17,9 @@
 //                       the graph is replayed unchanged)
 // Per-scene inputs (frames, poses and their inverses) are resident on the
 // device for the whole sequence.  Grid sizes of the two persistent kernels
-// come from the autotuner (vhm_batch_autotune) or the caller.
+// come from the autotuner (vhm_batch_autotune) or the caller; the filter and
+// allocation tile shapes and the integrator's chunk from
+// vhm_batch_autotune_tiles.
 #include <cstring>
 #include <string>
 #include <vector>
@@ -45,6 +47,8 @@ struct vhm_batch {
   float* mesh_cand = nullptr;  // marching cubes' cluster-centre scratch (kMeshCandBytes)
   uint32_t filter_total = 0, alloc_total = 0;
   int integrate_grid = 2048, mesh_grid = 2048, num_cus = 256;
+  int filter_tile_w = 32, alloc_tile_w = 16;  // tiles of w x (256 / w) pixels
+  uint32_t integrate_chunk = 0;              // 0: one contiguous range per workgroup
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool prepared = false;
@@ -83,22 +87,49 @@ void noteStep(vhm_batch* b) {
 }
 
 // A step's stages before integration: frame begin, filter, allocation, the
-// ordered numbering of new blocks, the active-list prefix.
-hipError_t enqueueFrontEnd(vhm_batch* b) {
+// ordered numbering of new blocks, the active-list prefix.  ev (optional):
+// events recorded after the filter and after the allocation.
+hipError_t enqueueFrontEnd(vhm_batch* b, hipEvent_t ev_filter = nullptr, hipEvent_t ev_alloc = nullptr) {
   const int S = b->S;
   hipError_t e = launchBatchFrameBegin(b->scenes, S, b->state, b->stream);
-  if (e == hipSuccess) e = launchFilterBatch(b->scenes, S, b->filter_prefix, b->filter_total, b->state, b->stream);
-  if (e == hipSuccess) e = launchAllocateBatch(b->scenes, S, b->alloc_prefix, b->alloc_total, b->state, b->stream);
+  if (e == hipSuccess)
+    e = launchFilterBatch(b->scenes, S, b->filter_prefix, b->filter_total, b->state, b->filter_tile_w, b->stream);
+  if (e == hipSuccess && ev_filter) e = hipEventRecord(ev_filter, b->stream);
+  if (e == hipSuccess)
+    e = launchAllocateBatch(b->scenes, S, b->alloc_prefix, b->alloc_total, b->state, b->alloc_tile_w, b->stream);
+  if (e == hipSuccess && ev_alloc) e = hipEventRecord(ev_alloc, b->stream);
   if (e == hipSuccess) e = launchNumberBlocksBatch(b->scenes, S, b->state, b->stream);
   if (e == hipSuccess) e = launchActivePrefix(b->scenes, S, b->state, b->active_prefix, b->stream);
   return e;
+}
+
+bool validTileWidth(int w) { return w == 16 || w == 32 || w == 64; }
+
+// Tile prefixes of the current tile shapes, uploaded.
+int uploadTilePrefixes(vhm_batch* b) {
+  std::vector<uint32_t> fp(b->S + 1), ap(b->S + 1);
+  uint32_t f = 0, a = 0;
+  for (int s = 0; s < b->S; ++s) {
+    fp[s] = f;
+    ap[s] = a;
+    f += filterTiles(b->host_scenes[s].fa.K, b->filter_tile_w);
+    a += allocTiles(b->host_scenes[s].aa.K, 1, b->alloc_tile_w);
+  }
+  fp[b->S] = f;
+  ap[b->S] = a;
+  b->filter_total = f;
+  b->alloc_total = a;
+  B_TRY(b, hipMemcpy(b->filter_prefix, fp.data(), 4 * (b->S + 1), hipMemcpyHostToDevice), "hipMemcpy prefixes");
+  B_TRY(b, hipMemcpy(b->alloc_prefix, ap.data(), 4 * (b->S + 1), hipMemcpyHostToDevice), "hipMemcpy prefixes");
+  return VHM_OK;
 }
 
 // Launches one step's kernels on b->stream (also what the graph captures).
 int enqueueStep(vhm_batch* b) {
   const int S = b->S;
   B_TRY(b, enqueueFrontEnd(b), "batch front-end launch");
-  B_TRY(b, launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, b->integrate_grid, b->stream),
+  B_TRY(b, launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, b->integrate_grid, b->integrate_chunk,
+                                b->stream),
         "batch integrate launch");
   B_TRY(b, launchMeshBatch(b->scenes, S, b->mesh_prefix, b->mesh_grid, b->mesh_cand, b->stream), "batch mesh launch");
   B_TRY(b, launchStepIncrement(b->state, b->stream), "batch step launch");
@@ -261,22 +292,12 @@ int vhm_batch_prepare(vhm_batch* b, int integrate_grid, int mesh_grid, int use_g
     if (!b->ready[s]) return bfail(b, VHM_ERR_INVALID_ARGUMENT, "batch prepare failed: a scene has no sequence");
   if (integrate_grid > 0) b->integrate_grid = integrate_grid;
   if (mesh_grid > 0) b->mesh_grid = mesh_grid;
-  std::vector<uint32_t> fp(b->S + 1), ap(b->S + 1);
-  uint32_t f = 0, a = 0;
-  for (int s = 0; s < b->S; ++s) {
-    fp[s] = f;
-    ap[s] = a;
-    f += filterTiles(b->host_scenes[s].fa.K);
-    a += allocTiles(b->host_scenes[s].aa.K, 1);
-  }
-  fp[b->S] = f;
-  ap[b->S] = a;
-  b->filter_total = f;
-  b->alloc_total = a;
   B_TRY(b, hipMemcpy(b->scenes, b->host_scenes.data(), sizeof(BatchScene) * b->S, hipMemcpyHostToDevice),
         "hipMemcpy scene table");
-  B_TRY(b, hipMemcpy(b->filter_prefix, fp.data(), 4 * (b->S + 1), hipMemcpyHostToDevice), "hipMemcpy prefixes");
-  B_TRY(b, hipMemcpy(b->alloc_prefix, ap.data(), 4 * (b->S + 1), hipMemcpyHostToDevice), "hipMemcpy prefixes");
+  {
+    const int rc = uploadTilePrefixes(b);
+    if (rc) return rc;
+  }
   // the volumes' own streams must be idle before the batch stream uses them
   for (vhm_ctx* c : b->vols) B_TRY(b, hipStreamSynchronize(vhmCtxStream(c)), "hipStreamSynchronize volume");
   if (b->exec) {
@@ -386,8 +407,10 @@ namespace {
 // One step with direct launches and events: ms[0] frame begin + filter +
 // allocation + prefix, ms[1] integration, ms[2] incremental meshing;
 // *updates / *remeshed: the step's voxel updates and re-meshed blocks over
-// all scenes.  Synchronous.
-int timedStep(vhm_batch* b, int gi, int gm, float ms[3], uint64_t* updates, uint64_t* remeshed) {
+// all scenes; fa_ms (optional): the filter's and the allocation's own times.
+// Synchronous.
+int timedStep(vhm_batch* b, int gi, int gm, float ms[3], uint64_t* updates, uint64_t* remeshed,
+              float* fa_ms = nullptr) {
   const int S = b->S;
   uint64_t u0 = 0;
   for (vhm_ctx* v : b->vols) {
@@ -396,12 +419,13 @@ int timedStep(vhm_batch* b, int gi, int gm, float ms[3], uint64_t* updates, uint
     if (rc) return bfail(b, rc, vhm_last_error(v));
     u0 += u;
   }
-  hipEvent_t ev[4] = {};
+  hipEvent_t ev[6] = {};
   for (auto& e : ev) B_TRY(b, hipEventCreate(&e), "hipEventCreate");
   hipError_t e = hipEventRecord(ev[0], b->stream);
-  if (e == hipSuccess) e = enqueueFrontEnd(b);
+  if (e == hipSuccess) e = enqueueFrontEnd(b, ev[4], ev[5]);
   if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
-  if (e == hipSuccess) e = launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, gi, b->stream);
+  if (e == hipSuccess)
+    e = launchIntegrateBatch(b->scenes, S, b->active_prefix, b->state, gi, b->integrate_chunk, b->stream);
   if (e == hipSuccess) e = hipEventRecord(ev[2], b->stream);
   if (e == hipSuccess) e = launchMeshBatch(b->scenes, S, b->mesh_prefix, gm, b->mesh_cand, b->stream);
   if (e == hipSuccess) e = hipEventRecord(ev[3], b->stream);
@@ -409,6 +433,10 @@ int timedStep(vhm_batch* b, int gi, int gm, float ms[3], uint64_t* updates, uint
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   if (e == hipSuccess) {
     for (int i = 0; i < 3; ++i) (void)hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+    if (fa_ms) {
+      (void)hipEventElapsedTime(&fa_ms[0], ev[0], ev[4]);  // frame begin + filter
+      (void)hipEventElapsedTime(&fa_ms[1], ev[4], ev[5]);  // allocation
+    }
   }
   for (auto& x : ev) (void)hipEventDestroy(x);
   B_TRY(b, e, "batch timed step");
@@ -471,6 +499,65 @@ int vhm_batch_autotune(vhm_batch* b, const int* candidates, int n_candidates, in
   if (mesh_grid_out) *mesh_grid_out = bm;
   if (b->exec) return vhm_batch_prepare(b, 0, 0, 1);  // re-capture with the chosen grids
   return VHM_OK;
+}
+
+// Times one step per candidate and keeps the fastest filter tile width and
+// allocation tile width (each timed by its own events; 16, 32 or 64 pixels
+// wide, 256 / width tall) and integrator chunk (0: one contiguous range per
+// workgroup, else entries per ticket; normalised by voxel updates).  Step k
+// uses candidate k of each list (the last one once a list is exhausted), so
+// max(n_filter, n_alloc, n_chunk) steps of the sequences are consumed
+// (results do not depend on these choices).
+int vhm_batch_autotune_tiles(vhm_batch* b, const int* filter_widths, int n_filter, const int* alloc_widths,
+                             int n_alloc, const int* chunks, int n_chunk, int* filter_width_out, int* alloc_width_out,
+                             int* chunk_out) {
+  if (!b || !b->prepared || !filter_widths || !alloc_widths || !chunks || n_filter <= 0 || n_alloc <= 0 ||
+      n_chunk <= 0)
+    return VHM_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < n_filter; ++i)
+    if (!validTileWidth(filter_widths[i]))
+      return bfail(b, VHM_ERR_INVALID_ARGUMENT, "batch tile autotune failed: filter tile width not 16, 32 or 64");
+  for (int i = 0; i < n_alloc; ++i)
+    if (!validTileWidth(alloc_widths[i]))
+      return bfail(b, VHM_ERR_INVALID_ARGUMENT, "batch tile autotune failed: allocation tile width not 16, 32 or 64");
+  for (int i = 0; i < n_chunk; ++i)
+    if (chunks[i] < 0 || chunks[i] > (1 << 20))
+      return bfail(b, VHM_ERR_INVALID_ARGUMENT, "batch tile autotune failed: bad integrator chunk");
+  const int rounds = n_filter > n_alloc ? (n_filter > n_chunk ? n_filter : n_chunk) : (n_alloc > n_chunk ? n_alloc : n_chunk);
+  float best_f = 3.4e38f, best_a = 3.4e38f;
+  double best_c = 1e300;
+  int bf = b->filter_tile_w, ba = b->alloc_tile_w, bc = static_cast<int>(b->integrate_chunk);
+  for (int k = 0; k < rounds; ++k) {
+    b->filter_tile_w = filter_widths[k < n_filter ? k : n_filter - 1];
+    b->alloc_tile_w = alloc_widths[k < n_alloc ? k : n_alloc - 1];
+    b->integrate_chunk = static_cast<uint32_t>(chunks[k < n_chunk ? k : n_chunk - 1]);
+    int rc = uploadTilePrefixes(b);
+    if (rc) return rc;
+    float ms[3] = {0.0f, 0.0f, 0.0f}, fa[2] = {0.0f, 0.0f};
+    uint64_t upd = 0, rm = 0;
+    rc = timedStep(b, b->integrate_grid, b->mesh_grid, ms, &upd, &rm, fa);
+    if (rc) return rc;
+    if (k < n_filter && fa[0] < best_f) {
+      best_f = fa[0];
+      bf = b->filter_tile_w;
+    }
+    if (k < n_alloc && fa[1] < best_a) {
+      best_a = fa[1];
+      ba = b->alloc_tile_w;
+    }
+    const double per_c = ms[1] / static_cast<double>(upd + 1);
+    if (k < n_chunk && per_c < best_c) {
+      best_c = per_c;
+      bc = static_cast<int>(b->integrate_chunk);
+    }
+  }
+  b->filter_tile_w = bf;
+  b->alloc_tile_w = ba;
+  b->integrate_chunk = static_cast<uint32_t>(bc);
+  if (filter_width_out) *filter_width_out = bf;
+  if (alloc_width_out) *alloc_width_out = ba;
+  if (chunk_out) *chunk_out = bc;
+  return vhm_batch_prepare(b, 0, 0, b->exec ? 1 : 0);  // prefixes of the chosen shapes, graph re-captured
 }
 
 // Clears every volume and restarts the sequences at frame 0, keeping the

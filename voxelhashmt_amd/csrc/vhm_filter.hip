@@ -6,9 +6,11 @@
 // twin is two kernels with a global round trip of the normals
 // (cuda_tsdf_integrator.cu:629-725).
 //
-// Tiling: a workgroup of 256 lanes (4 wavefronts) filters a 32x8 pixel tile.
-// LDS holds the tile's depth with a 3-pixel apron (38x14, converted from
-// uint16 once) and the normals of the tile with a 2-pixel apron (36x12), so
+// Tiling: a workgroup of 256 lanes (4 wavefronts) filters a TX x (256 / TX)
+// pixel tile (32x8 by default; 16x16 and 64x4 for the batched step's
+// autotuner).  LDS holds the tile's depth with a 3-pixel apron (38x14 at
+// 32x8, converted from uint16 once) and the normals of the tile with a
+// 2-pixel apron (36x12), so
 // every depth sample is read from HBM once per tile (plus the apron) and the
 // normals never leave the CU.  Spatial Gaussian weights are precomputed on the
 // host; the range and angular Gaussians use expf_glibc.
@@ -19,9 +21,8 @@
 namespace vhm {
 namespace {
 
-constexpr int kTX = 32, kTY = 8, kR = 2;
-constexpr int kDW = kTX + 2 * (kR + 1), kDH = kTY + 2 * (kR + 1);  // depth tile 38x14
-constexpr int kNW = kTX + 2 * kR, kNH = kTY + 2 * kR;              // normal tile 36x12
+constexpr int kR = 2;
+constexpr int kDefaultTX = 32;
 
 template <typename In>
 __device__ __forceinline__ float loadDepth(const In* in, size_t i, float scale);
@@ -40,11 +41,15 @@ __device__ __forceinline__ float loadDepth<uint16_t>(const uint16_t* in, size_t 
 
 }  // namespace
 
-// One 32x8 output tile (tile_x, tile_y) of the filter; the whole workgroup
-// (256 lanes) takes part.
-template <typename In>
+// One TX x (256 / TX) output tile (tile_x, tile_y) of the filter; the whole
+// workgroup (256 lanes) takes part.
+template <typename In, int kTX>
 __device__ void filterTile(const In* __restrict__ in, float* __restrict__ out, const FilterArgs& a, int tile_x,
                            int tile_y) {
+  static_assert(kTX == 16 || kTX == 32 || kTX == 64, "tile widths 16, 32, 64");
+  constexpr int kTY = 256 / kTX;
+  constexpr int kDW = kTX + 2 * (kR + 1), kDH = kTY + 2 * (kR + 1);  // depth tile (38x14 at 32x8)
+  constexpr int kNW = kTX + 2 * kR, kNH = kTY + 2 * kR;              // normal tile (36x12 at 32x8)
   __shared__ float sd[kDH][kDW];
   __shared__ float snx[kNH][kNW], sny[kNH][kNW], snz[kNH][kNW];
   __shared__ uint8_t sok[kNH][kNW];
@@ -139,11 +144,12 @@ namespace {
 
 template <typename In>
 __global__ __launch_bounds__(256) void filterKernel(const In* __restrict__ in, float* __restrict__ out, FilterArgs a) {
-  filterTile<In>(in, out, a, blockIdx.x, blockIdx.y);
+  filterTile<In, kDefaultTX>(in, out, a, blockIdx.x, blockIdx.y);
 }
 
 // Batched: workgroup i filters tile i of the concatenation of every scene's
 // tiles (tile_prefix[s] = first tile of scene s), frame `step` of each scene.
+template <int kTX>
 __global__ __launch_bounds__(256) void filterBatchKernel(const BatchScene* __restrict__ scenes, int S,
                                                          const uint32_t* __restrict__ tile_prefix,
                                                          const BatchState* __restrict__ st) {
@@ -153,30 +159,40 @@ __global__ __launch_bounds__(256) void filterBatchKernel(const BatchScene* __res
   if (step >= sc.n_frames) return;
   const uint32_t t = blockIdx.x - tile_prefix[s];
   const uint32_t tiles_x = (sc.fa.K.width + kTX - 1) / kTX;
-  filterTile<uint16_t>(sc.frames + static_cast<size_t>(step) * sc.frame_elems, sc.filtered, sc.fa, t % tiles_x,
-                       t / tiles_x);
+  filterTile<uint16_t, kTX>(sc.frames + static_cast<size_t>(step) * sc.frame_elems, sc.filtered, sc.fa,
+                            t % tiles_x, t / tiles_x);
 }
 
 }  // namespace
 
-uint32_t filterTiles(const Intrinsics& K) {
-  return ((K.width + kTX - 1) / kTX) * ((K.height + kTY - 1) / kTY);
+uint32_t filterTiles(const Intrinsics& K, int tile_w) {
+  const int th = 256 / tile_w;
+  return ((K.width + tile_w - 1) / tile_w) * ((K.height + th - 1) / th);
 }
 
 hipError_t launchFilterBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
-                             const BatchState* st, hipStream_t s) {
-  filterBatchKernel<<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+                             const BatchState* st, int tile_w, hipStream_t s) {
+  if (tile_w == 16)
+    filterBatchKernel<16><<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  else if (tile_w == 64)
+    filterBatchKernel<64><<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  else if (tile_w == 32)
+    filterBatchKernel<32><<<total_tiles, 256, 0, s>>>(scenes, S, tile_prefix, st);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 hipError_t launchFilterF32(const float* in, float* out, const FilterArgs& a, hipStream_t s) {
-  const dim3 grid((a.K.width + kTX - 1) / kTX, (a.K.height + kTY - 1) / kTY);
+  constexpr int kTY = 256 / kDefaultTX;
+  const dim3 grid((a.K.width + kDefaultTX - 1) / kDefaultTX, (a.K.height + kTY - 1) / kTY);
   filterKernel<float><<<grid, 256, 0, s>>>(in, out, a);
   return hipGetLastError();
 }
 
 hipError_t launchFilterU16(const uint16_t* in, float* out, const FilterArgs& a, hipStream_t s) {
-  const dim3 grid((a.K.width + kTX - 1) / kTX, (a.K.height + kTY - 1) / kTY);
+  constexpr int kTY = 256 / kDefaultTX;
+  const dim3 grid((a.K.width + kDefaultTX - 1) / kDefaultTX, (a.K.height + kTY - 1) / kTY);
   filterKernel<uint16_t><<<grid, 256, 0, s>>>(in, out, a);
   return hipGetLastError();
 }

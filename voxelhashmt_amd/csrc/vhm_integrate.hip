@@ -367,19 +367,13 @@ __global__ __launch_bounds__(256) void integrateDeferredKernel(const float* __re
 }
 
 // Batched: the same over the concatenation of every scene's active list
-// (prefix[s] = first entry of scene s); a chunk crossing scenes is split
-// per scene, so queued voxels always belong to the scene being processed.
-__global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __restrict__ scenes, int S,
-                                                            const uint32_t* __restrict__ prefix,
-                                                            const BatchState* __restrict__ st) {
-  __shared__ IntegrateQueue Q;
-  if (threadIdx.x == 0) Q.n = 0;
-  __syncthreads();
-  const uint32_t step = st->step;
-  const uint32_t total = prefix[S];
-  const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
-  uint32_t i = blockIdx.x * chunk;
-  const uint32_t end = i + chunk < total ? i + chunk : total;
+// (prefix[s] = first entry of scene s); a range crossing scenes is split per
+// scene, so queued voxels always belong to the scene being processed.
+// Ranges: chunk 0, one contiguous range of ceil(total / grid) entries per
+// workgroup; chunk > 0, chunk entries at a time from an atomic ticket.
+__device__ __forceinline__ void integrateBatchRange(IntegrateQueue& Q, const BatchScene* __restrict__ scenes, int S,
+                                                    const uint32_t* __restrict__ prefix, uint32_t step, uint32_t i,
+                                                    uint32_t end, uint64_t* exp_tab) {
   while (i < end) {
     const int s = __builtin_amdgcn_readfirstlane(findSegment(prefix, S, i));  // uniform: scene fields in SGPRs
     const BatchScene& sc = scenes[s];
@@ -388,12 +382,37 @@ __global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __
     a.T_wc = sc.poses[step];
     a.T_cw = sc.poses_inv[step];
     const DepthView<float> D{sc.filtered, a.K.width, a.K.height};
-    __shared__ uint64_t exp_tab[32];
-  const NormalParams np = normalParams(a, expTableLds(exp_tab));
+    const NormalParams np = normalParams(a, expTableLds(exp_tab));
     IntegrateTally t;
     integrateRange<Pass::kFast>(Q, D, np, sc.V, a, i - prefix[s], seg_end - prefix[s], t);
     flushTally(sc.V, t);
     i = seg_end;
+  }
+}
+
+__global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __restrict__ scenes, int S,
+                                                            const uint32_t* __restrict__ prefix,
+                                                            BatchState* __restrict__ st, uint32_t chunk) {
+  __shared__ IntegrateQueue Q;
+  __shared__ uint64_t exp_tab[32];
+  __shared__ uint32_t ticket;
+  if (threadIdx.x == 0) Q.n = 0;
+  __syncthreads();
+  const uint32_t step = st->step;
+  const uint32_t total = prefix[S];
+  if (chunk == 0u) {
+    const uint32_t c = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t i = blockIdx.x * c;
+    integrateBatchRange(Q, scenes, S, prefix, step, i, i + c < total ? i + c : total, exp_tab);
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) ticket = atomicAdd(&st->integrate_ticket, chunk);
+    __syncthreads();
+    const uint32_t i = ticket;
+    __syncthreads();  // ticket is rewritten by the next round
+    if (i >= total) break;  // workgroup-uniform
+    integrateBatchRange(Q, scenes, S, prefix, step, i, i + chunk < total ? i + chunk : total, exp_tab);
   }
 }
 
@@ -430,9 +449,9 @@ hipError_t launchIntegrate(const float* depth, const VolumeView& V, const Integr
   return hipGetLastError();
 }
 
-hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t* prefix, const BatchState* st,
-                                int grid, hipStream_t s) {
-  integrateBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix, st);
+hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t* prefix, BatchState* st, int grid,
+                                uint32_t chunk, hipStream_t s) {
+  integrateBatchKernel<<<grid, 256, 0, s>>>(scenes, S, prefix, st, chunk);
   integrateDeferredBatchKernel<<<dim3(64, S), 256, 0, s>>>(scenes, st);
   return hipGetLastError();
 }

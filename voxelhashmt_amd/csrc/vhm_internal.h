@@ -225,8 +225,9 @@ hipError_t launchObservedFlags(const VolumeView& V, uint32_t num_blocks, uint32_
 
 // ---- batched multi-scene step (vhm_batch.hip) ----
 struct BatchState {
-  uint32_t step;  // frame index every scene integrates in the current step
-  uint32_t pad[3];
+  uint32_t step;              // frame index every scene integrates in the current step
+  uint32_t integrate_ticket;  // batched integrator with chunk > 0: next entry to hand out (reset each step)
+  uint32_t pad[2];
 };
 
 struct BatchScene {
@@ -263,18 +264,22 @@ VHM_HD int findSegment(const uint32_t* prefix, int S, uint32_t i) {
   return lo;
 }
 
-uint32_t filterTiles(const Intrinsics& K);
-uint32_t allocTiles(const Intrinsics& K, int stride);
+// Tiles of tile_w x (256 / tile_w) pixels (tile_w 16, 32 or 64).
+uint32_t filterTiles(const Intrinsics& K, int tile_w);
+uint32_t allocTiles(const Intrinsics& K, int stride, int tile_w);
 hipError_t launchFilterBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
-                             const BatchState* st, hipStream_t s);
-hipError_t launchBatchFrameBegin(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s);
+                             const BatchState* st, int tile_w, hipStream_t s);
+hipError_t launchBatchFrameBegin(const BatchScene* scenes, int S, BatchState* st, hipStream_t s);
 hipError_t launchAllocateBatch(const BatchScene* scenes, int S, const uint32_t* tile_prefix, uint32_t total_tiles,
-                               const BatchState* st, hipStream_t s);
+                               const BatchState* st, int tile_w, hipStream_t s);
 hipError_t launchNumberBlocksBatch(const BatchScene* scenes, int S, const BatchState* st, hipStream_t s);
 hipError_t launchActivePrefix(const BatchScene* scenes, int S, const BatchState* st, uint32_t* prefix,
                               hipStream_t s);
-hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t* prefix, const BatchState* st,
-                                int grid, hipStream_t s);
+// chunk 0: workgroup w integrates entries [w c, (w + 1) c) of the
+// concatenated active lists, c = ceil(total / grid); chunk > 0: workgroups
+// take chunk entries at a time with an atomic ticket (st->integrate_ticket).
+hipError_t launchIntegrateBatch(const BatchScene* scenes, int S, const uint32_t* prefix, BatchState* st, int grid,
+                                uint32_t chunk, hipStream_t s);
 hipError_t launchStepIncrement(BatchState* st, hipStream_t s);
 // Incremental meshing of every scene's dirty blocks (prefix: S + 1 scratch).
 hipError_t launchMeshBatch(const BatchScene* scenes, int S, uint32_t* prefix, int grid, float* cand,
