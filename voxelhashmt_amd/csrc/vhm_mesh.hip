@@ -353,13 +353,24 @@ __device__ __forceinline__ void sortPairs(float* v) {
 // clustering (888-907), whose centres go to the lane's LDS column; (3) per
 // cluster in ascending order, the closest slot of each corner, the corner
 // values, completion and the triangles (909-934).
+// The fused pass's count keeps, in the scratch column of a cube with at
+// most kStoreMaxClusters clusters in reach (centres in rows 0..7), the
+// completed corner values of its first two clusters with triangles: rows
+// 16..23, then 8..15.  The emit then only triangulates them (emitStored).
+constexpr int kStoreMaxClusters = 8;
+__device__ __forceinline__ int storedRow(int e) { return 24 - 8 * (e + 1); }
+
 // (3) of processCube: the triangles of the ncl clusters whose centres are
 // centres[0], centres[256], ... (sc: the cube's candidates, rc: its corner
-// voxels in the staged region, (ox, oy, oz): its first voxel).
-template <bool kEmit>
+// voxels in the staged region, (ox, oy, oz): its first voxel).  kStore:
+// also keep the corner values of clusters with triangles in the column
+// `store` (the centres' own column, see storedRow); *nstored receives how
+// many were kept, or 3 when they did not all fit.
+template <bool kEmit, bool kStore = false>
 __device__ __forceinline__ uint32_t clusterTriangles(const MeshShared& S, const float (&sc)[8][kSlots],
                                                      const int (&rc)[8], int ox, int oy, int oz, const MeshArgs& a,
-                                                     const float* __restrict__ centres, int ncl, float* out) {
+                                                     const float* centres, int ncl, float* out,
+                                                     float* store = nullptr, int* nstored = nullptr) {
   const float cx[2] = {(static_cast<float>(ox) + 0.5f) * a.voxel, (static_cast<float>(ox + 1) + 0.5f) * a.voxel};
   const float cy[2] = {(static_cast<float>(oy) + 0.5f) * a.voxel, (static_cast<float>(oy + 1) + 0.5f) * a.voxel};
   const float cz[2] = {(static_cast<float>(oz) + 0.5f) * a.voxel, (static_cast<float>(oz + 1) + 0.5f) * a.voxel};
@@ -389,10 +400,37 @@ __device__ __forceinline__ uint32_t clusterTriangles(const MeshShared& S, const 
     if (completeCube(val, valid)) {
       const uint32_t t = S.num_tris[cubeCase(val, a.iso)];
       if (kEmit && t > 0) emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
+      if (kStore && t > 0) {
+        if (ncl <= kStoreMaxClusters && *nstored < 2) {
+          float* const row = store + 256 * storedRow(*nstored);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) row[256 * c] = val[c];
+          ++*nstored;
+        } else {
+          *nstored = 3;
+        }
+      }
       ntri += t;
     }
   }
   return ntri;
+}
+
+// Emits the triangles of a cube whose count kept the completed corner values
+// of its ns (1 or 2) clusters with triangles in scratch column col.
+__device__ __forceinline__ void emitStored(const MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
+                                           const float* __restrict__ col, int ns, float* out) {
+  const int ox = keyX(key) * kBlockSide + (cube & 7), oy = keyY(key) * kBlockSide + ((cube >> 3) & 7),
+            oz = keyZ(key) * kBlockSide + (cube >> 6);
+  uint32_t ntri = 0;
+  for (int e = 0; e < ns; ++e) {
+    float val[8];
+    const float* const row = col + 256 * storedRow(e);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) val[c] = row[256 * c];
+    emitCube(ox, oy, oz, a.voxel, val, a.iso, out + 9u * ntri);
+    ntri += S.num_tris[cubeCase(val, a.iso)];
+  }
 }
 
 // Emits the triangles of a cube whose in-reach cluster centres a count pass
@@ -422,10 +460,12 @@ __device__ __forceinline__ void emitFromCentres(const MeshShared& S, uint64_t ke
 
 // One cube: candidates, sort, clusters (the in-reach centres go to
 // cand[256 j + lane], their number to ncl_out), and the triangles: counted,
-// and written to out when kEmit.
-template <bool kEmit>
+// and written to out when kEmit; with kStore the corner values of clusters
+// with triangles are kept (clusterTriangles), their number in *nstored.
+template <bool kEmit, bool kStore = false>
 __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int cube, const MeshArgs& a,
-                                                float* __restrict__ cand, float* out, int& ncl_out) {
+                                                float* __restrict__ cand, float* out, int& ncl_out,
+                                                int* nstored = nullptr) {
   const int tid = threadIdx.x;
   const int lx = cube & 7, ly = (cube >> 3) & 7, lz = cube >> 6;
   const int ox = keyX(key) * kBlockSide + lx, oy = keyY(key) * kBlockSide + ly, oz = keyZ(key) * kBlockSide + lz;
@@ -508,7 +548,7 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
   ncl_out = ncl;
 
   // (3) per cluster within reach
-  return clusterTriangles<kEmit>(S, sc, rc, ox, oy, oz, a, cand + tid, ncl, out);
+  return clusterTriangles<kEmit, kStore>(S, sc, rc, ox, oy, oz, a, cand + tid, ncl, out, cand + tid, nstored);
 }
 
 // Copies the per-case triangle counts into LDS (workgroup-wide, 256 lanes).
@@ -666,10 +706,11 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
   for (uint32_t j = tid; j < n; j += 256) {  // one processCube in the code
     const int cube = S.list_cube[j];
     float* const set = cand + kFusedSetWords * (j >> 8);
-    int ncl;
-    const uint32_t c = processCube<false>(S, key, cube, a, set + 256, nullptr, ncl);
+    int ncl, nstored = 0;
+    const uint32_t c = processCube<false, true>(S, key, cube, a, set + 256, nullptr, ncl, &nstored);
     S.cnt[cube] = static_cast<uint8_t>(c);
-    set[tid] = __uint_as_float(static_cast<uint32_t>(cube) | (static_cast<uint32_t>(ncl) << 16));
+    set[tid] = __uint_as_float(static_cast<uint32_t>(cube) | (static_cast<uint32_t>(ncl) << 16) |
+                               (static_cast<uint32_t>(nstored) << 24));
     if (j < 256u) cj0 = c; else cj1 = c;
   }
   __syncthreads();  // every count is in S.cnt, every lane has read its count-list places
@@ -725,10 +766,14 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
       const uint32_t j = S.list_cube[e];
       const float* const set = cand + kFusedSetWords * (j >> 8);
       const uint32_t meta = __float_as_uint(set[j & 255u]);
-      const int cube = static_cast<int>(meta & 0xffffu), ncl = static_cast<int>(meta >> 16);
+      const int cube = static_cast<int>(meta & 0xffffu), ncl = static_cast<int>((meta >> 16) & 0xffu);
+      const int ns = static_cast<int>(meta >> 24);
       const uint32_t start = base + S.list_start[cube];
       const uint32_t c = S.cnt[cube];
-      emitFromCentres(S, key, cube, a, set + 256 + (j & 255u), ncl, tris + 9ull * start);
+      if (ns == 1 || ns == 2)
+        emitStored(S, key, cube, a, set + 256 + (j & 255u), ns, tris + 9ull * start);
+      else
+        emitFromCentres(S, key, cube, a, set + 256 + (j & 255u), ncl, tris + 9ull * start);
       if (tri_owner)
         for (uint32_t t = 0; t < c; ++t) tri_owner[start + t] = item;
     }
