@@ -126,7 +126,7 @@ VHM_HD uint32_t signBitsBoth(int d) { return (1u << (kSignBelow + d)) | (1u << (
 inline float meshNearBase(float voxel, float extract_scale) {
   float tol = voxel * extract_scale;
   tol = tol < 1e-6f ? 1e-6f : tol;
-  return 2.0f * tol + 1.875f * voxel;
+  return 2.0f * tol + 1.75f * voxel;
 }
 
 struct IntegrateArgs {
@@ -139,7 +139,7 @@ struct IntegrateArgs {
   float inv_r, inv_a;
   float inv_fx, inv_fy;  // RN(1 / fx), RN(1 / fy)
   int exact_fx, exact_fy;  // divByConst by fx / fy is exact (host-checked)
-  float near_base;  // marching cubes' near window without its rounding term: 2 extract_tol + 1.875 voxel
+  float near_base;  // marching cubes' near window without its rounding term: 2 extract_tol + 1.75 voxel
 };
 
 // RN(1 / f) and whether divByConst by f equals the division for every

@@ -156,10 +156,11 @@ __device__ bool stageRegion(MeshShared& S, const VolumeView& V, uint64_t key, ui
       // Only a slot that some cluster within reach of iso (processCube,
       // step 3) can select decides a triangle: its value at the cell centre
       // is within extract_tol of the cluster's, which is within reach
-      // (extract_tol + voxel + rounding) of iso, and its value here differs
-      // from that by at most 0.87 voxel plus rounding.  The sign bits count
-      // such slots only.
-      const float near = 2.0f * a.extract_tol + 1.875f * a.voxel +
+      // (extract_tol + 0.875 voxel + rounding) of iso, and its value here
+      // differs from that by at most |n . (centre - cell centre)| <=
+      // (sqrt(3) / 2) voxel < 0.875 voxel plus rounding.  The sign bits
+      // count such slots only.
+      const float near = 2.0f * a.extract_tol + 1.75f * a.voxel +
                          (fabsf(centre.x) + fabsf(centre.y) + fabsf(centre.z) + 4.0f * a.voxel) * 0x1.0p-18f;
 #pragma unroll
       for (int k = 0; k < kSlots; ++k) {
@@ -516,12 +517,13 @@ __device__ __forceinline__ uint32_t processCube(MeshShared& S, uint64_t key, int
 
   // A selected slot lies within extract_tol of its cluster's centre at the
   // cell centre, and its value at a corner centre differs from that by at
-  // most |n . (corner - cell centre)| <= 0.87 voxel plus the rounding of two
-  // dot products of magnitude <= R (the cube's coordinate extent); a cluster
-  // whose centre is farther than extract_tol + voxel + 2^-20 R from iso has
-  // every corner value (and every completion mean of them) on one side of
-  // iso and yields no triangle.
-  const float reach = a.extract_tol + a.voxel +
+  // most |n . (corner - cell centre)| <= |n| (sqrt(3) / 2) voxel < 0.875
+  // voxel (|n| = 1 up to rounding) plus the rounding of two dot products of
+  // magnitude <= R (the cube's coordinate extent); a cluster whose centre
+  // is farther than extract_tol + 0.875 voxel + 2^-20 R from iso has every
+  // corner value (and every completion mean of them) on one side of iso and
+  // yields no triangle.
+  const float reach = a.extract_tol + 0.875f * a.voxel +
                       (fabsf(cc.x) + fabsf(cc.y) + fabsf(cc.z) + 2.0f * a.voxel) * 0x1.0p-20f;
 
   // (2) clusters: a candidate joins the open cluster unless it lies more than
@@ -596,8 +598,7 @@ __device__ __forceinline__ void listAppend(MeshShared& S, bool take, int cube, u
 // 24 clusters x 5 triangles) for the emit pass.  The cubes passing
 // cubeMayCross are compacted first and then spread over the lanes.
 __device__ uint32_t countItem(MeshShared& S, float* __restrict__ cand, const VolumeView& V, uint32_t b, int dir,
-                              const MeshArgs& a,
-                              uint8_t* __restrict__ cube_counts) {
+                              const MeshArgs& a, uint8_t* __restrict__ cube_counts, uint32_t& staged) {
   const uint64_t key = V.pool_keys[b];
   if (!stageRegion(S, V, key, b, dir, a)) {
     cube_counts[threadIdx.x] = 0;
@@ -605,7 +606,7 @@ __device__ uint32_t countItem(MeshShared& S, float* __restrict__ cand, const Vol
     __syncthreads();  // S.any_dir is rewritten by the next item
     return 0u;
   }
-  if (threadIdx.x == 0) atomicAdd(&V.hdr->mesh_items, 1ull);
+  ++staged;  // items staged, added to the header once per workgroup
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int cube = threadIdx.x + 256 * half;
@@ -683,14 +684,14 @@ __device__ void emitItem(MeshShared& S, float* __restrict__ cand, const VolumeVi
 // One item: stage once, count every candidate cube, take the offset, emit.
 __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const VolumeView& V, uint32_t b, int dir,
                               const MeshArgs& a, uint32_t item, uint32_t* total_out, float* tris, uint32_t* tri_owner,
-                              uint32_t capacity, const uint32_t* src8 = nullptr) {
+                              uint32_t capacity, uint32_t& staged, const uint32_t* src8 = nullptr) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t key = V.pool_keys[b];
   if (!stageRegion(S, V, key, b, dir, a, src8)) {  // workgroup-uniform
     __syncthreads();  // the item's LDS is reused by the next item
     return 0u;
   }
-  if (tid == 0) atomicAdd(&V.hdr->mesh_items, 1ull);
+  ++staged;  // items staged, added to the header once per workgroup
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int cube = tid + 256 * half;
@@ -808,13 +809,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   loadCaseCounts(S);
   float* const mine = cand + kMeshFusedCandWords * blockIdx.x;
   const uint32_t items = kDirs * n;
+  uint32_t staged = 0;
   for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
     const uint32_t dir = item / n, i = item % n;
     // an item none of whose 8 sources observed the direction has no triangle
     if (dirs[i] & observed_dirs[i] & (1u << dir))  // workgroup-uniform
-      fusedItem(S, mine, V, list[i], static_cast<int>(dir), a, item, total_out, tris, tri_owner, capacity,
+      fusedItem(S, mine, V, list[i], static_cast<int>(dir), a, item, total_out, tris, tri_owner, capacity, staged,
                 nbr + 8ull * i);
   }
+  if (threadIdx.x == 0 && staged) atomicAdd(&V.hdr->mesh_items, static_cast<unsigned long long>(staged));
 }
 
 // Source block k of listed block i (self and the 7 positive neighbours,
@@ -865,13 +868,15 @@ __global__ __launch_bounds__(256) void meshCountKernel(VolumeView V, const uint3
   loadCaseCounts(S);
   float* const mine = cand + kMeshCandWords * blockIdx.x;
   const uint32_t items = kDirs * n;
+  uint32_t staged = 0;
   for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
     const uint32_t dir = item / n, i = item % n;
     uint32_t total = 0;
     if (!dirs || (dirs[i] & (1u << dir)))  // workgroup-uniform
-      total = countItem(S, mine, V, list ? list[i] : i, static_cast<int>(dir), a, cube_counts + 512ull * item);
+      total = countItem(S, mine, V, list ? list[i] : i, static_cast<int>(dir), a, cube_counts + 512ull * item, staged);
     if (threadIdx.x == 0) counts[item] = total;
   }
+  if (threadIdx.x == 0 && staged) atomicAdd(&V.hdr->mesh_items, static_cast<unsigned long long>(staged));
 }
 
 // Writes the triangles of item dir * n + i at offsets[item] in cube order;
@@ -1127,15 +1132,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   loadCaseCounts(M);
   float* const mine = cand + kMeshFusedCandWords * blockIdx.x;
   const uint32_t total = prefix[S];
+  // items staged, added to the scene's header when the scene changes and at the end
+  uint32_t staged = 0;
+  int staged_scene = -1;
   for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
     const int s = __builtin_amdgcn_readfirstlane(findSegment(prefix, S, i));  // uniform: scene fields in SGPRs
+    if (s != staged_scene) {
+      if (threadIdx.x == 0 && staged)
+        atomicAdd(&scenes[staged_scene].V.hdr->mesh_items, static_cast<unsigned long long>(staged));
+      staged = 0;
+      staged_scene = s;
+    }
     const BatchScene& sc = scenes[s];
     const uint32_t m = sc.out[0], j = i - prefix[s];
     const uint32_t dir = j / m, idx = j % m;
     if (sc.remesh_dirs[idx] & sc.mesh_obs_dirs[idx] & (1u << dir))  // workgroup-uniform
       fusedItem(M, mine, sc.V, sc.remesh_list[idx], static_cast<int>(dir), sc.ma, dir * m + idx, sc.out + 1, sc.tris,
-                sc.tri_owner, sc.tri_capacity, sc.mesh_nbr + 8ull * idx);
+                sc.tri_owner, sc.tri_capacity, staged, sc.mesh_nbr + 8ull * idx);
   }
+  if (threadIdx.x == 0 && staged)
+    atomicAdd(&scenes[staged_scene].V.hdr->mesh_items, static_cast<unsigned long long>(staged));
 }
 
 }  // namespace
