@@ -56,6 +56,7 @@ constexpr int kDnIn = 16;      // input channels (12 used)
 constexpr int kDnHid = 32;
 constexpr int kDnOut = 6;
 constexpr int kDnOutPad = 16;
+constexpr int kDnOutPad8 = 8;  // dY channels in the layer-3 data gradient (4 taps per K = 32)
 constexpr int kGrid = 10;      // padded grid edge
 constexpr int kGridVox = 1000;
 // master parameter offsets (oracle/denoiser_ref.py ORDER: W1 b1 W2 b2 W3 b3)
@@ -85,25 +86,32 @@ __device__ __forceinline__ int padIndex(int m, int t) {
 // for this wavefront's tiles: n-tile `nt` (16 output channels, weights
 // W [27][NTOT][CIN]) and m-tiles mt = mt0 + i * mstride, on
 // v_mfma_f32_16x16x32_bf16 (mfma32): K = 32 per instruction, one tap of 32
-// input channels, or for CIN = 16 two taps (k-groups 0-1: tap
-// 2 tp, 2-3: tap 2 tp + 1; the 28th tap is a zero operand).  A lane loads
-// 8 consecutive channels (16 bytes) of A and of B.
+// input channels, for CIN = 16 two taps (k-groups 0-1: tap 2 st, 2-3: tap
+// 2 st + 1) and for CIN = 8 four taps (k-group g: tap 4 st + g); taps past
+// the 27th are zero operands.  A lane loads 8 consecutive channels
+// (16 bytes) of A and of B.
 template <int CIN, int NTOT, int MT>
 __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* __restrict__ W, int nt, int mt0,
                                             int mstride, v4f (&acc)[MT]) {
-  static_assert(CIN == 16 || CIN == 32, "16 or 32 input channels");
+  static_assert(CIN == 8 || CIN == 16 || CIN == 32, "8, 16 or 32 input channels");
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, kg = lane >> 4;
-  constexpr int kSteps = CIN == 32 ? 27 : 14;
+  constexpr int kTapsPerStep = 32 / CIN;
+  constexpr int kSteps = (27 + kTapsPerStep - 1) / kTapsPerStep;
   for (int st = 0; st < kSteps; ++st) {
     int t, c0;
     bool live = true;
     if constexpr (CIN == 32) {
       t = st;
       c0 = kg * 8;
-    } else {
+    } else if constexpr (CIN == 16) {
       t = 2 * st + (kg >> 1);
       c0 = (kg & 1) * 8;
+      live = t < 27;
+      t = live ? t : 26;
+    } else {
+      t = 4 * st + kg;
+      c0 = 0;
       live = t < 27;
       t = live ? t : 26;
     }
@@ -286,8 +294,10 @@ __global__ __launch_bounds__(256) void dnPackKernel(const float* __restrict__ P,
   if (i < 27 * 16 * 32) {
     const int c = i % 32, n = (i / 32) % 16, t = i / 512;
     W3t[i] = n < 6 ? f2bf(P[kOffW3 + (t * 32 + c) * 6 + n]) : 0;
-    // W3b[t'][n=cin 32][k=o 16]
-    const int o = i % 16, cin = (i / 16) % 32, tt = i / 512;
+  }
+  if (i < 27 * 32 * kDnOutPad8) {
+    // W3b[t'][n=cin 32][k=o 8]
+    const int o = i % kDnOutPad8, cin = (i / kDnOutPad8) % 32, tt = i / (32 * kDnOutPad8);
     W3b[i] = o < 6 ? f2bf(P[kOffW3 + ((26 - tt) * 32 + cin) * 6 + o]) : 0;
   }
 }
@@ -355,7 +365,7 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
     v4f acc[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles32<kDnHid, 16, 2>(gridA, a.w.W3t, 0, wave, kFwdWaves, acc);
+    convTiles32<kDnHid, kDnOutPad, 2>(gridA, a.w.W3t, 0, wave, kFwdWaves, acc);
     const int n = lane & 15;
     if (n < kDnOut) {
       const float bb = a.w.b[kOffB3 + n];
@@ -547,9 +557,10 @@ __device__ void loadInterior(const uint16_t* __restrict__ src, uint16_t* grid) {
 
 // Data gradient through a layer: g = conv(dout padded, flipped weights),
 // masked by the layer input's ReLU (in > 0, read from the channel-major
-// grid inT), written as bf16 [512][32].  16 m-tiles x 2 n-tiles: wavefront w
-// takes n-tile w & 1 and m-tiles (w >> 1) + 4 i.
-template <int CIN>
+// grid inT: padded [32][1000] when kMaskPadded, else [32][512]), written as
+// bf16 [512][32].  16 m-tiles x 2 n-tiles: wavefront w takes n-tile w & 1
+// and m-tiles (w >> 1) + 4 i.
+template <int CIN, bool kMaskPadded>
 __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t* __restrict__ Wb,
                                          const uint16_t* inT, uint16_t* __restrict__ out, int wave) {
   const int lane = threadIdx.x & 63;
@@ -563,41 +574,134 @@ __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int m = ((wave >> 1) + 4 * i) * 16 + (lane >> 4) * 4 + r;
-      const bool on = inT[n * kGridVox + interiorIndex(m)] != 0;
+      const bool on = (kMaskPadded ? inT[n * kGridVox + interiorIndex(m)] : inT[n * 512 + m]) != 0;
       out[m * kDnHid + n] = on ? f2bf(g[i][r]) : 0;
     }
 }
 
+// Layer 3's weight gradient with the taps in N.  dW3[t][c][o] = sum over
+// the 512 voxels v of h2pad[v + k(t)][c] dY[v][o]; with w = v + k(t) - 1
+// (the h2 voxel; h2pad is zero outside the interior) that is the sum over
+// w of h2[w][c] dYpad[w + 2 - k(t)][o] (dYpad: dY with a zero 1-voxel
+// halo).  So one GEMM per block: M = c (2 tiles), N = (t, o) (162 columns,
+// 11 tiles of 16), K = w (16 steps of 32 voxels) -- 352 instructions where
+// the tap-major form with N = o padded to 16 takes 864.  A[i = c][k = w]
+// comes from h2T ([32][512]: a lane's 8 voxels are one x-row, one aligned
+// 16-byte read); B[k = w][n = (t, o)] from the channel-major padded dY grid
+// dyT ([6][1000]): the tap shifts the x-row, one 16-byte read.  Wavefront w
+// takes K steps 8 (w >> 2) .. 8 (w >> 2) + 7 and N-tiles (w & 3) + 4 j
+// (j < 3, tile < 11) for both M-tiles; the two K halves are summed at the
+// end of the kernel (sumDW3Halves).
+constexpr int kDw3Cols = 27 * kDnOut;                 // 162
+constexpr int kDw3Tiles = (kDw3Cols + 15) / 16;       // 11
+__device__ __forceinline__ void accumulateDW3(const uint16_t* h2T, const uint16_t* dyT, int wave,
+                                              v4f (&acc)[3][2]) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, kg = lane >> 4;
+  const int ng = wave & 3, kh = wave >> 2;
+  int boff[3];
+  bool bval[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int n = 16 * (ng + 4 * j) + li;
+    bval[j] = n < kDw3Cols;
+    const int tt = bval[j] ? n / kDnOut : 0, o = bval[j] ? n % kDnOut : 0;
+    const int kx = tt % 3, ky = (tt / 3) % 3, kz = tt / 9;
+    boff[j] = o * kGridVox + ((2 - kz) * kGrid + (2 - ky)) * kGrid + (2 - kx);
+  }
+  for (int kb = 8 * kh; kb < 8 * kh + 8; ++kb) {
+    const int w0 = kb * 32 + kg * 8;  // voxels w0 .. w0 + 7: x = 0..7 of row (y, z)
+    const int y = (w0 >> 3) & 7, z = w0 >> 6;
+    const v8s a0 = *reinterpret_cast<const v8s*>(h2T + li * 512 + w0);
+    const v8s a1 = *reinterpret_cast<const v8s*>(h2T + (16 + li) * 512 + w0);
+    const int row = (z * kGrid + y) * kGrid;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (ng + 4 * j >= kDw3Tiles) break;  // wavefront-uniform
+      v8s bv;
+      __builtin_memcpy(&bv, dyT + boff[j] + row, sizeof(bv));
+      if (!bval[j]) bv = v8s{0, 0, 0, 0, 0, 0, 0, 0};
+      acc[j][0] = mfma32(a0, bv, acc[j][0]);
+      acc[j][1] = mfma32(a1, bv, acc[j][1]);
+    }
+  }
+}
+
+// Adds the K-upper-half wavefronts' (4..7) dW3 tiles into the lower half's
+// (0..3), through LDS (scratch: at least 4 x 6 x 64 v4f).  Workgroup-wide.
+__device__ __forceinline__ void sumDW3Halves(v4f (&acc)[3][2], int wave, v4f* scratch) {
+  const int lane = threadIdx.x & 63;
+  __syncthreads();
+  if (wave >= 4)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) scratch[(((wave - 4) * 3 + j) * 2 + ct) * 64 + lane] = acc[j][ct];
+  __syncthreads();
+  if (wave < 4)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) acc[j][ct] += scratch[((wave * 3 + j) * 2 + ct) * 64 + lane];
+}
+
 // Layer 3: dW3, db3 and dA2 = conv(dY_pad, W3 flipped^T) * (h2 > 0).
 __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t h2T[kDnHid * kGridVox];       // h2, channel-major padded
-  __shared__ __align__(16) uint16_t dyp[kGridVox * kDnOutPad];   // padded dY (bf16, 16 ch)
-  __shared__ __align__(16) uint16_t dyc[512 * kDnOutPad];        // dY [512][16]
+  __shared__ __align__(16) uint16_t h2T[kDnHid * 512];               // h2, channel-major [32][512]
+  __shared__ __align__(16) uint16_t dyp[kGridVox * kDnOutPad8];      // padded dY, channels-last, 8 ch
+  __shared__ __align__(16) uint16_t dyT[kDnOut * kGridVox];          // padded dY, channel-major
   __shared__ float part[kBwdThreads];
-  const int wave = threadIdx.x >> 6;
-  v4f acc[kBwdTaps][2][1];
-  zeroDW<kDnHid, kDnOutPad>(acc);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  v4f acc[3][2];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) acc[j][0] = acc[j][1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
   float db = 0.0f;
-  zeroHaloT<kDnHid>(h2T);
-  zeroHalo<kDnOutPad>(dyp);
+  // halos (and dyp's channels 6, 7) stay zero: only interior channels < 6 are rewritten
+  for (int i = threadIdx.x; i < kGridVox * kDnOutPad8; i += kBwdThreads) dyp[i] = 0;
+  for (int i = threadIdx.x; i < kDnOut * kGridVox; i += kBwdThreads) dyT[i] = 0;
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
-    loadInteriorT<kDnHid>(a.H2 + static_cast<size_t>(b) * 512 * kDnHid, h2T);
+    {  // h2 [512][32] -> channel-major [32][512]
+      const uint16_t* src = a.H2 + static_cast<size_t>(b) * 512 * kDnHid;
+      for (int i = threadIdx.x; i < 512 * kDnHid / 4; i += kBwdThreads) {
+        const int m = (i * 4) / kDnHid, c = (i * 4) % kDnHid;
+        const uint2 q = *reinterpret_cast<const uint2*>(src + m * kDnHid + c);
+        h2T[(c + 0) * 512 + m] = static_cast<uint16_t>(q.x & 0xffffu);
+        h2T[(c + 1) * 512 + m] = static_cast<uint16_t>(q.x >> 16);
+        h2T[(c + 2) * 512 + m] = static_cast<uint16_t>(q.y & 0xffffu);
+        h2T[(c + 3) * 512 + m] = static_cast<uint16_t>(q.y >> 16);
+      }
+    }
     const float* dYb = a.dY + static_cast<size_t>(b) * 512 * kDnOut;
-    for (int i = threadIdx.x; i < 512 * kDnOutPad; i += kBwdThreads) {
-      const int m = i / kDnOutPad, o = i % kDnOutPad;
-      const uint16_t h = f2bf(o < kDnOut ? dYb[m * kDnOut + o] : 0.0f);
-      dyc[i] = h;
-      dyp[interiorIndex(m) * kDnOutPad + o] = h;
+    for (int i = threadIdx.x; i < 512 * kDnOut; i += kBwdThreads) {
+      const int m = i / kDnOut, o = i % kDnOut;
+      const uint16_t h = f2bf(dYb[i]);
+      const int r = interiorIndex(m);
+      dyp[r * kDnOutPad8 + o] = h;
+      dyT[o * kGridVox + r] = h;
     }
     __syncthreads();
     // db3 in fp32 from dY
     biasGrad<kDnOut, 8>([&](int v, int ch) { return dYb[v * kDnOut + ch]; }, part, db);
-    accumulateDW32<kDnHid, kDnOutPad, false>(h2T, dyc, wave, acc);
-    dataGrad<kDnOutPad>(dyp, a.w.W3b, h2T, a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, wave);
+    accumulateDW3(h2T, dyT, wave, acc);
+    dataGrad<kDnOutPad8, false>(dyp, a.w.W3b, h2T, a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, wave);
   }
+  sumDW3Halves(acc, wave, reinterpret_cast<v4f*>(h2T));
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
-  storeDW<kDnHid, kDnOutPad>(acc, P, kOffW3, kDnHid, kDnOut, wave);
+  if (wave < 4)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int n = 16 * (wave + 4 * j) + (lane & 15);
+      if (n >= kDw3Cols) continue;
+      const int tt = n / kDnOut, o = n % kDnOut;
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int c = ct * 16 + (lane >> 4) * 4 + r;
+          P[kOffW3 + (tt * kDnHid + c) * kDnOut + o] = acc[j][ct][r];
+        }
+    }
   if (threadIdx.x < kDnOut) P[kOffB3 + threadIdx.x] = db;
 }
 
@@ -619,7 +723,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
     __syncthreads();
     biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(dap[interiorIndex(v) * kDnHid + ch]); }, part, db);
     accumulateDW32<kDnHid, kDnHid, true>(h1T, dap, wave, acc);
-    dataGrad<kDnHid>(dap, a.w.W2b, h1T, a.dA1 + static_cast<size_t>(b) * 512 * kDnHid, wave);
+    dataGrad<kDnHid, true>(dap, a.w.W2b, h1T, a.dA1 + static_cast<size_t>(b) * 512 * kDnHid, wave);
   }
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
   storeDW<kDnHid, kDnHid>(acc, P, kOffW2, kDnHid, kDnHid, wave);

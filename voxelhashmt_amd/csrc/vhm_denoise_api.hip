@@ -97,7 +97,7 @@ int vhm_denoiser_create(int capacity, int batch_capacity, vhm_denoiser** out) {
   A(reinterpret_cast<void**>(&d->W2t), 27 * 32 * 32 * 2);
   A(reinterpret_cast<void**>(&d->W3t), 27 * 16 * 32 * 2);
   A(reinterpret_cast<void**>(&d->W2b), 27 * 32 * 32 * 2);
-  A(reinterpret_cast<void**>(&d->W3b), 27 * 32 * 16 * 2);
+  A(reinterpret_cast<void**>(&d->W3b), 27 * 32 * 8 * 2);
   A(reinterpret_cast<void**>(&d->partial), size_t(d->groups) * np * 4);
   A(reinterpret_cast<void**>(&d->X), cap * kXElems * 2);
   A(reinterpret_cast<void**>(&d->T), cap * kYElems * 4);

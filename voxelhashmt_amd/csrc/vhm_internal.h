@@ -320,7 +320,7 @@ struct DnWeights {       // packed device weights
   const uint16_t* W2t;   // [27][32 n][32 c]
   const uint16_t* W3t;   // [27][16 n][32 c]   (n >= 6 zero)
   const uint16_t* W2b;   // [27][32 n=c][32 k=o]  flipped taps, for dh1 = conv(da2, .)
-  const uint16_t* W3b;   // [27][32 n=c][16 k=o]  flipped taps, for dh2 = conv(dy, .)
+  const uint16_t* W3b;   // [27][32 n=c][8 k=o]   flipped taps, for dh2 = conv(dy, .) (o >= 6 zero)
   const float* b;        // fp32 master parameters (biases are read from here)
 };
 hipError_t dnLaunchGather(const VolumeView& V, const int32_t* keys, uint32_t n, float voxel, float trunc,
