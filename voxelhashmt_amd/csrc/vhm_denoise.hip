@@ -520,19 +520,45 @@ struct DnBwdArgs {
   float* partial;        // [gridDim.x][kNumParams]
 };
 
+// Copies n 16-byte chunks src[0..n) workgroup-wide: every load of a lane is
+// issued before any of its LDS writes (put(i, chunk)), so a block's staging
+// costs one memory round trip instead of one per chunk.
+template <int kMaxPerLane, class Put>
+__device__ __forceinline__ void stage16(const uint4* __restrict__ src, int n, Put put) {
+  uint4 q[kMaxPerLane];
+#pragma unroll
+  for (int k = 0; k < kMaxPerLane; ++k) {
+    const int i = threadIdx.x + k * kBwdThreads;
+    if (i < n) q[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxPerLane; ++k) {
+    const int i = threadIdx.x + k * kBwdThreads;
+    if (i < n) put(i, q[k]);
+  }
+}
+
+// The 8 bf16 of a 16-byte chunk, one per channel, at dst[e * stride]
+// (e = 0..7): a transposing LDS write.
+__device__ __forceinline__ void scatter8(uint16_t* dst, int stride, uint4 q) {
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    dst[(2 * e) * stride] = static_cast<uint16_t>(w[e] & 0xffffu);
+    dst[(2 * e + 1) * stride] = static_cast<uint16_t>(w[e] >> 16);
+  }
+}
+
 // Loads a [512][C] bf16 activation into the interior of a channel-major
 // padded grid ([C][1000]).
 template <int C>
 __device__ void loadInteriorT(const uint16_t* __restrict__ src, uint16_t* gridT) {
-  for (int i = threadIdx.x; i < 512 * C / 4; i += kBwdThreads) {
-    const int m = (i * 4) / C, c = (i * 4) % C;
-    const uint2 q = *reinterpret_cast<const uint2*>(src + m * C + c);
-    const int r = interiorIndex(m);
-    gridT[(c + 0) * kGridVox + r] = static_cast<uint16_t>(q.x & 0xffffu);
-    gridT[(c + 1) * kGridVox + r] = static_cast<uint16_t>(q.x >> 16);
-    gridT[(c + 2) * kGridVox + r] = static_cast<uint16_t>(q.y & 0xffffu);
-    gridT[(c + 3) * kGridVox + r] = static_cast<uint16_t>(q.y >> 16);
-  }
+  constexpr int kChunks = 512 * C / 8;
+  stage16<(kChunks + kBwdThreads - 1) / kBwdThreads>(
+      reinterpret_cast<const uint4*>(src), kChunks, [&](int i, uint4 q) {
+        const int m = (i * 8) / C, c = (i * 8) % C;
+        scatter8(gridT + c * kGridVox + interiorIndex(m), kGridVox, q);
+      });
 }
 
 // Zeroes the halo of a channel-major padded grid ([C][1000]).
@@ -549,16 +575,18 @@ __device__ void zeroHaloT(uint16_t* gridT) {
 // padded grid ([1000][C]).
 template <int C>
 __device__ void loadInterior(const uint16_t* __restrict__ src, uint16_t* grid) {
-  for (int i = threadIdx.x; i < 512 * C / 4; i += kBwdThreads) {
-    const int m = (i * 4) / C, c = (i * 4) % C;
-    *reinterpret_cast<uint2*>(grid + interiorIndex(m) * C + c) = *reinterpret_cast<const uint2*>(src + m * C + c);
-  }
+  constexpr int kChunks = 512 * C / 8;
+  stage16<(kChunks + kBwdThreads - 1) / kBwdThreads>(
+      reinterpret_cast<const uint4*>(src), kChunks, [&](int i, uint4 q) {
+        const int m = (i * 8) / C, c = (i * 8) % C;
+        *reinterpret_cast<uint4*>(grid + interiorIndex(m) * C + c) = q;
+      });
 }
 
 // Data gradient through a layer: g = conv(dout padded, flipped weights),
 // masked by the layer input's ReLU (in > 0, read from the channel-major
 // grid inT: padded [32][1000] when kMaskPadded, else [32][512]), written as
-// bf16 [512][32].  16 m-tiles x 2 n-tiles: wavefront w takes n-tile w & 1
+// bf16 [512][32].  32 m-tiles x 2 n-tiles: wavefront w takes n-tile w & 1
 // and m-tiles (w >> 1) + 4 i.
 template <int CIN, bool kMaskPadded>
 __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t* __restrict__ Wb,
@@ -661,24 +689,36 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
   for (int i = threadIdx.x; i < kDnOut * kGridVox; i += kBwdThreads) dyT[i] = 0;
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
-    {  // h2 [512][32] -> channel-major [32][512]
-      const uint16_t* src = a.H2 + static_cast<size_t>(b) * 512 * kDnHid;
-      for (int i = threadIdx.x; i < 512 * kDnHid / 4; i += kBwdThreads) {
-        const int m = (i * 4) / kDnHid, c = (i * 4) % kDnHid;
-        const uint2 q = *reinterpret_cast<const uint2*>(src + m * kDnHid + c);
-        h2T[(c + 0) * 512 + m] = static_cast<uint16_t>(q.x & 0xffffu);
-        h2T[(c + 1) * 512 + m] = static_cast<uint16_t>(q.x >> 16);
-        h2T[(c + 2) * 512 + m] = static_cast<uint16_t>(q.y & 0xffffu);
-        h2T[(c + 3) * 512 + m] = static_cast<uint16_t>(q.y >> 16);
-      }
-    }
     const float* dYb = a.dY + static_cast<size_t>(b) * 512 * kDnOut;
-    for (int i = threadIdx.x; i < 512 * kDnOut; i += kBwdThreads) {
-      const int m = i / kDnOut, o = i % kDnOut;
-      const uint16_t h = f2bf(dYb[i]);
-      const int r = interiorIndex(m);
-      dyp[r * kDnOutPad8 + o] = h;
-      dyT[o * kGridVox + r] = h;
+    {  // dY [512][6] (fp32, 768 chunks of 4) -> both padded bf16 grids
+      constexpr int kPer = 512 * kDnOut / 4 / kBwdThreads + 1;
+      float4 q[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int i = threadIdx.x + k * kBwdThreads;
+        if (i < 512 * kDnOut / 4) q[k] = reinterpret_cast<const float4*>(dYb)[i];
+      }
+      // h2 [512][32] -> channel-major [32][512]
+      stage16<512 * kDnHid / 8 / kBwdThreads>(
+          reinterpret_cast<const uint4*>(a.H2 + static_cast<size_t>(b) * 512 * kDnHid), 512 * kDnHid / 8,
+          [&](int i, uint4 v) {
+            const int m = (i * 8) / kDnHid, c = (i * 8) % kDnHid;
+            scatter8(h2T + c * 512 + m, 512, v);
+          });
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int i = threadIdx.x + k * kBwdThreads;
+        if (i >= 512 * kDnOut / 4) continue;
+        const float f[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int j = 4 * i + e, m = j / kDnOut, o = j % kDnOut;
+          const uint16_t h = f2bf(f[e]);
+          const int r = interiorIndex(m);
+          dyp[r * kDnOutPad8 + o] = h;
+          dyT[o * kGridVox + r] = h;
+        }
+      }
     }
     __syncthreads();
     // db3 in fp32 from dY
@@ -742,18 +782,20 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
     {
-      const uint16_t* xs = a.X + static_cast<size_t>(b) * kGridVox * kDnIn;
-      for (int i = threadIdx.x; i < kGridVox * kDnIn / 4; i += kBwdThreads) {
-        const int r = (i * 4) / kDnIn, c = (i * 4) % kDnIn;
-        const uint2 q = *reinterpret_cast<const uint2*>(xs + r * kDnIn + c);
-        xT[(c + 0) * kGridVox + r] = static_cast<uint16_t>(q.x & 0xffffu);
-        xT[(c + 1) * kGridVox + r] = static_cast<uint16_t>(q.x >> 16);
-        xT[(c + 2) * kGridVox + r] = static_cast<uint16_t>(q.y & 0xffffu);
-        xT[(c + 3) * kGridVox + r] = static_cast<uint16_t>(q.y >> 16);
-      }
+      constexpr int kXChunks = kGridVox * kDnIn / 8;  // 2000
+      uint4 dq[512 * kDnHid / 8 / kBwdThreads];
       const uint4* s2 = reinterpret_cast<const uint4*>(a.dA1 + static_cast<size_t>(b) * 512 * kDnHid);
+#pragma unroll
+      for (int k = 0; k < 512 * kDnHid / 8 / kBwdThreads; ++k) dq[k] = s2[threadIdx.x + k * kBwdThreads];
+      stage16<(kXChunks + kBwdThreads - 1) / kBwdThreads>(
+          reinterpret_cast<const uint4*>(a.X + static_cast<size_t>(b) * kGridVox * kDnIn), kXChunks,
+          [&](int i, uint4 v) {
+            const int r = (i * 8) / kDnIn, c = (i * 8) % kDnIn;
+            scatter8(xT + c * kGridVox + r, kGridVox, v);
+          });
       uint4* d2 = reinterpret_cast<uint4*>(da);
-      for (int i = threadIdx.x; i < 512 * kDnHid / 8; i += kBwdThreads) d2[i] = s2[i];
+#pragma unroll
+      for (int k = 0; k < 512 * kDnHid / 8 / kBwdThreads; ++k) d2[threadIdx.x + k * kBwdThreads] = dq[k];
     }
     __syncthreads();
     biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(da[v * kDnHid + ch]); }, part, db);
