@@ -74,6 +74,20 @@ VHM_HD float bf2f(uint16_t h) { return __builtin_bit_cast(float, static_cast<uin
 
 namespace {
 
+// Offset of channel c of voxel r in a channels-last padded LDS grid of C
+// channels.  A 32-channel grid is kept as two planes of 16 channels (32
+// bytes per voxel): an operand read takes one 16-byte chunk of 16
+// consecutive voxels, which at a 64-byte voxel stride falls into the same
+// LDS banks four at a time (about 1.6x the read time) and at 32 bytes does
+// not; the tap's offset stays a constant added to the address.
+template <int C>
+__device__ __forceinline__ int gridOff(int r, int c) {
+  if constexpr (C <= 16)
+    return r * C + c;
+  else
+    return (c >> 4) * (kGridVox * 16) + r * 16 + (c & 15);
+}
+
 __device__ __forceinline__ int padIndex(int m, int t) {
   // output voxel m of the 8^3 grid, tap t = kx + 3 (ky + 3 kz): index in the
   // 10^3 padded grid of the input it reads
@@ -120,7 +134,7 @@ __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* 
 #pragma unroll
     for (int i = 0; i < MT; ++i) {
       const int m = (mt0 + i * mstride) * 16 + li;
-      const v8s av = *reinterpret_cast<const v8s*>(in + padIndex(m, t) * CIN + c0);
+      const v8s av = *reinterpret_cast<const v8s*>(in + gridOff<CIN>(padIndex(m, t), c0));
       acc[i] = mfma32(av, b, acc[i]);
     }
   }
@@ -132,7 +146,7 @@ __device__ void zeroHalo(uint16_t* g) {
   for (int r = threadIdx.x; r < kGridVox; r += blockDim.x) {
     const int x = r % kGrid, y = (r / kGrid) % kGrid, z = r / 100;
     if (x == 0 || y == 0 || z == 0 || x == 9 || y == 9 || z == 9)
-      for (int c = 0; c < C; ++c) g[r * C + c] = 0;
+      for (int c = 0; c < C; ++c) g[gridOff<C>(r, c)] = 0;
   }
 }
 
@@ -157,7 +171,7 @@ __device__ void storeHidden(const v4f (&acc)[MT], const float* bias, int nt, int
       float v = acc[i][r] + bb;
       v = v > 0.0f ? v : 0.0f;
       const uint16_t h = f2bf(v);
-      grid[interiorIndex(m) * kDnHid + n] = h;
+      grid[gridOff<kDnHid>(interiorIndex(m), n)] = h;
       if (gout) gout[m * kDnHid + n] = h;
     }
 }
@@ -434,7 +448,8 @@ __device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int v = v0 + e;
-        bv[ot][e] = static_cast<short>(dout[(kPadded ? interiorIndex(v) : v) * COUTP + ot * 16 + li]);
+        bv[ot][e] = static_cast<short>(
+            dout[kPadded ? gridOff<COUTP>(interiorIndex(v), ot * 16 + li) : v * COUTP + ot * 16 + li]);
       }
 #pragma unroll
     for (int j = 0; j < kBwdTaps; ++j) {
@@ -579,7 +594,7 @@ __device__ void loadInterior(const uint16_t* __restrict__ src, uint16_t* grid) {
   stage16<(kChunks + kBwdThreads - 1) / kBwdThreads>(
       reinterpret_cast<const uint4*>(src), kChunks, [&](int i, uint4 q) {
         const int m = (i * 8) / C, c = (i * 8) % C;
-        *reinterpret_cast<uint4*>(grid + interiorIndex(m) * C + c) = q;
+        *reinterpret_cast<uint4*>(grid + gridOff<C>(interiorIndex(m), c)) = q;
       });
 }
 
@@ -761,7 +776,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
     loadInteriorT<kDnHid>(a.H1 + static_cast<size_t>(b) * 512 * kDnHid, h1T);
     loadInterior<kDnHid>(a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, dap);
     __syncthreads();
-    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(dap[interiorIndex(v) * kDnHid + ch]); }, part, db);
+    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(dap[gridOff<kDnHid>(interiorIndex(v), ch)]); }, part, db);
     accumulateDW32<kDnHid, kDnHid, true>(h1T, dap, wave, acc);
     dataGrad<kDnHid, true>(dap, a.w.W2b, h1T, a.dA1 + static_cast<size_t>(b) * 512 * kDnHid, wave);
   }
