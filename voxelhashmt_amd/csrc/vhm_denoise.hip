@@ -112,8 +112,11 @@ __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* 
   const int li = lane & 15, kg = lane >> 4;
   constexpr int kTapsPerStep = 32 / CIN;
   constexpr int kSteps = (27 + kTapsPerStep - 1) / kTapsPerStep;
-  for (int st = 0; st < kSteps; ++st) {
-    int t, c0;
+  int m[MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) m[i] = (mt0 + i * mstride) * 16 + li;
+  // step st's tap and channel offset (taps past the 27th are zero operands)
+  auto tapOf = [&](int st, int& t, int& c0) {
     bool live = true;
     if constexpr (CIN == 32) {
       t = st;
@@ -129,14 +132,27 @@ __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* 
       live = t < 27;
       t = live ? t : 26;
     }
-    v8s b = *reinterpret_cast<const v8s*>(W + (static_cast<size_t>(t) * NTOT + nt * 16 + li) * CIN + c0);
-    if (!live) b = v8s{0, 0, 0, 0, 0, 0, 0, 0};
+    return live;
+  };
+  auto loadB = [&](int st) {
+    int t, c0;
+    const bool live = tapOf(st, t, c0);
+    const v8s b = *reinterpret_cast<const v8s*>(W + (static_cast<size_t>(t) * NTOT + nt * 16 + li) * CIN + c0);
+    return live ? b : v8s{0, 0, 0, 0, 0, 0, 0, 0};
+  };
+  // the weight fragment of step st + 1 (from L2) is requested before step
+  // st's products, so its latency overlaps them
+  v8s b = loadB(0);
+  for (int st = 0; st < kSteps; ++st) {
+    const v8s bn = st + 1 < kSteps ? loadB(st + 1) : b;
+    int t, c0;
+    tapOf(st, t, c0);
 #pragma unroll
     for (int i = 0; i < MT; ++i) {
-      const int m = (mt0 + i * mstride) * 16 + li;
-      const v8s av = *reinterpret_cast<const v8s*>(in + gridOff<CIN>(padIndex(m, t), c0));
+      const v8s av = *reinterpret_cast<const v8s*>(in + gridOff<CIN>(padIndex(m[i], t), c0));
       acc[i] = mfma32(av, b, acc[i]);
     }
+    b = bn;
   }
 }
 
