@@ -88,6 +88,19 @@ __device__ __forceinline__ int gridOff(int r, int c) {
     return (c >> 4) * (kGridVox * 16) + r * 16 + (c & 15);
 }
 
+// Offset of weight (tap t, output n, input channel c) in a packed
+// convolution weight tensor with NTOT outputs and CIN inputs: [27][NTOT][CIN],
+// and for CIN = 32 [27][2][NTOT][16] (16-channel planes, 32 bytes per
+// output row: the B operand's 16 rows of a lane group fall into distinct
+// LDS banks when the tensor is staged in LDS).
+template <int CIN, int NTOT>
+VHM_HD int wOff(int t, int n, int c) {
+  if constexpr (CIN <= 16)
+    return (t * NTOT + n) * CIN + c;
+  else
+    return ((t * 2 + (c >> 4)) * NTOT + n) * 16 + (c & 15);
+}
+
 __device__ __forceinline__ int padIndex(int m, int t) {
   // output voxel m of the 8^3 grid, tap t = kx + 3 (ky + 3 kz): index in the
   // 10^3 padded grid of the input it reads
@@ -137,7 +150,7 @@ __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* 
   auto loadB = [&](int st) {
     int t, c0;
     const bool live = tapOf(st, t, c0);
-    const v8s b = *reinterpret_cast<const v8s*>(W + (static_cast<size_t>(t) * NTOT + nt * 16 + li) * CIN + c0);
+    const v8s b = *reinterpret_cast<const v8s*>(W + wOff<CIN, NTOT>(t, nt * 16 + li, c0));
     return live ? b : v8s{0, 0, 0, 0, 0, 0, 0, 0};
   };
   // the weight fragment of step st + 1 (from L2) is requested before step
@@ -316,14 +329,14 @@ __global__ __launch_bounds__(256) void dnPackKernel(const float* __restrict__ P,
   }
   if (i < 27 * 32 * 32) {
     const int c = i % 32, n = (i / 32) % 32, t = i / 1024;
-    W2t[i] = f2bf(P[kOffW2 + (t * 32 + c) * 32 + n]);
+    W2t[wOff<32, 32>(t, n, c)] = f2bf(P[kOffW2 + (t * 32 + c) * 32 + n]);
     // W2b[t'][n=c_in][k=o]: flipped tap 26 - t', input channel of the forward as output
     const int o = c, cin = n;
-    W2b[i] = f2bf(P[kOffW2 + ((26 - t) * 32 + cin) * 32 + o]);
+    W2b[wOff<32, 32>(t, cin, o)] = f2bf(P[kOffW2 + ((26 - t) * 32 + cin) * 32 + o]);
   }
   if (i < 27 * 16 * 32) {
     const int c = i % 32, n = (i / 32) % 16, t = i / 512;
-    W3t[i] = n < 6 ? f2bf(P[kOffW3 + (t * 32 + c) * 6 + n]) : 0;
+    W3t[wOff<32, 16>(t, n, c)] = n < 6 ? f2bf(P[kOffW3 + (t * 32 + c) * 6 + n]) : 0;
   }
   if (i < 27 * 32 * kDnOutPad8) {
     // W3b[t'][n=cin 32][k=o 8]
@@ -348,46 +361,69 @@ struct DnFwdArgs {
   float inv_count;   // 1 / max(sum M, 1)
 };
 
-// One workgroup of 16 wavefronts per block (4 per SIMD: LDS holds two padded
-// grids, 128 KiB, so one workgroup per CU).  conv1 and conv2: 64 output
-// tiles (2 n-tiles x 32 m-tiles), wavefront w takes n-tile w & 1 and m-tiles
-// (w >> 1) + 8 i; conv3: 32 tiles, m-tiles w and w + 16.
+// One workgroup of 16 wavefronts per block (4 per SIMD; LDS 148 KB, so one
+// workgroup per CU): the padded input grid (16 channels), one padded hidden
+// grid (h1, then h2) and the current layer's weights, staged from L2 once
+// per layer (the largest, conv2's, is 54 KB: streamed per product it would
+// not stay in the 32 KB L1).  conv1 and conv2: 64 output tiles (2 n-tiles x
+// 32 m-tiles), wavefront w takes n-tile w & 1 and m-tiles (w >> 1) + 8 i;
+// conv2 keeps its tiles in registers until every wavefront has read h1,
+// then writes h2 over it.  conv3: 32 tiles, m-tiles w and w + 16.
 constexpr int kFwdThreads = 1024;
 constexpr int kFwdWaves = kFwdThreads / 64;
 
+// dst[0..n) = src[0..n) in 16-byte chunks, every load of a lane issued
+// before its LDS writes.  Workgroup-wide (kFwdThreads lanes).
+template <int kN>
+__device__ __forceinline__ void copyToLds(uint16_t* dst, const uint16_t* __restrict__ src) {
+  constexpr int kChunks = kN / 8, kPer = (kChunks + kFwdThreads - 1) / kFwdThreads;
+  static_assert(kN % 8 == 0, "whole 16-byte chunks");
+  uint4 q[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = threadIdx.x + k * kFwdThreads;
+    if (i < kChunks) q[k] = reinterpret_cast<const uint4*>(src)[i];
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = threadIdx.x + k * kFwdThreads;
+    if (i < kChunks) reinterpret_cast<uint4*>(dst)[i] = q[k];
+  }
+}
+
 __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
-  __shared__ __align__(16) uint16_t gridA[kGridVox * kDnHid];  // X (16 ch) then h2
-  __shared__ __align__(16) uint16_t gridB[kGridVox * kDnHid];  // h1
+  __shared__ __align__(16) uint16_t gridX[kGridVox * kDnIn];   // X
+  __shared__ __align__(16) uint16_t gridH[kGridVox * kDnHid];  // h1, then h2
+  __shared__ __align__(16) uint16_t wbuf[27 * kDnHid * kDnHid];  // this layer's weights
   __shared__ float red[kFwdWaves];
   const int b = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint16_t* Xb = a.X + static_cast<size_t>(b) * kGridVox * kDnIn;
-  {  // X -> gridA (16 channels, 16-byte copies)
-    const uint4* src = reinterpret_cast<const uint4*>(Xb);
-    uint4* dst = reinterpret_cast<uint4*>(gridA);
-    for (int i = threadIdx.x; i < kGridVox * kDnIn / 8; i += kFwdThreads) dst[i] = src[i];
-  }
-  zeroHalo<kDnHid>(gridB);
+  copyToLds<kGridVox * kDnIn>(gridX, Xb);
+  copyToLds<27 * kDnHid * kDnIn>(wbuf, a.w.W1t);
+  zeroHalo<kDnHid>(gridH);  // the interior is rewritten by h1 and h2 only
   __syncthreads();
   {  // conv1 (valid on X)
     v4f acc[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles32<kDnIn, 32, 4>(gridA, a.w.W1t, wave & 1, wave >> 1, 8, acc);
-    storeHidden<4>(acc, a.w.b + kOffB1, wave & 1, wave >> 1, 8, gridB,
+    convTiles32<kDnIn, 32, 4>(gridX, wbuf, wave & 1, wave >> 1, 8, acc);
+    storeHidden<4>(acc, a.w.b + kOffB1, wave & 1, wave >> 1, 8, gridH,
                    a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
   __syncthreads();
-  zeroHalo<kDnHid>(gridA);  // X is dead; gridA becomes h2
+  copyToLds<27 * kDnHid * kDnHid>(wbuf, a.w.W2t);
   __syncthreads();
-  {  // conv2 (same on h1)
+  {  // conv2 (same on h1); h2 replaces h1 once every wavefront is done with it
     v4f acc[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles32<kDnHid, 32, 4>(gridB, a.w.W2t, wave & 1, wave >> 1, 8, acc);
-    storeHidden<4>(acc, a.w.b + kOffB2, wave & 1, wave >> 1, 8, gridA,
+    convTiles32<kDnHid, 32, 4>(gridH, wbuf, wave & 1, wave >> 1, 8, acc);
+    __syncthreads();
+    storeHidden<4>(acc, a.w.b + kOffB2, wave & 1, wave >> 1, 8, gridH,
                    a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
+  copyToLds<27 * kDnOutPad * kDnHid>(wbuf, a.w.W3t);
   __syncthreads();
   // conv3: 1 n-tile (6 of 16 used) x 32 m-tiles
   float lsum = 0.0f;
@@ -395,7 +431,7 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
     v4f acc[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles32<kDnHid, kDnOutPad, 2>(gridA, a.w.W3t, 0, wave, kFwdWaves, acc);
+    convTiles32<kDnHid, kDnOutPad, 2>(gridH, wbuf, 0, wave, kFwdWaves, acc);
     const int n = lane & 15;
     if (n < kDnOut) {
       const float bb = a.w.b[kOffB3 + n];
