@@ -117,9 +117,9 @@ __device__ __forceinline__ int padIndex(int m, int t) {
 // 2 st + 1) and for CIN = 8 four taps (k-group g: tap 4 st + g); taps past
 // the 27th are zero operands.  A lane loads 8 consecutive channels
 // (16 bytes) of A and of B.
-template <int CIN, int NTOT, int MT>
+template <int CIN, int NTOT, int MT, int NT = 1>
 __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* __restrict__ W, int nt, int mt0,
-                                            int mstride, v4f (&acc)[MT]) {
+                                            int mstride, v4f (&acc)[NT][MT]) {
   static_assert(CIN == 8 || CIN == 16 || CIN == 32, "8, 16 or 32 input channels");
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, kg = lane >> 4;
@@ -147,25 +147,34 @@ __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* 
     }
     return live;
   };
-  auto loadB = [&](int st) {
+  auto loadB = [&](int st, v8s (&b)[NT]) {
     int t, c0;
     const bool live = tapOf(st, t, c0);
-    const v8s b = *reinterpret_cast<const v8s*>(W + wOff<CIN, NTOT>(t, nt * 16 + li, c0));
-    return live ? b : v8s{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const v8s v = *reinterpret_cast<const v8s*>(W + wOff<CIN, NTOT>(t, (nt + j) * 16 + li, c0));
+      b[j] = live ? v : v8s{0, 0, 0, 0, 0, 0, 0, 0};
+    }
   };
-  // the weight fragment of step st + 1 (from L2) is requested before step
-  // st's products, so its latency overlaps them
-  v8s b = loadB(0);
+  // the weight fragments of step st + 1 are requested before step st's
+  // products, so their latency overlaps them
+  v8s b[NT];
+  loadB(0, b);
   for (int st = 0; st < kSteps; ++st) {
-    const v8s bn = st + 1 < kSteps ? loadB(st + 1) : b;
+    v8s bn[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) bn[j] = b[j];
+    if (st + 1 < kSteps) loadB(st + 1, bn);
     int t, c0;
     tapOf(st, t, c0);
 #pragma unroll
     for (int i = 0; i < MT; ++i) {
       const v8s av = *reinterpret_cast<const v8s*>(in + gridOff<CIN>(padIndex(m[i], t), c0));
-      acc[i] = mfma32(av, b, acc[i]);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[j][i] = mfma32(av, b[j], acc[j][i]);
     }
-    b = bn;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) b[j] = bn[j];
   }
 }
 
@@ -185,24 +194,27 @@ __device__ __forceinline__ int interiorIndex(int m) {
 }
 
 // Writes relu(acc + bias) as bf16 into the padded grid (interior) and,
-// optionally, to global memory [512][32].
-template <int MT>
-__device__ void storeHidden(const v4f (&acc)[MT], const float* bias, int nt, int mt0, int mstride, uint16_t* grid,
-                            uint16_t* gout) {
+// optionally, to global memory [512][32]: n-tiles nt .. nt + NT - 1.
+template <int MT, int NT = 1>
+__device__ void storeHidden(const v4f (&acc)[NT][MT], const float* bias, int nt, int mt0, int mstride,
+                            uint16_t* grid, uint16_t* gout) {
   const int lane = threadIdx.x & 63;
-  const int n = nt * 16 + (lane & 15);
-  const float bb = bias[n];
 #pragma unroll
-  for (int i = 0; i < MT; ++i)
+  for (int j = 0; j < NT; ++j) {
+    const int n = (nt + j) * 16 + (lane & 15);
+    const float bb = bias[n];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = (mt0 + i * mstride) * 16 + (lane >> 4) * 4 + r;
-      float v = acc[i][r] + bb;
-      v = v > 0.0f ? v : 0.0f;
-      const uint16_t h = f2bf(v);
-      grid[gridOff<kDnHid>(interiorIndex(m), n)] = h;
-      if (gout) gout[m * kDnHid + n] = h;
-    }
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = (mt0 + i * mstride) * 16 + (lane >> 4) * 4 + r;
+        float v = acc[j][i][r] + bb;
+        v = v > 0.0f ? v : 0.0f;
+        const uint16_t h = f2bf(v);
+        grid[gridOff<kDnHid>(interiorIndex(m), n)] = h;
+        if (gout) gout[m * kDnHid + n] = h;
+      }
+  }
 }
 
 }  // namespace
@@ -371,6 +383,7 @@ struct DnFwdArgs {
 // then writes h2 over it.  conv3: 32 tiles, m-tiles w and w + 16.
 constexpr int kFwdThreads = 1024;
 constexpr int kFwdWaves = kFwdThreads / 64;
+constexpr int kConvWaves = 8;  // wavefronts running conv1 / conv2: both n-tiles of 4 m-tiles each
 
 // dst[0..n) = src[0..n) in 16-byte chunks, every load of a lane issued
 // before its LDS writes.  Workgroup-wide (kFwdThreads lanes).
@@ -403,35 +416,31 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
   copyToLds<27 * kDnHid * kDnIn>(wbuf, a.w.W1t);
   zeroHalo<kDnHid>(gridH);  // the interior is rewritten by h1 and h2 only
   __syncthreads();
-  {  // conv1 (valid on X)
-    v4f acc[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles32<kDnIn, 32, 4>(gridX, wbuf, wave & 1, wave >> 1, 8, acc);
-    storeHidden<4>(acc, a.w.b + kOffB1, wave & 1, wave >> 1, 8, gridH,
-                   a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+  if (wave < kConvWaves) {  // conv1 (valid on X)
+    v4f acc[2][4] = {};
+    convTiles32<kDnIn, 32, 4, 2>(gridX, wbuf, 0, wave, kConvWaves, acc);
+    storeHidden<4, 2>(acc, a.w.b + kOffB1, 0, wave, kConvWaves, gridH,
+                      a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
   __syncthreads();
   copyToLds<27 * kDnHid * kDnHid>(wbuf, a.w.W2t);
   __syncthreads();
   {  // conv2 (same on h1); h2 replaces h1 once every wavefront is done with it
-    v4f acc[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles32<kDnHid, 32, 4>(gridH, wbuf, wave & 1, wave >> 1, 8, acc);
+    v4f acc[2][4] = {};
+    if (wave < kConvWaves) convTiles32<kDnHid, 32, 4, 2>(gridH, wbuf, 0, wave, kConvWaves, acc);
     __syncthreads();
-    storeHidden<4>(acc, a.w.b + kOffB2, wave & 1, wave >> 1, 8, gridH,
-                   a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+    if (wave < kConvWaves)
+      storeHidden<4, 2>(acc, a.w.b + kOffB2, 0, wave, kConvWaves, gridH,
+                        a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
   copyToLds<27 * kDnOutPad * kDnHid>(wbuf, a.w.W3t);
   __syncthreads();
   // conv3: 1 n-tile (6 of 16 used) x 32 m-tiles
   float lsum = 0.0f;
   {
-    v4f acc[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) acc[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    convTiles32<kDnHid, kDnOutPad, 2>(gridH, wbuf, 0, wave, kFwdWaves, acc);
+    v4f acc3[1][2] = {};
+    convTiles32<kDnHid, kDnOutPad, 2>(gridH, wbuf, 0, wave, kFwdWaves, acc3);
+    const v4f (&acc)[2] = acc3[0];
     const int n = lane & 15;
     if (n < kDnOut) {
       const float bb = a.w.b[kOffB3 + n];
@@ -659,10 +668,9 @@ template <int CIN, bool kMaskPadded>
 __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t* __restrict__ Wb,
                                          const uint16_t* inT, uint16_t* __restrict__ out, int wave) {
   const int lane = threadIdx.x & 63;
-  v4f g[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) g[i] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-  convTiles32<CIN, 32, 8>(doutPad, Wb, wave & 1, wave >> 1, 4, g);
+  v4f g8[1][8] = {};
+  convTiles32<CIN, 32, 8>(doutPad, Wb, wave & 1, wave >> 1, 4, g8);
+  const v4f (&g)[8] = g8[0];
   const int n = (wave & 1) * 16 + (lane & 15);
 #pragma unroll
   for (int i = 0; i < 8; ++i)
