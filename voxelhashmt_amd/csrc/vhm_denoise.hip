@@ -662,24 +662,26 @@ __device__ void loadInterior(const uint16_t* __restrict__ src, uint16_t* grid) {
 // Data gradient through a layer: g = conv(dout padded, flipped weights),
 // masked by the layer input's ReLU (in > 0, read from the channel-major
 // grid inT: padded [32][1000] when kMaskPadded, else [32][512]), written as
-// bf16 [512][32].  32 m-tiles x 2 n-tiles: wavefront w takes n-tile w & 1
-// and m-tiles (w >> 1) + 4 i.
+// bf16 [512][32].  32 m-tiles x 2 n-tiles: wavefront w takes both n-tiles
+// of m-tiles w + 8 i (each A fragment feeds two products).
 template <int CIN, bool kMaskPadded>
 __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t* __restrict__ Wb,
                                          const uint16_t* inT, uint16_t* __restrict__ out, int wave) {
   const int lane = threadIdx.x & 63;
-  v4f g8[1][8] = {};
-  convTiles32<CIN, 32, 8>(doutPad, Wb, wave & 1, wave >> 1, 4, g8);
-  const v4f (&g)[8] = g8[0];
-  const int n = (wave & 1) * 16 + (lane & 15);
+  v4f g[2][4] = {};
+  convTiles32<CIN, 32, 4, 2>(doutPad, Wb, 0, wave, kBwdWaves, g);
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int j = 0; j < 2; ++j) {
+    const int n = j * 16 + (lane & 15);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = ((wave >> 1) + 4 * i) * 16 + (lane >> 4) * 4 + r;
-      const bool on = (kMaskPadded ? inT[n * kGridVox + interiorIndex(m)] : inT[n * 512 + m]) != 0;
-      out[m * kDnHid + n] = on ? f2bf(g[i][r]) : 0;
-    }
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = (wave + kBwdWaves * i) * 16 + (lane >> 4) * 4 + r;
+        const bool on = (kMaskPadded ? inT[n * kGridVox + interiorIndex(m)] : inT[n * 512 + m]) != 0;
+        out[m * kDnHid + n] = on ? f2bf(g[j][i][r]) : 0;
+      }
+  }
 }
 
 // Layer 3's weight gradient with the taps in N.  dW3[t][c][o] = sum over
