@@ -101,6 +101,32 @@ VHM_HD int wOff(int t, int n, int c) {
     return ((t * 2 + (c >> 4)) * NTOT + n) * 16 + (c & 15);
 }
 
+// The 8 bf16 at base[e .. e + 8) of an LDS array whose base is 16-byte
+// aligned, for any e: two aligned 16-byte reads and a funnel shift
+// (v_alignbyte).  gfx950 serves a 16-byte LDS read that is not 16-byte
+// aligned about 9x slower than an aligned one (measured: 16 against 139
+// bytes per clock per CU), and the weight gradients' tap-shifted x-rows
+// are misaligned for two taps in three.  The array needs 8 elements of
+// slack after its end (read, not used).
+constexpr int kLdsSlack = 8;
+__device__ __forceinline__ v8s lds8(const uint16_t* base, int e) {
+  const int a = e & ~7, s = e & 7;
+  const uint4 lo = *reinterpret_cast<const uint4*>(base + a);
+  const uint4 hi = *reinterpret_cast<const uint4*>(base + a + 8);
+  // words 0..7 of the 32-byte window; skip (s >> 1) whole words in two
+  // selects (by 1, then by 2), then (s & 1) halves in the byte shift
+  // (scalar variables only: small arrays would be moved to LDS or scratch)
+  const bool by1 = (s & 2) != 0, by2 = (s & 4) != 0;
+  const uint32_t r = 2u * static_cast<uint32_t>(s & 1);
+  const uint32_t y0 = by1 ? lo.y : lo.x, y1 = by1 ? lo.z : lo.y, y2 = by1 ? lo.w : lo.z, y3 = by1 ? hi.x : lo.w,
+                 y4 = by1 ? hi.y : hi.x, y5 = by1 ? hi.z : hi.y, y6 = by1 ? hi.w : hi.z;
+  const uint32_t x0 = by2 ? y2 : y0, x1 = by2 ? y3 : y1, x2 = by2 ? y4 : y2, x3 = by2 ? y5 : y3,
+                 x4 = by2 ? y6 : y4;
+  const uint4 o = make_uint4(__builtin_amdgcn_alignbyte(x1, x0, r), __builtin_amdgcn_alignbyte(x2, x1, r),
+                             __builtin_amdgcn_alignbyte(x3, x2, r), __builtin_amdgcn_alignbyte(x4, x3, r));
+  return __builtin_bit_cast(v8s, o);
+}
+
 __device__ __forceinline__ int padIndex(int m, int t) {
   // output voxel m of the 8^3 grid, tap t = kx + 3 (ky + 3 kz): index in the
   // 10^3 padded grid of the input it reads
@@ -494,8 +520,10 @@ constexpr int kBwdTaps = (27 + kBwdWaves - 1) / kBwdWaves;  // taps per wavefron
 // the 8 voxels of a lane's k-group are one x-row, so its 8 values are one
 // (unaligned) 16-byte LDS read.  B (k = v, n = o) is gathered once per 32
 // voxels from dout (channels-last, [512][COUTP], or its padded grid's interior
-// when kPadded) and reused for every tap and input-channel tile.
-template <int CIN, int COUTP, bool kPadded>
+// when kPadded) and reused for every tap and input-channel tile.  kDoutT:
+// dout is channel-major [COUTP][512] instead, and a lane's 8 voxels of B are
+// one 16-byte read.
+template <int CIN, int COUTP, bool kPadded, bool kDoutT = false>
 __device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16_t* dout, int wave,
                                                v4f (&acc)[kBwdTaps][CIN / 16][COUTP / 16]) {
   const int lane = threadIdx.x & 63;
@@ -505,13 +533,18 @@ __device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16
     const int y = (v0 >> 3) & 7, z = v0 >> 6;
     v8s bv[COUTP / 16];
 #pragma unroll
-    for (int ot = 0; ot < COUTP / 16; ++ot)
+    for (int ot = 0; ot < COUTP / 16; ++ot) {
+      if constexpr (kDoutT) {
+        bv[ot] = *reinterpret_cast<const v8s*>(dout + (ot * 16 + li) * 512 + v0);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int v = v0 + e;
-        bv[ot][e] = static_cast<short>(
-            dout[kPadded ? gridOff<COUTP>(interiorIndex(v), ot * 16 + li) : v * COUTP + ot * 16 + li]);
+        for (int e = 0; e < 8; ++e) {
+          const int v = v0 + e;
+          bv[ot][e] = static_cast<short>(
+              dout[kPadded ? gridOff<COUTP>(interiorIndex(v), ot * 16 + li) : v * COUTP + ot * 16 + li]);
+        }
       }
+    }
 #pragma unroll
     for (int j = 0; j < kBwdTaps; ++j) {
       const int t = wave + kBwdWaves * j;
@@ -521,7 +554,7 @@ __device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16
 #pragma unroll
       for (int ct = 0; ct < CIN / 16; ++ct) {
         v8s av;
-        __builtin_memcpy(&av, inT + (ct * 16 + li) * kGridVox + p0, sizeof(av));
+        av = lds8(inT, (ct * 16 + li) * kGridVox + p0);
 #pragma unroll
         for (int ot = 0; ot < COUTP / 16; ++ot) acc[j][ct][ot] = mfma32(av, bv[ot], acc[j][ct][ot]);
       }
@@ -724,7 +757,7 @@ __device__ __forceinline__ void accumulateDW3(const uint16_t* h2T, const uint16_
     for (int j = 0; j < 3; ++j) {
       if (ng + 4 * j >= kDw3Tiles) break;  // wavefront-uniform
       v8s bv;
-      __builtin_memcpy(&bv, dyT + boff[j] + row, sizeof(bv));
+      bv = lds8(dyT, boff[j] + row);
       if (!bval[j]) bv = v8s{0, 0, 0, 0, 0, 0, 0, 0};
       acc[j][0] = mfma32(a0, bv, acc[j][0]);
       acc[j][1] = mfma32(a1, bv, acc[j][1]);
@@ -754,7 +787,7 @@ __device__ __forceinline__ void sumDW3Halves(v4f (&acc)[3][2], int wave, v4f* sc
 __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
   __shared__ __align__(16) uint16_t h2T[kDnHid * 512];               // h2, channel-major [32][512]
   __shared__ __align__(16) uint16_t dyp[kGridVox * kDnOutPad8];      // padded dY, channels-last, 8 ch
-  __shared__ __align__(16) uint16_t dyT[kDnOut * kGridVox];          // padded dY, channel-major
+  __shared__ __align__(16) uint16_t dyT[kDnOut * kGridVox + kLdsSlack];  // padded dY, channel-major
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   v4f acc[3][2];
@@ -824,7 +857,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
 
 // Layer 2: dW2, db2 and dA1 = conv(dA2_pad, W2 flipped^T) * (h1 > 0).
 __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t h1T[kDnHid * kGridVox];  // h1, channel-major padded
+  __shared__ __align__(16) uint16_t h1T[kDnHid * kGridVox + kLdsSlack];  // h1, channel-major padded
   __shared__ __align__(16) uint16_t dap[kGridVox * kDnHid];  // dA2, channels-last padded
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6;
@@ -849,34 +882,46 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
 
 // Layer 1: dW1, db1 from X and dA1.
 __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t xT[kDnIn * kGridVox];  // X, channel-major (halo included)
-  __shared__ __align__(16) uint16_t da[512 * kDnHid];      // dA1 [512][32]
+  __shared__ __align__(16) uint16_t xT[kDnIn * kGridVox + kLdsSlack];  // X, channel-major (halo included)
+  __shared__ __align__(16) uint16_t daT[kDnHid * 512];     // dA1, channel-major [32][512]
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6;
   v4f acc[kBwdTaps][1][2];
   zeroDW<kDnIn, kDnHid>(acc);
   float db = 0.0f;
+  // a block's inputs are fetched into registers one block ahead, so their
+  // memory latency overlaps the previous block's products
+  constexpr int kXChunks = kGridVox * kDnIn / 8, kDChunks = 512 * kDnHid / 8;  // 2000, 2048
+  constexpr int kXPer = (kXChunks + kBwdThreads - 1) / kBwdThreads, kDPer = kDChunks / kBwdThreads;
+  uint4 xq[kXPer], dq[kDPer];
+  auto fetch = [&](int b) {
+    const uint4* xs = reinterpret_cast<const uint4*>(a.X + static_cast<size_t>(b) * kGridVox * kDnIn);
+    const uint4* ds = reinterpret_cast<const uint4*>(a.dA1 + static_cast<size_t>(b) * 512 * kDnHid);
+#pragma unroll
+    for (int k = 0; k < kXPer; ++k) {
+      const int i = threadIdx.x + k * kBwdThreads;
+      if (i < kXChunks) xq[k] = xs[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kDPer; ++k) dq[k] = ds[threadIdx.x + k * kBwdThreads];
+  };
+  if (static_cast<int>(blockIdx.x) < a.n_blocks) fetch(blockIdx.x);
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
-    {
-      constexpr int kXChunks = kGridVox * kDnIn / 8;  // 2000
-      uint4 dq[512 * kDnHid / 8 / kBwdThreads];
-      const uint4* s2 = reinterpret_cast<const uint4*>(a.dA1 + static_cast<size_t>(b) * 512 * kDnHid);
 #pragma unroll
-      for (int k = 0; k < 512 * kDnHid / 8 / kBwdThreads; ++k) dq[k] = s2[threadIdx.x + k * kBwdThreads];
-      stage16<(kXChunks + kBwdThreads - 1) / kBwdThreads>(
-          reinterpret_cast<const uint4*>(a.X + static_cast<size_t>(b) * kGridVox * kDnIn), kXChunks,
-          [&](int i, uint4 v) {
-            const int r = (i * 8) / kDnIn, c = (i * 8) % kDnIn;
-            scatter8(xT + c * kGridVox + r, kGridVox, v);
-          });
-      uint4* d2 = reinterpret_cast<uint4*>(da);
-#pragma unroll
-      for (int k = 0; k < 512 * kDnHid / 8 / kBwdThreads; ++k) d2[threadIdx.x + k * kBwdThreads] = dq[k];
+    for (int k = 0; k < kXPer; ++k) {
+      const int i = threadIdx.x + k * kBwdThreads;
+      if (i < kXChunks) scatter8(xT + ((i * 8) % kDnIn) * kGridVox + (i * 8) / kDnIn, kGridVox, xq[k]);
     }
+#pragma unroll
+    for (int k = 0; k < kDPer; ++k) {
+      const int i = threadIdx.x + k * kBwdThreads;
+      scatter8(daT + ((i * 8) % kDnHid) * 512 + (i * 8) / kDnHid, 512, dq[k]);
+    }
+    if (b + static_cast<int>(gridDim.x) < a.n_blocks) fetch(b + gridDim.x);
     __syncthreads();
-    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(da[v * kDnHid + ch]); }, part, db);
-    accumulateDW32<kDnIn, kDnHid, false>(xT, da, wave, acc);
+    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(daT[ch * 512 + v]); }, part, db);
+    accumulateDW32<kDnIn, kDnHid, false, true>(xT, daT, wave, acc);
   }
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
   storeDW<kDnIn, kDnHid>(acc, P, kOffW1, 12, kDnHid, wave);
