@@ -109,6 +109,11 @@ VHM_HD int wOff(int t, int n, int c) {
 // are misaligned for two taps in three.  The array needs 8 elements of
 // slack after its end (read, not used).
 constexpr int kLdsSlack = 8;
+// Channel stride of the unpadded channel-major LDS buffers ([C][512] of
+// voxels): 520 elements (1,040 bytes).  A B or A operand read takes the same
+// 8 voxels of 16 channels; at a 1,024-byte stride those fall into the same
+// banks (21 bytes per clock per CU measured, against 79 at 1,040).
+constexpr int kChanStride = 520;
 __device__ __forceinline__ v8s lds8(const uint16_t* base, int e) {
   const int a = e & ~7, s = e & 7;
   const uint4 lo = *reinterpret_cast<const uint4*>(base + a);
@@ -521,7 +526,7 @@ constexpr int kBwdTaps = (27 + kBwdWaves - 1) / kBwdWaves;  // taps per wavefron
 // (unaligned) 16-byte LDS read.  B (k = v, n = o) is gathered once per 32
 // voxels from dout (channels-last, [512][COUTP], or its padded grid's interior
 // when kPadded) and reused for every tap and input-channel tile.  kDoutT:
-// dout is channel-major [COUTP][512] instead, and a lane's 8 voxels of B are
+// dout is channel-major (channel stride kChanStride) instead, and a lane's 8 voxels of B are
 // one 16-byte read.
 template <int CIN, int COUTP, bool kPadded, bool kDoutT = false>
 __device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16_t* dout, int wave,
@@ -535,7 +540,7 @@ __device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16
 #pragma unroll
     for (int ot = 0; ot < COUTP / 16; ++ot) {
       if constexpr (kDoutT) {
-        bv[ot] = *reinterpret_cast<const v8s*>(dout + (ot * 16 + li) * 512 + v0);
+        bv[ot] = *reinterpret_cast<const v8s*>(dout + (ot * 16 + li) * kChanStride + v0);
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -694,7 +699,8 @@ __device__ void loadInterior(const uint16_t* __restrict__ src, uint16_t* grid) {
 
 // Data gradient through a layer: g = conv(dout padded, flipped weights),
 // masked by the layer input's ReLU (in > 0, read from the channel-major
-// grid inT: padded [32][1000] when kMaskPadded, else [32][512]), written as
+// grid inT: padded [32][1000] when kMaskPadded, else 512 voxels per channel at
+// stride kChanStride), written as
 // bf16 [512][32].  32 m-tiles x 2 n-tiles: wavefront w takes both n-tiles
 // of m-tiles w + 8 i (each A fragment feeds two products).
 template <int CIN, bool kMaskPadded>
@@ -711,7 +717,7 @@ __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = (wave + kBwdWaves * i) * 16 + (lane >> 4) * 4 + r;
-        const bool on = (kMaskPadded ? inT[n * kGridVox + interiorIndex(m)] : inT[n * 512 + m]) != 0;
+        const bool on = (kMaskPadded ? inT[n * kGridVox + interiorIndex(m)] : inT[n * kChanStride + m]) != 0;
         out[m * kDnHid + n] = on ? f2bf(g[j][i][r]) : 0;
       }
   }
@@ -724,7 +730,7 @@ __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t
 // halo).  So one GEMM per block: M = c (2 tiles), N = (t, o) (162 columns,
 // 11 tiles of 16), K = w (16 steps of 32 voxels) -- 352 instructions where
 // the tap-major form with N = o padded to 16 takes 864.  A[i = c][k = w]
-// comes from h2T ([32][512]: a lane's 8 voxels are one x-row, one aligned
+// comes from h2T ([32][512] at stride kChanStride: a lane's 8 voxels are one x-row, one aligned
 // 16-byte read); B[k = w][n = (t, o)] from the channel-major padded dY grid
 // dyT ([6][1000]): the tap shifts the x-row, one 16-byte read.  Wavefront w
 // takes K steps 8 (w >> 2) .. 8 (w >> 2) + 7 and N-tiles (w & 3) + 4 j
@@ -750,8 +756,8 @@ __device__ __forceinline__ void accumulateDW3(const uint16_t* h2T, const uint16_
   for (int kb = 8 * kh; kb < 8 * kh + 8; ++kb) {
     const int w0 = kb * 32 + kg * 8;  // voxels w0 .. w0 + 7: x = 0..7 of row (y, z)
     const int y = (w0 >> 3) & 7, z = w0 >> 6;
-    const v8s a0 = *reinterpret_cast<const v8s*>(h2T + li * 512 + w0);
-    const v8s a1 = *reinterpret_cast<const v8s*>(h2T + (16 + li) * 512 + w0);
+    const v8s a0 = *reinterpret_cast<const v8s*>(h2T + li * kChanStride + w0);
+    const v8s a1 = *reinterpret_cast<const v8s*>(h2T + (16 + li) * kChanStride + w0);
     const int row = (z * kGrid + y) * kGrid;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -785,7 +791,7 @@ __device__ __forceinline__ void sumDW3Halves(v4f (&acc)[3][2], int wave, v4f* sc
 
 // Layer 3: dW3, db3 and dA2 = conv(dY_pad, W3 flipped^T) * (h2 > 0).
 __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t h2T[kDnHid * 512];               // h2, channel-major [32][512]
+  __shared__ __align__(16) uint16_t h2T[kDnHid * kChanStride];      // h2, channel-major
   __shared__ __align__(16) uint16_t dyp[kGridVox * kDnOutPad8];      // padded dY, channels-last, 8 ch
   __shared__ __align__(16) uint16_t dyT[kDnOut * kGridVox + kLdsSlack];  // padded dY, channel-major
   __shared__ float part[kBwdThreads];
@@ -813,7 +819,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
           reinterpret_cast<const uint4*>(a.H2 + static_cast<size_t>(b) * 512 * kDnHid), 512 * kDnHid / 8,
           [&](int i, uint4 v) {
             const int m = (i * 8) / kDnHid, c = (i * 8) % kDnHid;
-            scatter8(h2T + c * 512 + m, 512, v);
+            scatter8(h2T + c * kChanStride + m, kChanStride, v);
           });
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
@@ -883,7 +889,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
 // Layer 1: dW1, db1 from X and dA1.
 __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
   __shared__ __align__(16) uint16_t xT[kDnIn * kGridVox + kLdsSlack];  // X, channel-major (halo included)
-  __shared__ __align__(16) uint16_t daT[kDnHid * 512];     // dA1, channel-major [32][512]
+  __shared__ __align__(16) uint16_t daT[kDnHid * kChanStride];  // dA1, channel-major
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6;
   v4f acc[kBwdTaps][1][2];
@@ -916,11 +922,11 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
 #pragma unroll
     for (int k = 0; k < kDPer; ++k) {
       const int i = threadIdx.x + k * kBwdThreads;
-      scatter8(daT + ((i * 8) % kDnHid) * 512 + (i * 8) / kDnHid, 512, dq[k]);
+      scatter8(daT + ((i * 8) % kDnHid) * kChanStride + (i * 8) / kDnHid, kChanStride, dq[k]);
     }
     if (b + static_cast<int>(gridDim.x) < a.n_blocks) fetch(b + gridDim.x);
     __syncthreads();
-    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(daT[ch * 512 + v]); }, part, db);
+    biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(daT[ch * kChanStride + v]); }, part, db);
     accumulateDW32<kDnIn, kDnHid, false, true>(xT, daT, wave, acc);
   }
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
