@@ -109,11 +109,14 @@ VHM_HD int wOff(int t, int n, int c) {
 // are misaligned for two taps in three.  The array needs 8 elements of
 // slack after its end (read, not used).
 constexpr int kLdsSlack = 8;
-// Channel stride of the unpadded channel-major LDS buffers ([C][512] of
-// voxels): 520 elements (1,040 bytes).  A B or A operand read takes the same
-// 8 voxels of 16 channels; at a 1,024-byte stride those fall into the same
-// banks (21 bytes per clock per CU measured, against 79 at 1,040).
-constexpr int kChanStride = 520;
+// Channel strides of the channel-major LDS buffers: 528 elements for 512
+// voxels, 1,008 for a padded grid of 1,000.  An operand read takes 8
+// voxels of each of 16 channels; a 16-byte read pattern at channel stride S
+// measured (bytes per clock per CU, 16 wavefronts per CU): S = 1,024 bytes
+// 31, 1,040 112, 1,056 185, 2,000 116, 2,016 191 -- conflict-free when S is
+// 32 bytes off a multiple of 256.
+constexpr int kChanStride = 528;
+constexpr int kGridStride = 1008;
 __device__ __forceinline__ v8s lds8(const uint16_t* base, int e) {
   const int a = e & ~7, s = e & 7;
   const uint4 lo = *reinterpret_cast<const uint4*>(base + a);
@@ -521,7 +524,7 @@ constexpr int kBwdTaps = (27 + kBwdWaves - 1) / kBwdWaves;  // taps per wavefron
 
 // Weight-gradient tiles over one block, on mfma32: dW[t][c][o] += sum over
 // the 512 voxels v of in[pad(v, t)][c] * dout[v][o] for this wavefront's taps.
-// A (i = c, k = v) comes from the channel-major padded grid inT ([CIN][1000]):
+// A (i = c, k = v) comes from the channel-major padded grid inT ([CIN][1000], stride kGridStride):
 // the 8 voxels of a lane's k-group are one x-row, so its 8 values are one
 // (unaligned) 16-byte LDS read.  B (k = v, n = o) is gathered once per 32
 // voxels from dout (channels-last, [512][COUTP], or its padded grid's interior
@@ -559,7 +562,7 @@ __device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16
 #pragma unroll
       for (int ct = 0; ct < CIN / 16; ++ct) {
         v8s av;
-        av = lds8(inT, (ct * 16 + li) * kGridVox + p0);
+        av = lds8(inT, (ct * 16 + li) * kGridStride + p0);
 #pragma unroll
         for (int ot = 0; ot < COUTP / 16; ++ot) acc[j][ct][ot] = mfma32(av, bv[ot], acc[j][ct][ot]);
       }
@@ -664,24 +667,24 @@ __device__ __forceinline__ void scatter8(uint16_t* dst, int stride, uint4 q) {
 }
 
 // Loads a [512][C] bf16 activation into the interior of a channel-major
-// padded grid ([C][1000]).
+// padded grid ([C][1000] at channel stride kGridStride).
 template <int C>
 __device__ void loadInteriorT(const uint16_t* __restrict__ src, uint16_t* gridT) {
   constexpr int kChunks = 512 * C / 8;
   stage16<(kChunks + kBwdThreads - 1) / kBwdThreads>(
       reinterpret_cast<const uint4*>(src), kChunks, [&](int i, uint4 q) {
         const int m = (i * 8) / C, c = (i * 8) % C;
-        scatter8(gridT + c * kGridVox + interiorIndex(m), kGridVox, q);
+        scatter8(gridT + c * kGridStride + interiorIndex(m), kGridStride, q);
       });
 }
 
-// Zeroes the halo of a channel-major padded grid ([C][1000]).
+// Zeroes the halo of a channel-major padded grid ([C][1000] at stride kGridStride).
 template <int C>
 __device__ void zeroHaloT(uint16_t* gridT) {
   for (int r = threadIdx.x; r < kGridVox; r += kBwdThreads) {
     const int x = r % kGrid, y = (r / kGrid) % kGrid, z = r / 100;
     if (x == 0 || y == 0 || z == 0 || x == 9 || y == 9 || z == 9)
-      for (int c = 0; c < C; ++c) gridT[c * kGridVox + r] = 0;
+      for (int c = 0; c < C; ++c) gridT[c * kGridStride + r] = 0;
   }
 }
 
@@ -717,7 +720,7 @@ __device__ __forceinline__ void dataGrad(const uint16_t* doutPad, const uint16_t
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = (wave + kBwdWaves * i) * 16 + (lane >> 4) * 4 + r;
-        const bool on = (kMaskPadded ? inT[n * kGridVox + interiorIndex(m)] : inT[n * kChanStride + m]) != 0;
+        const bool on = (kMaskPadded ? inT[n * kGridStride + interiorIndex(m)] : inT[n * kChanStride + m]) != 0;
         out[m * kDnHid + n] = on ? f2bf(g[j][i][r]) : 0;
       }
   }
@@ -751,7 +754,7 @@ __device__ __forceinline__ void accumulateDW3(const uint16_t* h2T, const uint16_
     bval[j] = n < kDw3Cols;
     const int tt = bval[j] ? n / kDnOut : 0, o = bval[j] ? n % kDnOut : 0;
     const int kx = tt % 3, ky = (tt / 3) % 3, kz = tt / 9;
-    boff[j] = o * kGridVox + ((2 - kz) * kGrid + (2 - ky)) * kGrid + (2 - kx);
+    boff[j] = o * kGridStride + ((2 - kz) * kGrid + (2 - ky)) * kGrid + (2 - kx);
   }
   for (int kb = 8 * kh; kb < 8 * kh + 8; ++kb) {
     const int w0 = kb * 32 + kg * 8;  // voxels w0 .. w0 + 7: x = 0..7 of row (y, z)
@@ -793,7 +796,7 @@ __device__ __forceinline__ void sumDW3Halves(v4f (&acc)[3][2], int wave, v4f* sc
 __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
   __shared__ __align__(16) uint16_t h2T[kDnHid * kChanStride];      // h2, channel-major
   __shared__ __align__(16) uint16_t dyp[kGridVox * kDnOutPad8];      // padded dY, channels-last, 8 ch
-  __shared__ __align__(16) uint16_t dyT[kDnOut * kGridVox + kLdsSlack];  // padded dY, channel-major
+  __shared__ __align__(16) uint16_t dyT[kDnOut * kGridStride + kLdsSlack];  // padded dY, channel-major
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   v4f acc[3][2];
@@ -802,7 +805,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
   float db = 0.0f;
   // halos (and dyp's channels 6, 7) stay zero: only interior channels < 6 are rewritten
   for (int i = threadIdx.x; i < kGridVox * kDnOutPad8; i += kBwdThreads) dyp[i] = 0;
-  for (int i = threadIdx.x; i < kDnOut * kGridVox; i += kBwdThreads) dyT[i] = 0;
+  for (int i = threadIdx.x; i < kDnOut * kGridStride; i += kBwdThreads) dyT[i] = 0;
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
     const float* dYb = a.dY + static_cast<size_t>(b) * 512 * kDnOut;
@@ -832,7 +835,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
           const uint16_t h = f2bf(f[e]);
           const int r = interiorIndex(m);
           dyp[r * kDnOutPad8 + o] = h;
-          dyT[o * kGridVox + r] = h;
+          dyT[o * kGridStride + r] = h;
         }
       }
     }
@@ -863,7 +866,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward3Kernel(DnBwdArgs a) {
 
 // Layer 2: dW2, db2 and dA1 = conv(dA2_pad, W2 flipped^T) * (h1 > 0).
 __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t h1T[kDnHid * kGridVox + kLdsSlack];  // h1, channel-major padded
+  __shared__ __align__(16) uint16_t h1T[kDnHid * kGridStride + kLdsSlack];  // h1, channel-major padded
   __shared__ __align__(16) uint16_t dap[kGridVox * kDnHid];  // dA2, channels-last padded
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6;
@@ -888,7 +891,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
 
 // Layer 1: dW1, db1 from X and dA1.
 __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
-  __shared__ __align__(16) uint16_t xT[kDnIn * kGridVox + kLdsSlack];  // X, channel-major (halo included)
+  __shared__ __align__(16) uint16_t xT[kDnIn * kGridStride + kLdsSlack];  // X, channel-major (halo included)
   __shared__ __align__(16) uint16_t daT[kDnHid * kChanStride];  // dA1, channel-major
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6;
@@ -917,7 +920,7 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
 #pragma unroll
     for (int k = 0; k < kXPer; ++k) {
       const int i = threadIdx.x + k * kBwdThreads;
-      if (i < kXChunks) scatter8(xT + ((i * 8) % kDnIn) * kGridVox + (i * 8) / kDnIn, kGridVox, xq[k]);
+      if (i < kXChunks) scatter8(xT + ((i * 8) % kDnIn) * kGridStride + (i * 8) / kDnIn, kGridStride, xq[k]);
     }
 #pragma unroll
     for (int k = 0; k < kDPer; ++k) {
