@@ -212,13 +212,15 @@ __device__ __forceinline__ void convTiles32(const uint16_t* in, const uint16_t* 
   }
 }
 
-// Zero a padded grid's 488 halo voxels (the 8^3 interior is overwritten).
+// Zero a padded grid's 488 halo voxels (the 8^3 interior is overwritten),
+// 8 channels (16 bytes) per store.
 template <int C>
 __device__ void zeroHalo(uint16_t* g) {
   for (int r = threadIdx.x; r < kGridVox; r += blockDim.x) {
     const int x = r % kGrid, y = (r / kGrid) % kGrid, z = r / 100;
     if (x == 0 || y == 0 || z == 0 || x == 9 || y == 9 || z == 9)
-      for (int c = 0; c < C; ++c) g[gridOff<C>(r, c)] = 0;
+#pragma unroll
+      for (int c = 0; c < C; c += 8) *reinterpret_cast<uint4*>(g + gridOff<C>(r, c)) = make_uint4(0, 0, 0, 0);
   }
 }
 
