@@ -91,8 +91,7 @@ __device__ __forceinline__ int gridOff(int r, int c) {
 // Offset of weight (tap t, output n, input channel c) in a packed
 // convolution weight tensor with NTOT outputs and CIN inputs: [27][NTOT][CIN],
 // and for CIN = 32 [27][2][NTOT][16] (16-channel planes, 32 bytes per
-// output row: the B operand's 16 rows of a lane group fall into distinct
-// LDS banks when the tensor is staged in LDS).
+// output row, the layout of the activation grids).
 template <int CIN, int NTOT>
 VHM_HD int wOff(int t, int n, int c) {
   if constexpr (CIN <= 16)
@@ -409,79 +408,66 @@ struct DnFwdArgs {
   float inv_count;   // 1 / max(sum M, 1)
 };
 
-// One workgroup of 16 wavefronts per block (4 per SIMD; LDS 148 KB, so one
-// workgroup per CU): the padded input grid (16 channels), one padded hidden
-// grid (h1, then h2) and the current layer's weights, staged from L2 once
-// per layer (the largest, conv2's, is 54 KB: streamed per product it would
-// not stay in the 32 KB L1).  conv1 and conv2: 64 output tiles (2 n-tiles x
-// 32 m-tiles), wavefront w takes n-tile w & 1 and m-tiles (w >> 1) + 8 i;
-// conv2 keeps its tiles in registers until every wavefront has read h1,
-// then writes h2 over it.  conv3: 32 tiles, m-tiles w and w + 16.
-constexpr int kFwdThreads = 1024;
+// One workgroup of 8 wavefronts per block with 64 KB of LDS, so two
+// workgroups share a CU and one's memory round trips and barriers overlap
+// the other's products.  One padded grid holds X (16 channels), then h1,
+// then h2 (32 channels as two 16-channel planes): each convolution keeps its
+// output tiles in registers until every wavefront has read its input.  The
+// weight fragments come from L2.  conv1 and conv2: 64 output tiles (2
+// n-tiles x 32 m-tiles), wavefront w takes both n-tiles of m-tiles w + 8 i
+// (each A fragment feeds two products); conv3: 32 tiles, m-tiles w + 8 i.
+constexpr int kFwdThreads = 512;
 constexpr int kFwdWaves = kFwdThreads / 64;
-constexpr int kConvWaves = 8;  // wavefronts running conv1 / conv2: both n-tiles of 4 m-tiles each
-
-// dst[0..n) = src[0..n) in 16-byte chunks, every load of a lane issued
-// before its LDS writes.  Workgroup-wide (kFwdThreads lanes).
-template <int kN>
-__device__ __forceinline__ void copyToLds(uint16_t* dst, const uint16_t* __restrict__ src) {
-  constexpr int kChunks = kN / 8, kPer = (kChunks + kFwdThreads - 1) / kFwdThreads;
-  static_assert(kN % 8 == 0, "whole 16-byte chunks");
-  uint4 q[kPer];
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int i = threadIdx.x + k * kFwdThreads;
-    if (i < kChunks) q[k] = reinterpret_cast<const uint4*>(src)[i];
-  }
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int i = threadIdx.x + k * kFwdThreads;
-    if (i < kChunks) reinterpret_cast<uint4*>(dst)[i] = q[k];
-  }
-}
 
 __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
-  __shared__ __align__(16) uint16_t gridX[kGridVox * kDnIn];   // X
-  __shared__ __align__(16) uint16_t gridH[kGridVox * kDnHid];  // h1, then h2
-  __shared__ __align__(16) uint16_t wbuf[27 * kDnHid * kDnHid];  // this layer's weights
+  __shared__ __align__(16) uint16_t grid[kGridVox * kDnHid];  // X, then h1, then h2
   __shared__ float red[kFwdWaves];
   const int b = blockIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint16_t* Xb = a.X + static_cast<size_t>(b) * kGridVox * kDnIn;
-  copyToLds<kGridVox * kDnIn>(gridX, Xb);
-  copyToLds<27 * kDnHid * kDnIn>(wbuf, a.w.W1t);
-  zeroHalo<kDnHid>(gridH);  // the interior is rewritten by h1 and h2 only
+  {  // X -> grid as [1000][16] (16-byte chunks; a lane's loads first)
+    constexpr int kChunks = kGridVox * kDnIn / 8, kPer = (kChunks + kFwdThreads - 1) / kFwdThreads;
+    uint4 q[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = threadIdx.x + k * kFwdThreads;
+      if (i < kChunks) q[k] = reinterpret_cast<const uint4*>(Xb)[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = threadIdx.x + k * kFwdThreads;
+      if (i < kChunks) reinterpret_cast<uint4*>(grid)[i] = q[k];
+    }
+  }
   __syncthreads();
-  if (wave < kConvWaves) {  // conv1 (valid on X)
+  {  // conv1 (valid on X); h1 replaces X once every wavefront is done with it
     v4f acc[2][4] = {};
-    convTiles32<kDnIn, 32, 4, 2>(gridX, wbuf, 0, wave, kConvWaves, acc);
-    storeHidden<4, 2>(acc, a.w.b + kOffB1, 0, wave, kConvWaves, gridH,
+    convTiles32<kDnIn, 32, 4, 2>(grid, a.w.W1t, 0, wave, kFwdWaves, acc);
+    __syncthreads();
+    zeroHalo<kDnHid>(grid);
+    storeHidden<4, 2>(acc, a.w.b + kOffB1, 0, wave, kFwdWaves, grid,
                       a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
   __syncthreads();
-  copyToLds<27 * kDnHid * kDnHid>(wbuf, a.w.W2t);
-  __syncthreads();
   {  // conv2 (same on h1); h2 replaces h1 once every wavefront is done with it
     v4f acc[2][4] = {};
-    if (wave < kConvWaves) convTiles32<kDnHid, 32, 4, 2>(gridH, wbuf, 0, wave, kConvWaves, acc);
+    convTiles32<kDnHid, 32, 4, 2>(grid, a.w.W2t, 0, wave, kFwdWaves, acc);
     __syncthreads();
-    if (wave < kConvWaves)
-      storeHidden<4, 2>(acc, a.w.b + kOffB2, 0, wave, kConvWaves, gridH,
-                        a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+    storeHidden<4, 2>(acc, a.w.b + kOffB2, 0, wave, kFwdWaves, grid,
+                      a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
   }
-  copyToLds<27 * kDnOutPad * kDnHid>(wbuf, a.w.W3t);
   __syncthreads();
   // conv3: 1 n-tile (6 of 16 used) x 32 m-tiles
   float lsum = 0.0f;
   {
-    v4f acc3[1][2] = {};
-    convTiles32<kDnHid, kDnOutPad, 2>(gridH, wbuf, 0, wave, kFwdWaves, acc3);
-    const v4f (&acc)[2] = acc3[0];
+    v4f acc3[1][4] = {};
+    convTiles32<kDnHid, kDnOutPad, 4>(grid, a.w.W3t, 0, wave, kFwdWaves, acc3);
+    const v4f (&acc)[4] = acc3[0];
     const int n = lane & 15;
     if (n < kDnOut) {
       const float bb = a.w.b[kOffB3 + n];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int m = (wave + kFwdWaves * i) * 16 + (lane >> 4) * 4 + r;
@@ -503,10 +489,7 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
     if (lane == 0) red[wave] = lsum;
     __syncthreads();
     if (threadIdx.x == 0) {
-      float r8[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) r8[i] = red[2 * i] + red[2 * i + 1];
-      a.loss[b] = ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+      a.loss[b] = ((red[0] + red[1]) + (red[2] + red[3])) + ((red[4] + red[5]) + (red[6] + red[7]));
     }
   }
 }
