@@ -702,6 +702,11 @@ __device__ uint32_t fusedItem(MeshShared& S, float* __restrict__ cand, const Vol
   // count: a lane takes places j = tid and tid + 256 of the count list; the
   // centres of each go to scratch set j >> 8, column tid
   const uint32_t n = S.nlist;
+  // No cube can cross (about 4 items in 10): the item has no triangle.
+  // Workgroup-uniform; what the next item's staging writes before its
+  // first barrier (the sources, and the list lengths, already zero) is
+  // nothing this one still reads.
+  if (n == 0u) return 0u;
   uint32_t cj0 = 0u, cj1 = 0u;
 #pragma unroll 1
   for (uint32_t j = tid; j < n; j += 256) {  // one processCube in the code
