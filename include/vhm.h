@@ -195,6 +195,14 @@ int vhm_block_digests(vhm_ctx* ctx, const int32_t* keys, size_t n, uint64_t* out
  * Writes min(capacity, n) triangles; *n_out = n.  tris_out == NULL with
  * capacity 0 only counts.  Synchronous. */
 int vhm_extract_mesh(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capacity, size_t* n_out);
+/* vhm_extract_mesh with tris_out in out_memkind memory.  VHM_DEVICE (a
+ * device pointer of this context's device): the emit pass writes the
+ * triangles in place, with no host copy -- a full mesh kept on the GPU for
+ * rendering or further GPU work.  Same triangles, same order; writes
+ * min(capacity, n) triangles, *n_out = n.  No reference counterpart (the
+ * reference returns the mesh on the host).  Synchronous. */
+int vhm_extract_mesh_to(vhm_ctx* ctx, float iso_level, float* tris_out, size_t capacity, int out_memkind,
+                        size_t* n_out);
 
 /* Incremental extraction over the (block, direction) pairs changed since the
  * previous incremental call: every direction the integrator (or the

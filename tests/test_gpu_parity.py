@@ -222,6 +222,28 @@ def test_determinism(gpu):
         assert np.asarray(a).tobytes() == np.asarray(b).tobytes()
 
 
+def test_full_extraction_into_device_memory(gpu):
+    """vhm_extract_mesh_to with a device buffer writes the bytes the host
+    extraction returns, in the same order; a smaller device buffer receives
+    their first `capacity` triangles."""
+    import torch
+    w = synth.plain_small(n_frames=3)
+    frames = synth.render(w.scene, w.K, w.poses, w.noise)
+    with vhm.Volume(_params(w)) as vol:
+        for i in range(3):
+            vol.integrate(frames[i], w.K, w.poses[i])
+        host = vol.extract_mesh()
+        dev = vol.extract_mesh(device=True)
+        assert len(host) > 1000
+        assert dev.cpu().numpy().tobytes() == host.tobytes()
+        cap = len(host) // 3
+        out = torch.full((cap, 3, 3), float("nan"), dtype=torch.float32, device="cuda")
+        n = vhm.C.c_size_t()
+        assert vhm.lib().vhm_extract_mesh_to(vol._h, 0.0, out.data_ptr(), cap, vhm.VHM_DEVICE, vhm.C.byref(n)) == 0
+        assert n.value == len(host)
+        assert out.cpu().numpy().tobytes() == host[:cap].tobytes()
+
+
 def test_point_cloud_blocks_numbered_in_key_order(gpu):
     """A point cloud's new blocks take pool indices in packed-key order."""
     w = synth.tiny(n_frames=1, seed=4)

@@ -93,6 +93,8 @@ _SIGS = {
     "vhm_download_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "vhm_observed_block_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vhm_extract_mesh": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vhm_extract_mesh_to": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t, C.c_int,
+                                      C.POINTER(C.c_size_t)]),
     "vhm_extract_mesh_incremental": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_size_t,
                                                C.POINTER(C.c_size_t), C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
                                                C.POINTER(C.c_size_t)]),
@@ -398,8 +400,12 @@ class Volume:
             self._check(lib().vhm_download_blocks(self._h, keys.ctypes.data, len(keys), out.ctypes.data))
         return out
 
-    def extract_mesh(self, iso_level: float = 0.0) -> np.ndarray:
-        """Triangles (n, 3, 3) float32."""
+    def extract_mesh(self, iso_level: float = 0.0, device: bool = False):
+        """Triangles (n, 3, 3) float32: a numpy array, or with device=True a
+        torch tensor on the current GPU written in place by the emit pass
+        (vhm_extract_mesh_to, no host copy)."""
+        if device:
+            return self._extract_mesh_device(iso_level)
         n = C.c_size_t()
         self._check(lib().vhm_extract_mesh(self._h, iso_level, None, 0, C.byref(n)))
         out = np.zeros((max(n.value, 1), 3, 3), dtype=np.float32)
@@ -407,6 +413,22 @@ class Volume:
         if n.value:
             self._check(lib().vhm_extract_mesh(self._h, iso_level, out.ctypes.data, n.value, C.byref(m)))
         return out[: n.value]
+
+    def _extract_mesh_device(self, iso_level: float):
+        import torch
+        # capacity: the last mesh's size, else a guess from the block count;
+        # a mesh larger than the guess is extracted once more into a buffer
+        # of its size
+        cap = max(1 << 20, getattr(self, "_tri_hint", 0), 48 * self.block_count())
+        for _ in range(2):
+            out = torch.empty((cap, 3, 3), dtype=torch.float32, device="cuda")
+            n = C.c_size_t()
+            self._check(lib().vhm_extract_mesh_to(self._h, iso_level, out.data_ptr(), cap, VHM_DEVICE, C.byref(n)))
+            if n.value <= cap:
+                self._tri_hint = n.value
+                return out[: n.value]
+            cap = n.value
+        raise VhmError(3, "device mesh extraction: mesh grew between two calls")
 
     def extract_mesh_incremental(self, iso_level: float = 0.0, device_tris=None, device_items=None):
         """Re-meshes the (block, direction) pairs changed since the last

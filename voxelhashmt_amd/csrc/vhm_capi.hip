@@ -871,7 +871,11 @@ int vhm_block_digests(vhm_ctx* c, const int32_t* keys, size_t n, uint64_t* out_h
 }
 
 int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, size_t* n_out) {
-  if (!c || !n_out) return VHM_ERR_INVALID_ARGUMENT;
+  return vhm_extract_mesh_to(c, iso, tris_out, capacity, VHM_HOST, n_out);
+}
+
+int vhm_extract_mesh_to(vhm_ctx* c, float iso, float* tris_out, size_t capacity, int out_memkind, size_t* n_out) {
+  if (!c || !n_out || (out_memkind != VHM_HOST && out_memkind != VHM_DEVICE)) return VHM_ERR_INVALID_ARGUMENT;
   DevHeader h{};
   int rc = readHeader(c, &h);
   if (rc) return rc;
@@ -905,12 +909,21 @@ int vhm_extract_mesh(vhm_ctx* c, float iso, float* tris_out, size_t capacity, si
   VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   *n_out = total;
   if (tris_out && capacity > 0 && total > 0) {
-    if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
-    VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, nullptr, total, c->mesh_cand, c->stream),
-            "mesh emit kernel launch");
-    const size_t m = capacity < total ? capacity : total;
-    VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, m * 9u * sizeof(float), hipMemcpyDeviceToHost, c->stream),
-            "hipMemcpy D2H triangles");
+    if (out_memkind == VHM_DEVICE && capacity >= total) {
+      // the whole mesh fits: the emit pass writes it in place
+      VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, tris_out,
+                                nullptr, total, c->mesh_cand, c->stream),
+              "mesh emit kernel launch");
+    } else {
+      if ((rc = ensure(c, &c->tris, &c->tris_cap, size_t(total) * 9u, "hipMalloc triangles"))) return rc;
+      VHM_TRY(c, launchMeshEmit(view(c), nullptr, nb, ma, c->mesh_counts, c->cube_counts, c->mesh_offsets, c->tris, nullptr, total, c->mesh_cand, c->stream),
+              "mesh emit kernel launch");
+      const size_t m = capacity < total ? capacity : total;
+      VHM_TRY(c, hipMemcpyAsync(tris_out, c->tris, m * 9u * sizeof(float),
+                                out_memkind == VHM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                c->stream),
+              "hipMemcpy triangles");
+    }
   }
   endTimed(c, ev);
   VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");

@@ -17,11 +17,12 @@ Comparison (learned + end-to-end config): unseen seeded rooms in the
 plain-at-scale style (8x8x3 m, 150 primitives, 1280x720, random walk, voxel
 5 mm, truncation 2 cm).  Plain fusion of all 300 frames against fusion of
 30 of them (every 10th) followed by the denoiser on every observed block in
-one launch, both finished by a full marching-cubes extraction; and the same
-30 frames without the denoiser.  Reported per scene: Chamfer distance (mm)
-against the analytic surface, TSDF L1 (mm) on a common block sample, and
-wall time on the GPU (fusion + denoiser + extraction, frames resident in
-HBM).  Both volumes use the same units of trunc and voxel (4 voxels), which
+one launch, both finished by a full marching-cubes extraction into device
+memory; and the same 30 frames without the denoiser.  Reported per scene:
+Chamfer distance (mm) against the analytic surface, TSDF L1 (mm) on a
+common block sample, and wall time on the GPU (fusion + denoiser +
+extraction, frames resident in HBM, the mesh left in HBM; the metrics copy
+it to the host afterwards).  Both volumes use the same units of trunc and voxel (4 voxels), which
 the network's normalised inputs rely on.
 """
 from __future__ import annotations
@@ -205,8 +206,9 @@ def compare_on_scene(dn: Denoiser, w, frames_plain=300, frames_learned=30, apply
     dn.append(vol, keys[:n])
     dn.forward(0, n, fetch=False)
     r = dn.apply(vol, **(apply_kwargs or {}))
-    tris = vol.extract_mesh()
+    tris = vol.extract_mesh(device=True)
     t += time.perf_counter() - t0
+    tris = tris.cpu().numpy()  # for the metrics, after the timed region
     rng = np.random.default_rng(w.noise.seed)
     sample_keys = keys[rng.choice(len(keys), min(l1_blocks, len(keys)), replace=False)] if len(keys) else keys
     out["learned"] = {**_metrics(vol, w, gt, sample_keys, t, tris), **r, "blocks_denoised": int(n)}
@@ -214,16 +216,16 @@ def compare_on_scene(dn: Denoiser, w, frames_plain=300, frames_learned=30, apply
     vol.reset()
     t = fuse(vol, w, frames, few_idx)
     t0 = time.perf_counter()
-    tris = vol.extract_mesh()
+    tris = vol.extract_mesh(device=True)
     t += time.perf_counter() - t0
-    out["few"] = _metrics(vol, w, gt, sample_keys, t, tris)
+    out["few"] = _metrics(vol, w, gt, sample_keys, t, tris.cpu().numpy())
     # plain: every frame
     vol.reset()
     t = fuse(vol, w, frames, range(frames_plain))
     t0 = time.perf_counter()
-    tris = vol.extract_mesh()
+    tris = vol.extract_mesh(device=True)
     t += time.perf_counter() - t0
-    out["plain"] = _metrics(vol, w, gt, sample_keys, t, tris)
+    out["plain"] = _metrics(vol, w, gt, sample_keys, t, tris.cpu().numpy())
     vol.close()
     del frames
     log(f"[compare] seed {w.noise.seed}: plain {frames_plain}f chamfer {out['plain']['chamfer_mm']:.3f} mm "
