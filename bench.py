@@ -405,7 +405,7 @@ def main():
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only (--gpus must be 1); fusion would shard "
+        print(json.dumps({"error": "bench.py measures one GPU only (--gpus must be 1); fusion would shard "
                                    "by block ownership, one volume shard per GPU"}))
         sys.exit(2)
     if args.calibrate_copy:

@@ -11,7 +11,7 @@
 // Options beyond the reference's: --max-blocks N (block pool capacity;
 // default: half the free device memory) and --hash-log2 k (2^k hash
 // buckets; default 22).  --backend accepts auto|cpu|gpu as the reference
-// does; this build integrates on the GPU only, so cpu is refused.
+// does; libvhm integrates on the GPU only, so cpu is refused.
 
 #include <algorithm>
 #include <chrono>
@@ -146,14 +146,14 @@ Options parseArgs(int argc, char** argv) {  // main.cpp:83-152
 }
 
 // TsdfIntegrationBackend (reconstruction_backend.cpp:222-314) with one
-// libvhm volume; the CPU preference has no counterpart in this build.
+// libvhm volume; the CPU preference has no counterpart in libvhm.
 class GpuFusion {
  public:
   GpuFusion(const Options& o, int max_width, int max_height) {
     char msg[256] = {0};
     const bool have_gpu = vhm_query_device(msg, sizeof(msg)) != 0;
     if (o.backend == "cpu")
-      throw std::runtime_error("CPU backend requested, but this build integrates on the GPU only "
+      throw std::runtime_error("CPU backend requested, but libvhm integrates on the GPU only "
                                "(use --backend auto or gpu)");
     if (!have_gpu) {
       if (o.backend == "gpu") throw std::runtime_error(std::string("GPU backend requested but HIP is unavailable: ") + msg);
