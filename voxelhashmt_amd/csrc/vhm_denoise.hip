@@ -505,86 +505,131 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
 
 constexpr int kBwdThreads = 512;
 constexpr int kBwdWaves = kBwdThreads / 64;
-constexpr int kBwdTaps = (27 + kBwdWaves - 1) / kBwdWaves;  // taps per wavefront: t = wave + kBwdWaves j
 
 // Weight-gradient tiles over one block, on mfma32: dW[t][c][o] += sum over
-// the 512 voxels v of in[pad(v, t)][c] * dout[v][o] for this wavefront's taps.
-// A (i = c, k = v) comes from the channel-major padded grid inT ([CIN][1000], stride kGridStride):
-// the 8 voxels of a lane's k-group are one x-row, so its 8 values are one
-// (unaligned) 16-byte LDS read.  B (k = v, n = o) is gathered once per 32
-// voxels from dout (channels-last, [512][COUTP], or its padded grid's interior
-// when kPadded) and reused for every tap and input-channel tile.  kDoutT:
-// dout is channel-major (channel stride kChanStride) instead, and a lane's 8 voxels of B are
-// one 16-byte read.
+// the 512 voxels v of in[pad(v, t)][c] * dout[v][o].  A (i = c, k = v) comes
+// from the channel-major padded grid inT ([CIN][1000], stride kGridStride):
+// the 8 voxels of a lane's k-group are one x-row.  B (k = v, n = o) is
+// gathered once per 32 voxels from dout (channels-last, [512][COUTP], or its
+// padded grid's interior when kPadded) and reused for every tap the
+// wavefront holds; kDoutT: dout is channel-major (channel stride
+// kChanStride) instead, and a lane's 8 voxels of B are one 16-byte read.
+//
+// The taps are taken by rows: the three taps kx = 0, 1, 2 of row group
+// g = ky + 3 kz read the same padded x-row of 10 entries at x = kx .. kx + 7,
+// so one pair of aligned 16-byte reads (ldsRow3) yields all three A operands
+// (one read per three taps instead of one per tap).  Work units: 18 per
+// layer, unit u = (g = u >> 1, h = u & 1).  With 32 input channels h is
+// the input-channel tile and the unit covers both output tiles; with 16
+// input channels it is the output tile.  Wavefront w takes units w, w + 8,
+// w + 16 (< 18), so its units share h = w & 1.  Every tile accumulates the
+// same products in the same order as a tap-by-tap loop would.
+constexpr int kDwUnits = 18;
+constexpr int kDwUnitsPerWave = (kDwUnits + kBwdWaves - 1) / kBwdWaves;  // 3
+template <int CIN>
+constexpr int dwOutTiles() { return CIN == 32 ? 2 : 1; }  // output tiles per unit
+
+// The three 8-element operands base[e + kx .. e + kx + 8), kx = 0, 1, 2, of an
+// LDS array whose base is 16-byte aligned, for an even e: the two aligned
+// 16-byte reads around them, a selection of 5 of their 8 words (e even: the
+// row starts on a word), and one 2-byte funnel shift for kx = 1.  The array
+// needs 8 elements of slack after its end (read, not used).
+__device__ __forceinline__ void ldsRow3(const uint16_t* base, int e, v8s& f0, v8s& f1, v8s& f2) {
+  const int a = e & ~7, s = e & 7;
+  const uint4 lo = *reinterpret_cast<const uint4*>(base + a);
+  const uint4 hi = *reinterpret_cast<const uint4*>(base + a + 8);
+  // words (s >> 1) .. (s >> 1) + 4 of the 32-byte window: skip 2 words, then 1
+  // (scalar variables only: small arrays would be moved to LDS or scratch)
+  const bool by1 = (s & 2) != 0, by2 = (s & 4) != 0;
+  const uint32_t u0 = by2 ? lo.z : lo.x, u1 = by2 ? lo.w : lo.y, u2 = by2 ? hi.x : lo.z, u3 = by2 ? hi.y : lo.w,
+                 u4 = by2 ? hi.z : hi.x, u5 = by2 ? hi.w : hi.y;
+  const uint32_t w0 = by1 ? u1 : u0, w1 = by1 ? u2 : u1, w2 = by1 ? u3 : u2, w3 = by1 ? u4 : u3,
+                 w4 = by1 ? u5 : u4;
+  f0 = __builtin_bit_cast(v8s, make_uint4(w0, w1, w2, w3));
+  f1 = __builtin_bit_cast(v8s, make_uint4(__builtin_amdgcn_alignbyte(w1, w0, 2u), __builtin_amdgcn_alignbyte(w2, w1, 2u),
+                                          __builtin_amdgcn_alignbyte(w3, w2, 2u), __builtin_amdgcn_alignbyte(w4, w3, 2u)));
+  f2 = __builtin_bit_cast(v8s, make_uint4(w1, w2, w3, w4));
+}
+
 template <int CIN, int COUTP, bool kPadded, bool kDoutT = false>
-__device__ __forceinline__ void accumulateDW32(const uint16_t* inT, const uint16_t* dout, int wave,
-                                               v4f (&acc)[kBwdTaps][CIN / 16][COUTP / 16]) {
+__device__ __forceinline__ void accumulateDWRows(const uint16_t* inT, const uint16_t* dout, int wave,
+                                                 v4f (&acc)[kDwUnitsPerWave][3][dwOutTiles<CIN>()]) {
+  static_assert((CIN == 16 || CIN == 32) && COUTP == 32, "16 or 32 input channels, 32 output channels");
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, kg = lane >> 4;
+  const int h = wave & 1;  // the h of every unit of this wavefront
   for (int kb = 0; kb < 16; ++kb) {
     const int v0 = kb * 32 + kg * 8;  // voxels v0 .. v0 + 7: x = 0..7 of row (y, z)
     const int y = (v0 >> 3) & 7, z = v0 >> 6;
-    v8s bv[COUTP / 16];
+    v8s bv[dwOutTiles<CIN>()];
 #pragma unroll
-    for (int ot = 0; ot < COUTP / 16; ++ot) {
+    for (int q = 0; q < dwOutTiles<CIN>(); ++q) {
+      const int ot = CIN == 32 ? q : h;
       if constexpr (kDoutT) {
-        bv[ot] = *reinterpret_cast<const v8s*>(dout + (ot * 16 + li) * kChanStride + v0);
+        bv[q] = *reinterpret_cast<const v8s*>(dout + (ot * 16 + li) * kChanStride + v0);
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int v = v0 + e;
-          bv[ot][e] = static_cast<short>(
+          bv[q][e] = static_cast<short>(
               dout[kPadded ? gridOff<COUTP>(interiorIndex(v), ot * 16 + li) : v * COUTP + ot * 16 + li]);
         }
       }
     }
 #pragma unroll
-    for (int j = 0; j < kBwdTaps; ++j) {
-      const int t = wave + kBwdWaves * j;
-      if (t >= 27) break;
-      const int kx = t % 3, ky = (t / 3) % 3, kz = t / 9;
-      const int p0 = ((z + kz) * kGrid + (y + ky)) * kGrid + kx;
+    for (int j = 0; j < kDwUnitsPerWave; ++j) {
+      const int u = wave + kBwdWaves * j;
+      if (u >= kDwUnits) break;
+      const int g = u >> 1, ky = g % 3, kz = g / 3;
+      const int ct = CIN == 32 ? h : 0;
+      v8s f0, f1, f2;
+      ldsRow3(inT, (ct * 16 + li) * kGridStride + ((z + kz) * kGrid + (y + ky)) * kGrid, f0, f1, f2);
 #pragma unroll
-      for (int ct = 0; ct < CIN / 16; ++ct) {
-        v8s av;
-        av = lds8(inT, (ct * 16 + li) * kGridStride + p0);
-#pragma unroll
-        for (int ot = 0; ot < COUTP / 16; ++ot) acc[j][ct][ot] = mfma32(av, bv[ot], acc[j][ct][ot]);
+      for (int q = 0; q < dwOutTiles<CIN>(); ++q) {
+        acc[j][0][q] = mfma32(f0, bv[q], acc[j][0][q]);
+        acc[j][1][q] = mfma32(f1, bv[q], acc[j][1][q]);
+        acc[j][2][q] = mfma32(f2, bv[q], acc[j][2][q]);
       }
     }
   }
 }
 
-template <int CIN, int COUTP>
-__device__ __forceinline__ void zeroDW(v4f (&acc)[kBwdTaps][CIN / 16][COUTP / 16]) {
+template <int CIN>
+__device__ __forceinline__ void zeroDWRows(v4f (&acc)[kDwUnitsPerWave][3][dwOutTiles<CIN>()]) {
 #pragma unroll
-  for (int j = 0; j < kBwdTaps; ++j)
+  for (int j = 0; j < kDwUnitsPerWave; ++j)
 #pragma unroll
-    for (int c = 0; c < CIN / 16; ++c)
+    for (int kx = 0; kx < 3; ++kx)
 #pragma unroll
-      for (int o = 0; o < COUTP / 16; ++o) acc[j][c][o] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int q = 0; q < dwOutTiles<CIN>(); ++q) acc[j][kx][q] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 }
 
 // Writes the wavefront's weight-gradient tiles into partial row P at offset
 // off ([27][cin][cout] of the master layout): input channels c < cin and
 // output channels o < cout are real.
-template <int CIN, int COUTP>
-__device__ __forceinline__ void storeDW(const v4f (&acc)[kBwdTaps][CIN / 16][COUTP / 16], float* P, int off, int cin,
-                                        int cout, int wave) {
+template <int CIN>
+__device__ __forceinline__ void storeDWRows(const v4f (&acc)[kDwUnitsPerWave][3][dwOutTiles<CIN>()], float* P,
+                                            int off, int cin, int cout, int wave) {
   const int lane = threadIdx.x & 63;
+  const int h = wave & 1;
 #pragma unroll
-  for (int j = 0; j < kBwdTaps; ++j) {
-    const int t = wave + kBwdWaves * j;
-    if (t >= 27) break;
+  for (int j = 0; j < kDwUnitsPerWave; ++j) {
+    const int u = wave + kBwdWaves * j;
+    if (u >= kDwUnits) break;
+    const int g = u >> 1, ct = CIN == 32 ? h : 0;
 #pragma unroll
-    for (int ct = 0; ct < CIN / 16; ++ct)
+    for (int kx = 0; kx < 3; ++kx) {
+      const int t = kx + 3 * g;
 #pragma unroll
-      for (int ot = 0; ot < COUTP / 16; ++ot)
+      for (int q = 0; q < dwOutTiles<CIN>(); ++q) {
+        const int ot = CIN == 32 ? q : h;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int c = ct * 16 + (lane >> 4) * 4 + r, o = ot * 16 + (lane & 15);
-          if (c < cin && o < cout) P[off + (t * cin + c) * cout + o] = acc[j][ct][ot][r];
+          if (c < cin && o < cout) P[off + (t * cin + c) * cout + o] = acc[j][kx][q][r];
         }
+      }
+    }
   }
 }
 
@@ -855,8 +900,8 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
   __shared__ __align__(16) uint16_t dap[kGridVox * kDnHid];  // dA2, channels-last padded
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6;
-  v4f acc[kBwdTaps][2][2];
-  zeroDW<kDnHid, kDnHid>(acc);
+  v4f acc[kDwUnitsPerWave][3][dwOutTiles<kDnHid>()];
+  zeroDWRows<kDnHid>(acc);
   float db = 0.0f;
   zeroHaloT<kDnHid>(h1T);
   zeroHalo<kDnHid>(dap);
@@ -866,11 +911,11 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
     loadInterior<kDnHid>(a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, dap);
     __syncthreads();
     biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(dap[gridOff<kDnHid>(interiorIndex(v), ch)]); }, part, db);
-    accumulateDW32<kDnHid, kDnHid, true>(h1T, dap, wave, acc);
+    accumulateDWRows<kDnHid, kDnHid, true>(h1T, dap, wave, acc);
     dataGrad<kDnHid, true>(dap, a.w.W2b, h1T, a.dA1 + static_cast<size_t>(b) * 512 * kDnHid, wave);
   }
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
-  storeDW<kDnHid, kDnHid>(acc, P, kOffW2, kDnHid, kDnHid, wave);
+  storeDWRows<kDnHid>(acc, P, kOffW2, kDnHid, kDnHid, wave);
   if (threadIdx.x < kDnHid) P[kOffB2 + threadIdx.x] = db;
 }
 
@@ -880,8 +925,8 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
   __shared__ __align__(16) uint16_t daT[kDnHid * kChanStride];  // dA1, channel-major
   __shared__ float part[kBwdThreads];
   const int wave = threadIdx.x >> 6;
-  v4f acc[kBwdTaps][1][2];
-  zeroDW<kDnIn, kDnHid>(acc);
+  v4f acc[kDwUnitsPerWave][3][dwOutTiles<kDnIn>()];
+  zeroDWRows<kDnIn>(acc);
   float db = 0.0f;
   // a block's inputs are fetched into registers one block ahead, so their
   // memory latency overlaps the previous block's products
@@ -915,10 +960,10 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
     if (b + static_cast<int>(gridDim.x) < a.n_blocks) fetch(b + gridDim.x);
     __syncthreads();
     biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(daT[ch * kChanStride + v]); }, part, db);
-    accumulateDW32<kDnIn, kDnHid, false, true>(xT, daT, wave, acc);
+    accumulateDWRows<kDnIn, kDnHid, false, true>(xT, daT, wave, acc);
   }
   float* P = a.partial + static_cast<size_t>(blockIdx.x) * kNumParams;
-  storeDW<kDnIn, kDnHid>(acc, P, kOffW1, 12, kDnHid, wave);
+  storeDWRows<kDnIn>(acc, P, kOffW1, 12, kDnHid, wave);
   if (threadIdx.x < kDnHid) P[kOffB1 + threadIdx.x] = db;
 }
 
