@@ -495,8 +495,6 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
 }
 
 // ------------------------------------------------------------- backward --
-
-// ------------------------------------------------------------- backward --
 //
 // Per layer one kernel of kBwdThreads lanes per workgroup, persistent over
 // blocks (workgroup g takes blocks g, g + groups, ...), accumulating its
