@@ -974,7 +974,10 @@ __global__ __launch_bounds__(256) void dnReduceKernel(const float* __restrict__ 
   const int pi = threadIdx.x & 63, sl = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + pi;
   float s = 0.0f;
+  // unrolled so that 8 rows' loads are in flight together (the sum keeps
+  // its order)
   if (i < kNumParams)
+#pragma unroll 8
     for (int g = sl; g < groups; g += kReduceSlices) s += partial[static_cast<size_t>(g) * kNumParams + i];
   part[sl][pi] = s;
   __syncthreads();
