@@ -250,7 +250,11 @@ int vhm_incremental_mesh_copy(vhm_ctx* ctx, float* tris_out, uint32_t* tri_item_
 
 /* Per-kernel GPU time (ms, HIP events on the context stream) accumulated
  * while timing is enabled: [0] filter, [1] allocate, [2] integrate, [3] mesh;
- * launches[i] counts launches.  Enabling resets the accumulators. */
+ * launches[i] counts launches.  Enabling resets the accumulators.
+ * enabled: 0 off, 1 every stage, or VHM_TIMING_STAGES | (1 << i for each
+ * stage i to time): each timed stage records two events per launch, which
+ * leave a few microseconds between the launches. */
+#define VHM_TIMING_STAGES 0x100
 int vhm_set_timing(vhm_ctx* ctx, int enabled);
 int vhm_kernel_times(vhm_ctx* ctx, double ms_out[4], uint64_t launches_out[4]);
 

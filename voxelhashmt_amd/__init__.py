@@ -462,8 +462,17 @@ class Volume:
                                                         C.byref(n)))
         return keys[: nb.value].copy(), dirs[: nb.value].copy(), tris[: nt.value].copy(), items[: nt.value].copy()
 
-    def set_timing(self, enabled: bool):
-        self._check(lib().vhm_set_timing(self._h, int(enabled)))
+    TIMING_STAGES = ("filter", "allocate", "integrate", "mesh")
+
+    def set_timing(self, enabled: bool, stages=None):
+        """Per-stage GPU timing with HIP events (vhm_set_timing); stages: the
+        names to time (default all), each costing two events per launch."""
+        mode = int(bool(enabled))
+        if enabled and stages is not None:
+            mode = 0x100
+            for s in stages:
+                mode |= 1 << self.TIMING_STAGES.index(s)
+        self._check(lib().vhm_set_timing(self._h, mode))
 
     def kernel_times(self) -> dict:
         ms = (C.c_double * 4)()

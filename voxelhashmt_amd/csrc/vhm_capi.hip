@@ -160,6 +160,7 @@ struct vhm_ctx {
   size_t inc_tris = 0;  // triangles of the last host-output incremental extraction
   // timing
   bool timing = false;
+  uint32_t timed_stages = 0;  // bit i: stage i records events
   struct Ev {
     hipEvent_t a, b;
     int kind;
@@ -248,17 +249,21 @@ int checkFrame(vhm_ctx* c, const void* depth, const vhm_intrinsics* K) {
   return VHM_OK;
 }
 
+// Timing events only read timestamps: created without the system-scope
+// fence on record, which would write back and invalidate the caches
+// between the stages.
 void beginTimed(vhm_ctx* c, int kind, hipEvent_t* b) {
-  if (!c->timing) return;
+  *b = nullptr;
+  if (!c->timing || !(c->timed_stages & (1u << kind))) return;
   hipEvent_t a;
-  hipEventCreate(&a);
-  hipEventCreate(b);
+  hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
+  hipEventCreateWithFlags(b, hipEventDisableSystemFence);
   hipEventRecord(a, c->stream);
   c->events.push_back({a, *b, kind});
 }
 
 void endTimed(vhm_ctx* c, hipEvent_t b) {
-  if (c->timing) hipEventRecord(b, c->stream);
+  if (c->timing && b) hipEventRecord(b, c->stream);
 }
 
 void drainEvents(vhm_ctx* c) {
@@ -1120,6 +1125,7 @@ int vhm_set_timing(vhm_ctx* c, int enabled) {
   hipStreamSynchronize(c->stream);
   drainEvents(c);
   c->timing = enabled != 0;
+  c->timed_stages = (enabled & VHM_TIMING_STAGES) ? (static_cast<uint32_t>(enabled) & 0xFu) : 0xFu;
   for (int i = 0; i < 4; ++i) {
     c->ms[i] = 0.0;
     c->launches[i] = 0;
