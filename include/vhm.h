@@ -286,7 +286,8 @@ int vhm_denoiser_forward(vhm_denoiser* d, size_t offset, size_t n, float* y_out)
  * reduction, Adam with lr; lr <= 0 leaves the parameters unchanged).
  * loss_out and grad_out (43,270 floats, host) optional. */
 int vhm_denoiser_train_step(vhm_denoiser* d, size_t offset, size_t n, float lr, float* loss_out, float* grad_out);
-/* Writes the last predictions of samples [offset, offset+n) into vol.  A cell
+/* Writes the predictions of samples [offset, offset+n) made by the last
+ * vhm_denoiser_forward over them (training steps keep none) into vol.  A cell
  * (voxel, direction) is replaced only if its input weight >= min_weight and
  * |prediction - input| <= max_delta (tsdf units); a block whose mean
  * |prediction - input| exceeds block_max_delta keeps plain fusion. */

@@ -252,6 +252,17 @@ __device__ void storeHidden(const v4f (&acc)[NT][MT], const float* bias, int nt,
   }
 }
 
+// Copies the interior of a channels-last padded grid to global [512][C]
+// with 16-byte loads and stores (8 channels of one voxel each).
+template <int C>
+__device__ void copyOutInterior(const uint16_t* grid, uint16_t* __restrict__ gout) {
+  constexpr int kChunks = 512 * C / 8;
+  for (int i = threadIdx.x; i < kChunks; i += blockDim.x) {
+    const int m = i / (C / 8), q = i % (C / 8);
+    reinterpret_cast<uint4*>(gout)[i] = *reinterpret_cast<const uint4*>(grid + gridOff<C>(interiorIndex(m), q * 8));
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- gather --
@@ -397,7 +408,7 @@ __global__ __launch_bounds__(256) void dnPackKernel(const float* __restrict__ P,
 struct DnFwdArgs {
   const uint16_t* X;
   DnWeights w;
-  float* Y;          // [n][512][6] prediction (tsdf units)
+  float* Y;          // [n][512][6] prediction (tsdf units); null in training
   // training outputs (null for inference)
   uint16_t* H1;      // [n][512][32]
   uint16_t* H2;      // [n][512][32]
@@ -445,21 +456,21 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
     convTiles32<kDnIn, 32, 4, 2>(grid, a.w.W1t, 0, wave, kFwdWaves, acc);
     __syncthreads();
     zeroHalo<kDnHid>(grid);
-    storeHidden<4, 2>(acc, a.w.b + kOffB1, 0, wave, kFwdWaves, grid,
-                      a.H1 ? a.H1 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+    storeHidden<4, 2>(acc, a.w.b + kOffB1, 0, wave, kFwdWaves, grid, nullptr);
   }
   __syncthreads();
   {  // conv2 (same on h1); h2 replaces h1 once every wavefront is done with it
+    if (a.H1) copyOutInterior<kDnHid>(grid, a.H1 + static_cast<size_t>(b) * 512 * kDnHid);
     v4f acc[2][4] = {};
     convTiles32<kDnHid, 32, 4, 2>(grid, a.w.W2t, 0, wave, kFwdWaves, acc);
     __syncthreads();
-    storeHidden<4, 2>(acc, a.w.b + kOffB2, 0, wave, kFwdWaves, grid,
-                      a.H2 ? a.H2 + static_cast<size_t>(b) * 512 * kDnHid : nullptr);
+    storeHidden<4, 2>(acc, a.w.b + kOffB2, 0, wave, kFwdWaves, grid, nullptr);
   }
   __syncthreads();
   // conv3: 1 n-tile (6 of 16 used) x 32 m-tiles
   float lsum = 0.0f;
   {
+    if (a.H2) copyOutInterior<kDnHid>(grid, a.H2 + static_cast<size_t>(b) * 512 * kDnHid);
     v4f acc3[1][4] = {};
     convTiles32<kDnHid, kDnOutPad, 4>(grid, a.w.W3t, 0, wave, kFwdWaves, acc3);
     const v4f (&acc)[4] = acc3[0];
@@ -474,7 +485,7 @@ __global__ __launch_bounds__(kFwdThreads) void dnForwardKernel(DnFwdArgs a) {
           const float x0 = bf2f(Xb[interiorIndex(m) * kDnIn + n]);
           const float y = x0 + (acc[i][r] + bb);
           const size_t o = (static_cast<size_t>(b) * 512 + m) * kDnOut + n;
-          a.Y[o] = y;
+          if (a.Y) a.Y[o] = y;
           if (a.dY) {
             const float mk = a.M[o];
             const float diff = (y - a.T[o]) * mk;

@@ -234,7 +234,8 @@ int vhm_denoiser_train_step(vhm_denoiser* d, size_t offset, size_t n, float lr, 
   for (size_t i = 0; i < n; ++i) msum += static_cast<double>(d->mcount[offset + i]);
   const float inv = 1.0f / static_cast<float>(msum > 1.0 ? msum : 1.0);
   const uint16_t* X = d->X + offset * kXElems;
-  DN_TRY(d, dnLaunchForward(X, weights(d), static_cast<uint32_t>(n), d->Y + offset * kYElems, d->H1, d->H2,
+  // the step's predictions are not kept (apply uses vhm_denoiser_forward's)
+  DN_TRY(d, dnLaunchForward(X, weights(d), static_cast<uint32_t>(n), nullptr, d->H1, d->H2,
                             d->T + offset * kYElems, d->M + offset * kYElems, d->dY, d->loss, inv, d->stream),
          "denoiser training forward launch");
   DN_TRY(d, dnLaunchBackward(X, d->H1, d->H2, d->dY, d->dA2, d->dA1, weights(d), static_cast<uint32_t>(n), d->groups,
@@ -258,7 +259,7 @@ int vhm_denoiser_train_step(vhm_denoiser* d, size_t offset, size_t n, float lr, 
 }
 
 // Writes the predictions of samples [offset, offset + n) (from the last
-// forward) back into `vol`, with the confidence fallback described in
+// vhm_denoiser_forward over them) back into `vol`, with the confidence fallback described in
 // vhm_denoise.hip (dnApplyKernel).
 int vhm_denoiser_apply(vhm_denoiser* d, vhm_ctx* vol, size_t offset, size_t n, float min_weight, float max_delta,
                        float block_max_delta, uint64_t* cells_replaced, uint64_t* blocks_kept_plain) {
