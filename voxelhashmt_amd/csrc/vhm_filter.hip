@@ -13,7 +13,7 @@
 // 2-pixel apron (36x12), so
 // every depth sample is read from HBM once per tile (plus the apron) and the
 // normals never leave the CU.  Spatial Gaussian weights are precomputed on the
-// host; the range and angular Gaussians use expf_glibc.
+// host; the range and angular Gaussians use expf_glibc (its table in LDS).
 #include "vhm_internal.h"
 
 #pragma clang fp contract(off)
@@ -53,11 +53,15 @@ __device__ void filterTile(const In* __restrict__ in, float* __restrict__ out, c
   __shared__ float sd[kDH][kDW];
   __shared__ float snx[kNH][kNW], sny[kNH][kNW], snz[kNH][kNW];
   __shared__ uint8_t sok[kNH][kNW];
+  // expf_glibc's table in LDS: a divergent 8-byte LDS read per exp instead
+  // of a load from global memory
+  __shared__ uint64_t exp_tab[32];
 
   const int W = a.K.width, H = a.K.height;
   const int tid = threadIdx.x;
   const int x0 = tile_x * kTX, y0 = tile_y * kTY;
   const int dx0 = x0 - (kR + 1), dy0 = y0 - (kR + 1);
+  if (tid < 32) exp_tab[tid] = kExp2Tab32[tid];
 
   for (int i = tid; i < kDW * kDH; i += 256) {
     const int lx = i % kDW, ly = i / kDW;
@@ -122,14 +126,14 @@ __device__ void filterTile(const In* __restrict__ in, float* __restrict__ out, c
         if (s <= 0.0f) continue;
         if (fabsf(s - c) > thr) continue;
         const float dd = s - c;
-        float w = a.spatial_w[(dv + kR) * (2 * kR + 1) + (du + kR)] * gaussianWeight(dd * dd, a.inv_r);
+        float w = a.spatial_w[(dv + kR) * (2 * kR + 1) + (du + kR)] * gaussianWeightTab(dd * dd, a.inv_r, exp_tab);
         if (has_n) {
           if (sok[cny + dv][cnx + du] == 0) continue;
           const F3 sn{snx[cny + dv][cnx + du], sny[cny + dv][cnx + du], snz[cny + dv][cnx + du]};
           const float cosv = sclamp(dot3(cn, sn), -1.0f, 1.0f);
           if (cosv < a.min_cos) continue;
           const float ad = 1.0f - cosv;
-          w *= gaussianWeight(ad * ad, a.inv_a);
+          w *= gaussianWeightTab(ad * ad, a.inv_a, exp_tab);
         }
         wd += w * s;
         ws += w;
