@@ -255,9 +255,13 @@ int checkFrame(vhm_ctx* c, const void* depth, const vhm_intrinsics* K) {
 void beginTimed(vhm_ctx* c, int kind, hipEvent_t* b) {
   *b = nullptr;
   if (!c->timing || !(c->timed_stages & (1u << kind))) return;
-  hipEvent_t a;
-  hipEventCreateWithFlags(&a, hipEventDisableSystemFence);
-  hipEventCreateWithFlags(b, hipEventDisableSystemFence);
+  hipEvent_t a = nullptr;
+  if (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess) return;  // this launch goes untimed
+  if (hipEventCreateWithFlags(b, hipEventDisableSystemFence) != hipSuccess) {
+    hipEventDestroy(a);
+    *b = nullptr;
+    return;
+  }
   hipEventRecord(a, c->stream);
   c->events.push_back({a, *b, kind});
 }
