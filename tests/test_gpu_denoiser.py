@@ -207,9 +207,10 @@ def test_trained_network_beats_plain_few_frame_fusion(gpu):
 
 
 def test_adam_kernel_matches_oracle(gpu):
-    """dnAdamKernel (FP32 master weights and moments) against the oracle's
-    adam_step (FP64; itself pinned to torch.optim.Adam) over two steps fed
-    the GPU's own gradients; the gradient reduction is deterministic."""
+    """Adam's step in dnReduceKernel (FP32 master weights and moments)
+    against the oracle's adam_step (FP64; itself pinned to
+    torch.optim.Adam) over two steps fed the GPU's own gradients; the
+    gradient reduction is deterministic."""
     w, vol = _scene_volume(seed=9, frames=4)
     keys = O.sorted_keys(vol.block_keys())
     obs = (vol.download_blocks(keys)["weight"] > 0).reshape(len(keys), -1).sum(1)
@@ -239,6 +240,30 @@ def test_adam_kernel_matches_oracle(gpu):
     print(f"[denoiser] adam max abs err vs fp64: step 1 {err1:.2e}, step 2 {err2:.2e}", flush=True)
     assert np.allclose(p1, r1, rtol=1e-6, atol=1e-8) and np.allclose(p2, r2, rtol=1e-6, atol=1e-8)
     assert np.abs(p1 - p0).max() > 5e-4  # the step moved the weights (lr 1e-3)
+    vol.close()
+
+
+def test_update_rewrites_packed_weights(gpu):
+    """The reduction's Adam step rewrites the packed bf16 weights of every
+    parameter it updates: after two training steps the network predicts
+    exactly what a fresh denoiser given the updated parameters (packed by
+    dnPackKernel) predicts, bit for bit."""
+    w, vol = _scene_volume(seed=7, frames=3)
+    keys = O.sorted_keys(vol.block_keys())
+    obs = (vol.download_blocks(keys)["weight"] > 0).reshape(len(keys), -1).sum(1)
+    pick = keys[np.argsort(-obs)[:12]]
+    p0 = D.params_to_flat(D.init_params(3)).astype(np.float32)
+    with vhm.Denoiser(64, 16) as a, vhm.Denoiser(64, 16) as b:
+        a.set_params(p0)
+        a.append(vol, pick, vol)
+        for _ in range(2):
+            a.train_step(0, len(pick), 1e-2)
+        p = a.get_params()
+        assert np.abs(p - p0).max() > 1e-3
+        b.set_params(p)
+        b.append(vol, pick, vol)
+        ya, yb = a.forward(0, len(pick)), b.forward(0, len(pick))
+    assert np.array_equal(ya.view(np.uint32), yb.view(np.uint32))
     vol.close()
 
 

@@ -976,11 +976,34 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward1Kernel(DnBwdArgs a) {
   if (threadIdx.x < kDnHid) P[kOffB1 + threadIdx.x] = db;
 }
 
+// The packed bf16 copies of parameter i (the entries dnPackKernel writes
+// for it): W1t; W2t and the flipped W2b; W3t and the flipped W3b.  Biases
+// are read from P directly and have none.
+__device__ __forceinline__ void packParam(int i, float p, const DnUpdate& u) {
+  const uint16_t h = f2bf(p);
+  if (i < kOffB1) {  // W1 (t, c < 12, n)
+    const int t = i / (12 * 32), c = (i / 32) % 12, n = i % 32;
+    u.W1t[(t * 32 + n) * 16 + c] = h;
+  } else if (i >= kOffW2 && i < kOffB2) {  // W2 (t, c, n)
+    const int j = i - kOffW2, t = j / (32 * 32), c = (j / 32) % 32, n = j % 32;
+    u.W2t[wOff<32, 32>(t, n, c)] = h;
+    u.W2b[wOff<32, 32>(26 - t, c, n)] = h;
+  } else if (i >= kOffW3 && i < kOffB3) {  // W3 (t, c, n < 6)
+    const int j = i - kOffW3, t = j / (32 * kDnOut), c = (j / kDnOut) % 32, n = j % kDnOut;
+    u.W3t[wOff<32, 16>(t, n, c)] = h;
+    u.W3b[((26 - t) * 32 + c) * kDnOutPad8 + n] = h;
+  }
+}
+
 // grad[i] = sum over workgroups g (in order) of partial[g][i]: 4 slices per
 // parameter (slice s takes rows s, s + 4, ...), combined in slice order.
+// With an update, the same lane then takes Adam's step (Kingma & Ba, bias
+// corrected, FP32 master parameters and moments) on parameter i and rewrites
+// its packed copies, so a training step needs no separate update and pack
+// launches.
 constexpr int kReduceSlices = 4;
 __global__ __launch_bounds__(256) void dnReduceKernel(const float* __restrict__ partial, int groups,
-                                                      float* __restrict__ grad) {
+                                                      float* __restrict__ grad, DnUpdate u, int update) {
   __shared__ float part[kReduceSlices][64];
   const int pi = threadIdx.x & 63, sl = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + pi;
@@ -992,21 +1015,17 @@ __global__ __launch_bounds__(256) void dnReduceKernel(const float* __restrict__ 
     for (int g = sl; g < groups; g += kReduceSlices) s += partial[static_cast<size_t>(g) * kNumParams + i];
   part[sl][pi] = s;
   __syncthreads();
-  if (sl == 0 && i < kNumParams) grad[i] = (part[0][pi] + part[1][pi]) + (part[2][pi] + part[3][pi]);
-}
-
-// Adam (Kingma & Ba) with bias correction; fp32 master parameters.
-__global__ __launch_bounds__(256) void dnAdamKernel(float* __restrict__ P, const float* __restrict__ G,
-                                                    float* __restrict__ Mm, float* __restrict__ Vv, float lr, float b1,
-                                                    float b2, float eps, float c1, float c2) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= kNumParams) return;
-  const float g = G[i];
-  const float m = b1 * Mm[i] + (1.0f - b1) * g;
-  const float v = b2 * Vv[i] + (1.0f - b2) * g * g;
-  Mm[i] = m;
-  Vv[i] = v;
-  P[i] = P[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+  if (sl != 0 || i >= kNumParams) return;
+  const float g = (part[0][pi] + part[1][pi]) + (part[2][pi] + part[3][pi]);
+  grad[i] = g;
+  if (!update) return;
+  const float m = u.b1 * u.Mm[i] + (1.0f - u.b1) * g;
+  const float v = u.b2 * u.Vv[i] + (1.0f - u.b2) * g * g;
+  u.Mm[i] = m;
+  u.Vv[i] = v;
+  const float p = u.P[i] - u.lr * (m / u.c1) / (sqrtf(v / u.c2) + u.eps);
+  u.P[i] = p;
+  packParam(i, p, u);
 }
 
 // Write-back with confidence fallback.  For each voxel and direction whose
@@ -1146,7 +1165,7 @@ hipError_t dnLaunchForward(const uint16_t* X, const DnWeights& w, uint32_t n, fl
 }
 hipError_t dnLaunchBackward(const uint16_t* X, const uint16_t* H1, const uint16_t* H2, const float* dY,
                             uint16_t* dA2, uint16_t* dA1, const DnWeights& w, uint32_t n, int groups, float* partial,
-                            float* grad, hipStream_t s) {
+                            float* grad, const DnUpdate* update, int step, hipStream_t s) {
   if (n == 0) return hipSuccess;
   DnBwdArgs a{static_cast<int>(n), X, H1, H2, dY, dA2, dA1, w, partial};
   // each layer kernel writes every partial entry of its layer, so the
@@ -1154,13 +1173,13 @@ hipError_t dnLaunchBackward(const uint16_t* X, const uint16_t* H1, const uint16_
   dnBackward3Kernel<<<groups, kBwdThreads, 0, s>>>(a);
   dnBackward2Kernel<<<groups, kBwdThreads, 0, s>>>(a);
   dnBackward1Kernel<<<groups, kBwdThreads, 0, s>>>(a);
-  dnReduceKernel<<<(kNumParams + 63) / 64, 256, 0, s>>>(partial, groups, grad);
-  return hipGetLastError();
-}
-hipError_t dnLaunchAdam(float* P, const float* G, float* M, float* V, float lr, float b1, float b2, float eps,
-                        int step, hipStream_t s) {
-  const float c1 = 1.0f - powf(b1, static_cast<float>(step)), c2 = 1.0f - powf(b2, static_cast<float>(step));
-  dnAdamKernel<<<(kNumParams + 255) / 256, 256, 0, s>>>(P, G, M, V, lr, b1, b2, eps, c1, c2);
+  DnUpdate u{};
+  if (update) {
+    u = *update;
+    u.c1 = 1.0f - powf(u.b1, static_cast<float>(step));
+    u.c2 = 1.0f - powf(u.b2, static_cast<float>(step));
+  }
+  dnReduceKernel<<<(kNumParams + 63) / 64, 256, 0, s>>>(partial, groups, grad, u, update ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t dnLaunchApply(const VolumeView& V, const int32_t* keys, uint32_t n, const float* Y, const uint16_t* X,

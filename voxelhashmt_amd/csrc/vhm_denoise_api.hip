@@ -238,15 +238,14 @@ int vhm_denoiser_train_step(vhm_denoiser* d, size_t offset, size_t n, float lr, 
   DN_TRY(d, dnLaunchForward(X, weights(d), static_cast<uint32_t>(n), nullptr, d->H1, d->H2,
                             d->T + offset * kYElems, d->M + offset * kYElems, d->dY, d->loss, inv, d->stream),
          "denoiser training forward launch");
+  // backward, gradient reduction and (lr > 0) Adam's step with the packed
+  // weights rewritten, all on the stream
+  const bool update = lr > 0.0f;
+  if (update) ++d->step;
+  const DnUpdate up{d->P, d->Mm, d->Vv, d->W1t, d->W2t, d->W3t, d->W2b, d->W3b, lr, 0.9f, 0.999f, 1e-8f, 0.0f, 0.0f};
   DN_TRY(d, dnLaunchBackward(X, d->H1, d->H2, d->dY, d->dA2, d->dA1, weights(d), static_cast<uint32_t>(n), d->groups,
-                             d->partial, d->G, d->stream),
+                             d->partial, d->G, update ? &up : nullptr, d->step, d->stream),
          "denoiser backward launch");
-  if (lr > 0.0f) {
-    ++d->step;
-    DN_TRY(d, dnLaunchAdam(d->P, d->G, d->Mm, d->Vv, lr, 0.9f, 0.999f, 1e-8f, d->step, d->stream), "adam launch");
-    int rc = repack(d);
-    if (rc) return rc;
-  }
   std::vector<float> lp(n);
   DN_TRY(d, hipMemcpyAsync(lp.data(), d->loss, n * 4, hipMemcpyDeviceToHost, d->stream), "hipMemcpy loss");
   if (grad_out)

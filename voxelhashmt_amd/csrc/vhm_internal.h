@@ -331,11 +331,22 @@ hipError_t dnLaunchPack(const float* P, uint16_t* W1t, uint16_t* W2t, uint16_t* 
                         hipStream_t s);
 hipError_t dnLaunchForward(const uint16_t* X, const DnWeights& w, uint32_t n, float* Y, uint16_t* H1, uint16_t* H2,
                            const float* T, const float* M, float* dY, float* loss, float inv_count, hipStream_t s);
+// Adam's step, taken by the gradient reduction of a training step: FP32
+// master parameters P and moments Mm, Vv, and the packed bf16 copies of the
+// weights, rewritten for every updated parameter (c1, c2: the bias
+// corrections, set by the launcher from the step number).
+struct DnUpdate {
+  float* P;
+  float* Mm;
+  float* Vv;
+  uint16_t *W1t, *W2t, *W3t, *W2b, *W3b;
+  float lr, b1, b2, eps, c1, c2;
+};
+// Backward pass and gradient reduction into grad; with update (not null),
+// Adam's step number `step` as well.
 hipError_t dnLaunchBackward(const uint16_t* X, const uint16_t* H1, const uint16_t* H2, const float* dY,
                             uint16_t* dA2, uint16_t* dA1, const DnWeights& w, uint32_t n, int groups, float* partial,
-                            float* grad, hipStream_t s);
-hipError_t dnLaunchAdam(float* P, const float* G, float* M, float* V, float lr, float b1, float b2, float eps,
-                        int step, hipStream_t s);
+                            float* grad, const DnUpdate* update, int step, hipStream_t s);
 hipError_t dnLaunchApply(const VolumeView& V, const int32_t* keys, uint32_t n, const float* Y, const uint16_t* X,
                          float voxel, float trunc, float max_weight, float min_w, float max_delta,
                          float block_max_delta, unsigned long long* counters, hipStream_t s);
