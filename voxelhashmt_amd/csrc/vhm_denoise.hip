@@ -914,10 +914,25 @@ __global__ __launch_bounds__(kBwdThreads) void dnBackward2Kernel(DnBwdArgs a) {
   float db = 0.0f;
   zeroHaloT<kDnHid>(h1T);
   zeroHalo<kDnHid>(dap);
+  // a block's h1 is fetched into registers one block ahead, so its memory
+  // latency overlaps the previous block's products
+  constexpr int kHPer = 512 * kDnHid / 8 / kBwdThreads;  // 4
+  uint4 hq[kHPer];
+  auto fetchH = [&](int b) {
+    const uint4* hs = reinterpret_cast<const uint4*>(a.H1 + static_cast<size_t>(b) * 512 * kDnHid);
+#pragma unroll
+    for (int k = 0; k < kHPer; ++k) hq[k] = hs[threadIdx.x + k * kBwdThreads];
+  };
+  if (static_cast<int>(blockIdx.x) < a.n_blocks) fetchH(blockIdx.x);
   for (int b = blockIdx.x; b < a.n_blocks; b += gridDim.x) {
     __syncthreads();
-    loadInteriorT<kDnHid>(a.H1 + static_cast<size_t>(b) * 512 * kDnHid, h1T);
+#pragma unroll
+    for (int k = 0; k < kHPer; ++k) {
+      const int i = threadIdx.x + k * kBwdThreads, m = (i * 8) / kDnHid, c = (i * 8) % kDnHid;
+      scatter8(h1T + c * kGridStride + interiorIndex(m), kGridStride, hq[k]);
+    }
     loadInterior<kDnHid>(a.dA2 + static_cast<size_t>(b) * 512 * kDnHid, dap);
+    if (b + static_cast<int>(gridDim.x) < a.n_blocks) fetchH(b + gridDim.x);
     __syncthreads();
     biasGrad<kDnHid, 32>([&](int v, int ch) { return bf2f(dap[gridOff<kDnHid>(interiorIndex(v), ch)]); }, part, db);
     accumulateDWRows<kDnHid, kDnHid, true>(h1T, dap, wave, acc);
