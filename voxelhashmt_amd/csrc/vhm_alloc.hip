@@ -173,13 +173,7 @@ __global__ __launch_bounds__(256) void allocateKernel(const float* __restrict__ 
   allocTile<kTileW>(depth, V, a, blockIdx.x, blockIdx.y);
 }
 
-__global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) {
-  hdr->active_count = 0u;
-  hdr->defer_count = 0u;
-  hdr->integrate_ticket = 0u;
-  hdr->frame = frame;
-  hdr->number_base = hdr->num_blocks;
-}
+__global__ void frameBeginKernel(DevHeader* hdr, uint32_t frame) { beginFrame(hdr, frame); }
 
 __global__ void numberBeginKernel(DevHeader* hdr) { hdr->number_base = hdr->num_blocks; }
 

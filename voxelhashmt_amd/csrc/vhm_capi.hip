@@ -296,8 +296,14 @@ int stageDepth(vhm_ctx* c, const T* depth, int memkind, const vhm_intrinsics* K,
   return VHM_OK;
 }
 
-int filterStage(vhm_ctx* c, const void* dev_in, bool u16, float scale, const vhm_intrinsics* K, float* out) {
-  const FilterArgs fa = filterArgs(c, K, scale);
+// begin_frame: the launch also opens frame c->frame (the integration path)
+int filterStage(vhm_ctx* c, const void* dev_in, bool u16, float scale, const vhm_intrinsics* K, float* out,
+                bool begin_frame = false) {
+  FilterArgs fa = filterArgs(c, K, scale);
+  if (begin_frame) {
+    fa.begin_hdr = c->hdr;
+    fa.begin_frame = c->frame;
+  }
   hipEvent_t ev{};
   beginTimed(c, 0, &ev);
   const hipError_t e = u16 ? launchFilterU16(static_cast<const uint16_t*>(dev_in), out, fa, c->stream)
@@ -307,9 +313,13 @@ int filterStage(vhm_ctx* c, const void* dev_in, bool u16, float scale, const vhm
   return VHM_OK;
 }
 
-int allocStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, const vhm_pose* T_wc, int stride) {
-  ++c->frame;
-  VHM_TRY(c, launchFrameBegin(c->hdr, c->frame, c->stream), "frame begin kernel launch");
+// frame_open: the filter launch already opened frame c->frame
+int allocStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, const vhm_pose* T_wc, int stride,
+               bool frame_open = false) {
+  if (!frame_open) {
+    ++c->frame;
+    VHM_TRY(c, launchFrameBegin(c->hdr, c->frame, c->stream), "frame begin kernel launch");
+  }
   AllocArgs aa{};
   aa.K = toK(K);
   aa.T_wc = toPose(T_wc);
@@ -367,13 +377,15 @@ int integrateFrame(vhm_ctx* c, const T* depth, bool u16, float scale, int memkin
   const T* dev = nullptr;
   if ((rc = stageDepth(c, depth, memkind, K, &dev))) return rc;
   const float* fdepth = nullptr;
-  if (flags & VHM_FLAG_PREFILTERED) {
+  const bool filter = !(flags & VHM_FLAG_PREFILTERED);
+  if (!filter) {
     fdepth = reinterpret_cast<const float*>(dev);
   } else {
-    if ((rc = filterStage(c, dev, u16, scale, K, c->filtered))) return rc;
+    ++c->frame;  // opened by the filter launch
+    if ((rc = filterStage(c, dev, u16, scale, K, c->filtered, true))) return rc;
     fdepth = c->filtered;
   }
-  if ((rc = allocStage(c, fdepth, K, T_wc, stride))) return rc;
+  if ((rc = allocStage(c, fdepth, K, T_wc, stride, filter))) return rc;
   return integrateStage(c, fdepth, K, T_wc);
 }
 

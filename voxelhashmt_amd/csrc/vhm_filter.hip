@@ -148,6 +148,8 @@ namespace {
 
 template <typename In>
 __global__ __launch_bounds__(256) void filterKernel(const In* __restrict__ in, float* __restrict__ out, FilterArgs a) {
+  // opening the frame here saves the frame-begin launch before allocation
+  if (a.begin_hdr && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) beginFrame(a.begin_hdr, a.begin_frame);
   filterTile<In, kDefaultTX>(in, out, a, blockIdx.x, blockIdx.y);
 }
 

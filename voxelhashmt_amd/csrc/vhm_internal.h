@@ -98,7 +98,19 @@ struct FilterArgs {
   float spatial_w[25];  // exp(-r^2 * inv_s) for the 5x5 taps, row-major dv, du
   float inv_r, inv_a;
   float disc_base, disc_rel, min_cos;
+  DevHeader* begin_hdr;  // not null: the filter launch also opens frame begin_frame (beginFrame)
+  uint32_t begin_frame;
 };
+
+// Opens a frame: the per-frame counters of the device header (one thread;
+// everything of the previous frame has finished on the stream).
+__device__ __forceinline__ void beginFrame(DevHeader* hdr, uint32_t frame) {
+  hdr->active_count = 0u;
+  hdr->defer_count = 0u;
+  hdr->integrate_ticket = 0u;
+  hdr->frame = frame;
+  hdr->number_base = hdr->num_blocks;
+}
 
 struct AllocArgs {
   Intrinsics K;
