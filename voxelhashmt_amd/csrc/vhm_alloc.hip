@@ -273,7 +273,7 @@ __global__ void batchFrameBeginKernel(const BatchScene* __restrict__ scenes, int
   if (s == 0) st->integrate_ticket = 0u;
   if (s >= S) return;
   scenes[s].V.hdr->active_count = 0u;
-  scenes[s].V.hdr->defer_count = 0u;
+  scenes[s].V.hdr->defer_count[scenes[s].ia.defer_slot] = 0u;
   scenes[s].V.hdr->integrate_ticket = 0u;
   scenes[s].V.hdr->frame = st->step + 1u;
   scenes[s].V.hdr->number_base = scenes[s].V.hdr->num_blocks;

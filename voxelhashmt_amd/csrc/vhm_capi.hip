@@ -92,6 +92,13 @@ struct vhm_ctx {
   int device = 0;
   int num_cus = 256;
   hipStream_t stream = nullptr;
+  // The deferred integration pass of frame f runs on `side`, after f's main
+  // pass (main_done), while frame f + 1's filter and allocation run on
+  // `stream`; anything else that touches the volume first makes `stream`
+  // wait for it (joinSide).
+  hipStream_t side = nullptr;
+  hipEvent_t main_done = nullptr, side_done = nullptr;
+  bool side_pending = false;
   uint32_t frame = 0;
   uint64_t frames_integrated = 0;
   // persistent volume
@@ -108,6 +115,7 @@ struct vhm_ctx {
   // per-frame buffers
   void* depth_in = nullptr;  // max W*H*4 bytes (f32 or u16 staging)
   float* filtered = nullptr;
+  float* filtered_alt = nullptr;  // the filter output of odd frames (a pending deferred pass reads the other)
   // grown on demand
   uint32_t* mesh_counts = nullptr;
   uint32_t* mesh_offsets = nullptr;
@@ -166,8 +174,8 @@ struct vhm_ctx {
     int kind;
   };
   std::vector<Ev> events;
-  double ms[4] = {0, 0, 0, 0};
-  uint64_t launches[4] = {0, 0, 0, 0};
+  double ms[5] = {0, 0, 0, 0, 0};  // [4]: deferred integration passes, reported with [2]
+  uint64_t launches[5] = {0, 0, 0, 0, 0};
   std::string err;
 };
 
@@ -188,7 +196,17 @@ int hipFail(vhm_ctx* c, hipError_t e, const char* action) {
     if (_e != hipSuccess) return hipFail(ctx, _e, action); \
   } while (0)
 
-VolumeView view(vhm_ctx* c) {
+// Orders `stream` after the pending deferred integration pass.
+void joinSide(vhm_ctx* c) {
+  if (!c->side_pending) return;
+  hipStreamWaitEvent(c->stream, c->side_done, 0);
+  c->side_pending = false;
+}
+
+// The volume's device view, for work on `stream` that may overlap a pending
+// deferred pass: the frame's allocation and numbering only create and
+// renumber new blocks, which that pass does not touch.
+VolumeView viewOverlapping(vhm_ctx* c) {
   VolumeView V;
   V.keys = c->keys;
   V.vals = c->vals;
@@ -207,6 +225,13 @@ VolumeView view(vhm_ctx* c) {
   V.defer_list = c->defer_list;
   V.sort_tmp = c->sort_tmp;
   return V;
+}
+
+// The volume's device view for any other work on `stream`, ordered after the
+// pending deferred pass.
+VolumeView view(vhm_ctx* c) {
+  joinSide(c);
+  return viewOverlapping(c);
 }
 
 Intrinsics toK(const vhm_intrinsics* K) { return {K->width, K->height, K->fx, K->fy, K->cx, K->cy}; }
@@ -252,9 +277,11 @@ int checkFrame(vhm_ctx* c, const void* depth, const vhm_intrinsics* K) {
 // Timing events only read timestamps: created without the system-scope
 // fence on record, which would write back and invalidate the caches
 // between the stages.
-void beginTimed(vhm_ctx* c, int kind, hipEvent_t* b) {
+void beginTimed(vhm_ctx* c, int kind, hipEvent_t* b, hipStream_t s = nullptr) {
   *b = nullptr;
-  if (!c->timing || !(c->timed_stages & (1u << kind))) return;
+  // kind 4 (the deferred integration pass) is timed with the integrate stage
+  if (!c->timing || !(c->timed_stages & (1u << (kind == 4 ? 2 : kind)))) return;
+  if (!s) s = c->stream;
   hipEvent_t a = nullptr;
   if (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess) return;  // this launch goes untimed
   if (hipEventCreateWithFlags(b, hipEventDisableSystemFence) != hipSuccess) {
@@ -262,12 +289,12 @@ void beginTimed(vhm_ctx* c, int kind, hipEvent_t* b) {
     *b = nullptr;
     return;
   }
-  hipEventRecord(a, c->stream);
+  hipEventRecord(a, s);
   c->events.push_back({a, *b, kind});
 }
 
-void endTimed(vhm_ctx* c, hipEvent_t b) {
-  if (c->timing && b) hipEventRecord(b, c->stream);
+void endTimed(vhm_ctx* c, hipEvent_t b, hipStream_t s = nullptr) {
+  if (c->timing && b) hipEventRecord(b, s ? s : c->stream);
 }
 
 void drainEvents(vhm_ctx* c) {
@@ -329,14 +356,17 @@ int allocStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, cons
   aa.frame = c->frame;
   hipEvent_t ev{};
   beginTimed(c, 1, &ev);
-  hipError_t e = launchAllocate(dev_depth, view(c), aa, c->stream);
-  if (e == hipSuccess) e = launchNumberBlocks(view(c), c->stream);
+  hipError_t e = launchAllocate(dev_depth, viewOverlapping(c), aa, c->stream);
+  if (e == hipSuccess) e = launchNumberBlocks(viewOverlapping(c), c->stream);
   endTimed(c, ev);
   VHM_TRY(c, e, "block allocation kernel launch");
   return VHM_OK;
 }
 
-int integrateStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, const vhm_pose* T_wc) {
+// side: the deferred pass goes to the side stream (dev_depth is one of the
+// library's own filter buffers, which the next frame does not overwrite)
+int integrateStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, const vhm_pose* T_wc,
+                   bool side = false) {
   IntegrateArgs ia{};
   ia.K = toK(K);
   ia.T_wc = toPose(T_wc);
@@ -357,11 +387,27 @@ int integrateStage(vhm_ctx* c, const float* dev_depth, const vhm_intrinsics* K, 
   ia.inv_a = invTwoSigmaSq(c->P.normal_sigma_angular);
   focalDivision(ia.K.fx, &ia.inv_fx, &ia.exact_fx);
   focalDivision(ia.K.fy, &ia.inv_fy, &ia.exact_fy);
+  ia.defer_slot = c->frame & 1u;
   hipEvent_t ev{};
   beginTimed(c, 2, &ev);
-  const hipError_t e = launchIntegrate(dev_depth, view(c), ia, c->num_cus * 8, c->stream);
+  hipError_t e = launchIntegrate(dev_depth, view(c), ia, c->num_cus * 8, c->stream);
   endTimed(c, ev);
   VHM_TRY(c, e, "integrate kernel launch");
+  hipStream_t ds = c->stream;
+  if (side) {
+    VHM_TRY(c, hipEventRecord(c->main_done, c->stream), "hipEventRecord");
+    VHM_TRY(c, hipStreamWaitEvent(c->side, c->main_done, 0), "hipStreamWaitEvent");
+    ds = c->side;
+  }
+  hipEvent_t ev2{};
+  beginTimed(c, 4, &ev2, ds);
+  e = launchIntegrateDeferred(dev_depth, viewOverlapping(c), ia, ds);
+  endTimed(c, ev2, ds);
+  VHM_TRY(c, e, "deferred integrate kernel launch");
+  if (side) {
+    VHM_TRY(c, hipEventRecord(c->side_done, c->side), "hipEventRecord");
+    c->side_pending = true;
+  }
   ++c->frames_integrated;
   return VHM_OK;
 }
@@ -382,14 +428,18 @@ int integrateFrame(vhm_ctx* c, const T* depth, bool u16, float scale, int memkin
     fdepth = reinterpret_cast<const float*>(dev);
   } else {
     ++c->frame;  // opened by the filter launch
-    if ((rc = filterStage(c, dev, u16, scale, K, c->filtered, true))) return rc;
-    fdepth = c->filtered;
+    // frames alternate filter buffers: the previous frame's deferred pass may
+    // still read the other one
+    float* fbuf = (c->frame & 1u) ? c->filtered_alt : c->filtered;
+    if ((rc = filterStage(c, dev, u16, scale, K, fbuf, true))) return rc;
+    fdepth = fbuf;
   }
   if ((rc = allocStage(c, fdepth, K, T_wc, stride, filter))) return rc;
-  return integrateStage(c, fdepth, K, T_wc);
+  return integrateStage(c, fdepth, K, T_wc, filter);
 }
 
 int readHeader(vhm_ctx* c, DevHeader* h) {
+  joinSide(c);
   VHM_TRY(c, hipMemcpyAsync(h, c->hdr, sizeof(DevHeader), hipMemcpyDeviceToHost, c->stream), "hipMemcpy D2H header");
   VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   if (c->timing) drainEvents(c);
@@ -522,6 +572,9 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   hipGetDeviceProperties(&prop, c->device);
   c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->main_done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming);
   const size_t cap = size_t(1) << p->hash_log2_buckets;
   const size_t px = size_t(p->max_width) * p->max_height;
   if (e == hipSuccess) e = hipMalloc(&c->keys, cap * sizeof(uint64_t));
@@ -536,6 +589,7 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->hdr, sizeof(DevHeader));
   if (e == hipSuccess) e = hipMalloc(&c->depth_in, px * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->filtered, px * 4u);
+  if (e == hipSuccess) e = hipMalloc(&c->filtered_alt, px * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_flags, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_pos, size_t(p->max_blocks) * 4u);
   if (e == hipSuccess) e = hipMalloc(&c->remesh_list, size_t(p->max_blocks) * 4u);
@@ -563,19 +617,23 @@ int vhm_create(const vhm_params* p, vhm_ctx** out) {
 
 int vhm_destroy(vhm_ctx* c) {
   if (!c) return VHM_OK;
+  if (c->side) hipStreamSynchronize(c->side);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (auto& ev : c->events) {
     hipEventDestroy(ev.a);
     hipEventDestroy(ev.b);
   }
   void* bufs[] = {c->keys,         c->vals,         c->stamp,         c->active, c->pool_keys, c->pool,
-                  c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->mesh_counts,
+                  c->obs,          c->dirty,  c->dirmask,        c->hdr,           c->depth_in, c->filtered, c->filtered_alt, c->mesh_counts,
                   c->mesh_offsets, c->mesh_partials, c->tris,          c->key_buf, c->gather_buf,
                   c->remesh_flags, c->remesh_pos,    c->remesh_list,   c->remesh_partials, c->tri_owner, c->cube_counts, c->remesh_dirs, c->mesh_cand, c->mesh_counters, c->mesh_nbr, c->mesh_obs_dirs,
                   c->defer_flags, c->defer_list, c->sort_tmp, c->pc_points, c->pc_info, c->pc_keys, c->pc_keys_tmp,
                   c->pc_vals, c->pc_vals_tmp, c->pc_count, c->pc_hist, c->pc_offsets, c->pc_partials};
   for (void* b : bufs)
     if (b) hipFree(b);
+  if (c->main_done) hipEventDestroy(c->main_done);
+  if (c->side_done) hipEventDestroy(c->side_done);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return VHM_OK;
@@ -585,6 +643,7 @@ const char* vhm_last_error(const vhm_ctx* c) { return c ? c->err.c_str() : "null
 
 int vhm_reset(vhm_ctx* c) {
   if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  joinSide(c);
   return initVolume(c);
 }
 
@@ -602,6 +661,7 @@ static int filterEntry(vhm_ctx* c, const void* depth, bool u16, float scale, int
                        float* out, int out_kind) {
   int rc = checkFrame(c, depth, K);
   if (rc) return rc;
+  joinSide(c);  // c->filtered may be a pending deferred pass's input
   if (!out) return fail(c, VHM_ERR_INVALID_ARGUMENT, "filter failed: null output");
   const void* dev = nullptr;
   if (u16)
@@ -634,6 +694,7 @@ int vhm_allocate_active_blocks(vhm_ctx* c, const float* depth, int memkind, cons
   int rc = checkFrame(c, depth, K);
   if (rc) return rc;
   if (!T_wc || !n_out) return fail(c, VHM_ERR_INVALID_ARGUMENT, "allocate failed: null argument");
+  joinSide(c);
   const float* dev = nullptr;
   if ((rc = stageDepth(c, depth, memkind, K, &dev))) return rc;
   if ((rc = allocStage(c, dev, K, T_wc, stride))) return rc;
@@ -659,6 +720,7 @@ constexpr size_t kPointCloudMaxRecords = size_t(1) << 25;
 
 int vhm_integrate_point_cloud(vhm_ctx* c, const float* points, size_t n, int memkind, const vhm_pose* T_wc) {
   if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  joinSide(c);
   if (!T_wc || (n && !points)) return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: null argument");
   if (memkind != VHM_HOST && memkind != VHM_DEVICE)
     return fail(c, VHM_ERR_INVALID_ARGUMENT, "point cloud integration failed: bad memkind");
@@ -1138,11 +1200,12 @@ int vhm_extract_mesh_incremental_noalloc(vhm_ctx* c, float iso, int32_t* block_k
 
 int vhm_set_timing(vhm_ctx* c, int enabled) {
   if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  joinSide(c);
   hipStreamSynchronize(c->stream);
   drainEvents(c);
   c->timing = enabled != 0;
   c->timed_stages = (enabled & VHM_TIMING_STAGES) ? (static_cast<uint32_t>(enabled) & 0xFu) : 0xFu;
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 5; ++i) {
     c->ms[i] = 0.0;
     c->launches[i] = 0;
   }
@@ -1151,10 +1214,13 @@ int vhm_set_timing(vhm_ctx* c, int enabled) {
 
 int vhm_kernel_times(vhm_ctx* c, double ms_out[4], uint64_t launches_out[4]) {
   if (!c) return VHM_ERR_INVALID_ARGUMENT;
+  joinSide(c);
   VHM_TRY(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   drainEvents(c);
   for (int i = 0; i < 4; ++i) {
-    if (ms_out) ms_out[i] = c->ms[i];
+    // the integrate stage: main and deferred passes (GPU time of both), one
+    // launch per frame
+    if (ms_out) ms_out[i] = c->ms[i] + (i == 2 ? c->ms[4] : 0.0);
     if (launches_out) launches_out[i] = c->launches[i];
   }
   return VHM_OK;
@@ -1231,7 +1297,10 @@ void vhm::focalDivision(float f, float* inv, int* exact) {
 
 // Context internals for the other host translation units (vhm_internal.h).
 vhm::VolumeView vhmCtxView(vhm_ctx* c) { return view(c); }
-hipStream_t vhmCtxStream(vhm_ctx* c) { return c->stream; }
+hipStream_t vhmCtxStream(vhm_ctx* c) {
+  joinSide(c);  // work ordered after this stream's is ordered after the deferred pass too
+  return c->stream;
+}
 const vhm_params* vhmCtxParams(vhm_ctx* c) { return &c->P; }
 void vhmCtxAdvance(vhm_ctx* c, uint32_t last_frame_id, uint64_t frames) {
   if (last_frame_id > c->frame) c->frame = last_frame_id;

@@ -188,7 +188,7 @@ __device__ __forceinline__ void enqueueBlock(IntegrateQueue& Q, const DepthView<
     if constexpr (P == Pass::kFast) {
       // list the block once (the flag is cleared by the deferred pass)
       if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
-        V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
+        V.defer_list[atomicAdd(&V.hdr->defer_count[a.defer_slot], 1u)] = p;
       pass = pass && !border;
     } else {
       pass = border;
@@ -290,7 +290,7 @@ __device__ __forceinline__ void integrateWave(WaveQueue& Q, const DepthView<floa
       const bool border = pass && !fastNormalApplies(u, v);
       // list the block once for the deferred pass (which clears the flag)
       if (__any(border) && lane == 0 && atomicOr(&V.defer_flags[p], 1u) == 0u)
-        V.defer_list[atomicAdd(&V.hdr->defer_count, 1u)] = p;
+        V.defer_list[atomicAdd(&V.hdr->defer_count[a.defer_slot], 1u)] = p;
       pass = pass && !border;
       const unsigned long long mask = __ballot(pass);
       if (pass) {
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void integrateDeferredKernel(const float* __re
   const DepthView<float> D{depth, a.K.width, a.K.height};
   __shared__ uint64_t exp_tab[32];
   const NormalParams np = normalParams(a, expTableLds(exp_tab));
-  const uint32_t n = V.hdr->defer_count;  // reset by the next frame's frameBeginKernel
+  const uint32_t n = V.hdr->defer_count[a.defer_slot];  // the next frame resets the other slot
   const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const uint32_t begin = blockIdx.x * chunk;
   const uint32_t end = begin + chunk < n ? begin + chunk : n;
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(256) void integrateBatchKernel(const BatchScene* __
 __global__ __launch_bounds__(256) void integrateDeferredBatchKernel(const BatchScene* __restrict__ scenes,
                                                                     const BatchState* __restrict__ st) {
   const BatchScene& sc = scenes[blockIdx.y];
-  const uint32_t n = sc.V.hdr->defer_count;  // reset by the next step's frame begin
+  const uint32_t n = sc.V.hdr->defer_count[sc.ia.defer_slot];  // reset by the next step's frame begin
   const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const uint32_t begin = blockIdx.x * chunk;
   const uint32_t end = begin + chunk < n ? begin + chunk : n;
@@ -445,6 +445,10 @@ __global__ __launch_bounds__(256) void integrateDeferredBatchKernel(const BatchS
 hipError_t launchIntegrate(const float* depth, const VolumeView& V, const IntegrateArgs& a, int grid,
                            hipStream_t s) {
   integrateKernel<<<grid, 256, 0, s>>>(depth, V, a);
+  return hipGetLastError();
+}
+
+hipError_t launchIntegrateDeferred(const float* depth, const VolumeView& V, const IntegrateArgs& a, hipStream_t s) {
   integrateDeferredKernel<<<kDeferredGrid, 256, 0, s>>>(depth, V, a);
   return hipGetLastError();
 }

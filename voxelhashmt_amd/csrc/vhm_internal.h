@@ -52,7 +52,8 @@ struct DevHeader {
   unsigned long long updates;   // voxel updates (weight > 0, valid direction)
   unsigned long long visited;   // voxels visited (512 per integrated block)
   unsigned long long mesh_items;  // (block, direction) items marching cubes staged (each 9^3 x 3 slots)
-  uint32_t defer_count;  // blocks appended to the integrator's deferred list this frame
+  uint32_t defer_count[2];  // blocks appended to the integrator's deferred list, per frame parity: the
+                            // deferred pass of frame f reads [f & 1] while frame f + 1 opens
   uint32_t integrate_ticket;  // integrator: next active-list entry to hand out this frame
   uint32_t number_base;       // num_blocks when the current insertion pass began
 };
@@ -106,7 +107,7 @@ struct FilterArgs {
 // everything of the previous frame has finished on the stream).
 __device__ __forceinline__ void beginFrame(DevHeader* hdr, uint32_t frame) {
   hdr->active_count = 0u;
-  hdr->defer_count = 0u;
+  hdr->defer_count[frame & 1u] = 0u;
   hdr->integrate_ticket = 0u;
   hdr->frame = frame;
   hdr->number_base = hdr->num_blocks;
@@ -152,6 +153,7 @@ struct IntegrateArgs {
   float inv_fx, inv_fy;  // RN(1 / fx), RN(1 / fy)
   int exact_fx, exact_fy;  // divByConst by fx / fy is exact (host-checked)
   float near_base;  // marching cubes' near window without its rounding term: 2 extract_tol + 1.75 voxel
+  uint32_t defer_slot;  // this frame's deferred-list count: DevHeader::defer_count[defer_slot]
 };
 
 // RN(1 / f) and whether divByConst by f equals the division for every
@@ -191,8 +193,11 @@ hipError_t launchAllocate(const float* depth, const VolumeView& V, const AllocAr
 // before anything reads their pool indices.
 hipError_t launchNumberBegin(DevHeader* hdr, hipStream_t s);
 hipError_t launchNumberBlocks(const VolumeView& V, hipStream_t s);
+// The integration pass of one frame: the main kernel, then (ordered after it
+// by the caller, on any stream) the deferred pass over the border voxels.
 hipError_t launchIntegrate(const float* depth, const VolumeView& V, const IntegrateArgs& a, int grid,
                            hipStream_t s);
+hipError_t launchIntegrateDeferred(const float* depth, const VolumeView& V, const IntegrateArgs& a, hipStream_t s);
 hipError_t launchGatherBlocksAoS(const int32_t* keys, uint32_t n, const VolumeView& V, uint32_t* out,
                                  hipStream_t s);
 // out[b]: polynomial digest (sum w_i 0x100000001B3^i mod 2^64) of the AoS
