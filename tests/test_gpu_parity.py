@@ -222,6 +222,33 @@ def test_determinism(gpu):
         assert np.asarray(a).tobytes() == np.asarray(b).tobytes()
 
 
+def test_back_to_back_frames_equal_synchronised_frames(gpu):
+    """A frame's deferred border pass runs on a second stream beside the next
+    frame's filter and allocation.  Frames from device memory integrated back
+    to back (nothing waits between them) leave the same volume, bit for bit,
+    as the same frames with a synchronisation after each, which orders every
+    pass one after another; 1280x720 frames of the plain-at-scale walk."""
+    import torch
+    w = synth.plain_scale(n_frames=8)
+    frames = torch.empty((8, w.K.height, w.K.width), dtype=torch.int16, device="cuda:0")
+    synth.render(w.scene, w.K, w.poses, w.noise, 0, frames)
+    torch.cuda.synchronize()
+    digests, keys = [], None
+    for sync in (False, True):
+        with vhm.Volume(_params(w)) as vol:
+            for i in range(8):
+                vol.integrate(frames[i], w.K, w.poses[i])
+                if sync:
+                    vol.synchronize()
+            vol.synchronize()
+            k = vol.block_keys()
+            keys = k if keys is None else keys
+            assert np.array_equal(k, keys)
+            digests.append((vol.block_digests(keys), vol.stats()["updates"], vol.observed_voxel_count()))
+            print(f"[back-to-back] sync={sync}: {len(k)} blocks, {digests[-1][1]} updates", flush=True)
+    assert np.array_equal(digests[0][0], digests[1][0]) and digests[0][1:] == digests[1][1:]
+
+
 def test_full_extraction_into_device_memory(gpu):
     """vhm_extract_mesh_to with a device buffer writes the bytes the host
     extraction returns, in the same order; a smaller device buffer receives
