@@ -248,9 +248,11 @@ int vhm_extract_mesh_incremental_noalloc(vhm_ctx* ctx, float iso_level, int32_t*
  * whose first buffer was too small; *n_out = n.  Synchronous. */
 int vhm_incremental_mesh_copy(vhm_ctx* ctx, float* tris_out, uint32_t* tri_item_out, size_t capacity, size_t* n_out);
 
-/* Per-kernel GPU time (ms, HIP events on the context stream) accumulated
- * while timing is enabled: [0] filter, [1] allocate, [2] integrate, [3] mesh;
- * launches[i] counts launches.  Enabling resets the accumulators.
+/* Per-kernel GPU time (ms, HIP events on the context's streams) accumulated
+ * while timing is enabled: [0] filter, [1] allocate, [2] integrate (the main
+ * and the deferred border pass of each frame, summed), [3] mesh;
+ * launches[i] counts launches (per frame for [2]).  Enabling resets the
+ * accumulators.
  * enabled: 0 off, 1 every stage, or VHM_TIMING_STAGES | (1 << i for each
  * stage i to time): each timed stage records two events per launch, which
  * leave a few microseconds between the launches. */
